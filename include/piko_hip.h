@@ -1,0 +1,167 @@
+/*
+ * piko_hip.h -- C-ABI of the MI355X (gfx950) Piko backend, libpiko_hip.so.
+ *
+ * Two layers, both plain C (pointers and sizes, no C++ or torch types):
+ *
+ * 1. The raster pipeline (piko_raster_*): the pipe methods pikoc generates
+ *    for RasterPipe (rasterPipelineFixPt/rasterPipe.h:16-45) --
+ *    allocate/prepare/run_single/destroy plus PikoScreen::getData -- backed
+ *    by hand-written HIP kernels instead of generated PTX.
+ *
+ * 2. The runtime shim (piko_rt_*): a 1:1 functional cover of the CUDA driver
+ *    calls the generated host code of the PTX backend makes
+ *    (pikoc/src/Backend/PTXBackend.cpp), over hipModule*, for code emitted
+ *    against a HIP target.
+ *
+ * Errors: every call returns 0 (PIKO_OK) on success and a negative
+ * piko_status otherwise; nothing aborts.  The reference prints and exits
+ * through CUDACHECK (api/include/internal/cudaMacros.h:10-20); its C++
+ * wrapper keeps that behaviour with PIKO_CHECK.
+ * Threading: one pipe may be driven by one host thread at a time; distinct
+ * pipes are independent (each owns its device memory and HIP stream).
+ */
+#ifndef PIKO_HIP_H
+#define PIKO_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  PIKO_OK = 0,
+  PIKO_ERR_INVALID = -1,     /* bad argument */
+  PIKO_ERR_HIP = -2,         /* a HIP runtime call failed */
+  PIKO_ERR_NO_DEVICE = -3,   /* no GPU (the reference silently returns,
+                                PTXBackend.cpp:685-690) */
+  PIKO_ERR_CAPACITY = -4,    /* a queue would overflow: reported, never
+                                wrapped (Bin::insert wraps, datatypes.h:88-93) */
+  PIKO_ERR_OOM = -5,         /* device allocation failed */
+  PIKO_ERR_BUSY = -6         /* admission refused (serving budget) */
+} piko_status;
+
+const char *piko_status_string(int status);
+
+/* ---- raster pipeline --------------------------------------------------- */
+
+/* Per-frame uniforms: ConstantState (api/include/piko/builtinTypes.h:20-33)
+ * reduced to the fields the raster path reads, plus the Gouraud lighting
+ * constants the reference hard-codes (main.cpp:317-328). */
+typedef struct {
+  int screenSizeX, screenSizeY;
+  float halfW, halfH;          /* 0.5f*(float)W, 0.5f*(float)H (main.cpp:104-105) */
+  float viewProjMatrix[16];    /* column-major, matmultfloat4 (math.h:365-373) */
+  float light[3];              /* unit direction towards the light */
+  float matcol[3];
+  float ambient[3];
+} piko_constant_state;
+
+enum { PIKO_INPUT_PRETRANSFORMED = 0, PIKO_INPUT_WORLD = 1 };
+
+typedef struct {
+  int bin_size;           /* raster bin edge T in pixels: 8, 16, 32 or 64 */
+  int subpixel_bits;      /* S in [1, 8] (reference: 4) */
+  int input_kind;         /* PIKO_INPUT_* */
+  long long pair_capacity;/* (triangle, bin) queue capacity; 0 = automatic,
+                             grows on demand between frames */
+} piko_raster_options;
+
+typedef struct piko_raster_pipe piko_raster_pipe;
+
+/* RasterPipe::allocate (PTXBackend.cpp:649-789 emits it): device memory for
+ * the input and every stage, input upload (PikoArray::copyData,
+ * datatypes.h:122-135, without its silent 15M truncation).
+ *   pos  : [count*9] float  -- screen xyz per vertex (PRETRANSFORMED) or
+ *                              world xyz per vertex (WORLD)
+ *   attr : [count*3] uint32 -- packed ABGR8 per vertex (PRETRANSFORMED), or
+ *          [count*9] float  -- unit normal per vertex (WORLD) */
+int piko_raster_allocate(piko_raster_pipe **pipe, const piko_constant_state *cs,
+                         const float *pos, const void *attr, int count,
+                         const piko_raster_options *opt);
+
+/* Same, with inputs already resident in device memory (not copied; must
+ * outlive the pipe). */
+int piko_raster_allocate_device(piko_raster_pipe **pipe, const piko_constant_state *cs,
+                                const float *d_pos, const void *d_attr, int count,
+                                const piko_raster_options *opt);
+
+/* RasterPipe::prepare (PTXBackend.cpp:791-809): per-frame constant state
+ * (camera).  The depth reset of resetDepthBuffer (main.cpp:402-409) and the
+ * colour clear are folded into the raster kernel. */
+int piko_raster_prepare(piko_raster_pipe *pipe, const piko_constant_state *cs);
+
+/* RasterPipe::run_single (PTXBackend.cpp:811-829): one frame, synchronous. */
+int piko_raster_run_single(piko_raster_pipe *pipe);
+
+/* nframes frames back to back on the pipe's stream, one synchronisation at
+ * the end (throughput measurement). */
+int piko_raster_run_frames(piko_raster_pipe *pipe, int nframes);
+
+/* Like run_frames, timing each kernel of the frame with HIP events on the
+ * pipe's stream.  kernel_ms[k] receives the mean duration of kernel k over
+ * the frames; names[k] its name (static strings); *nk the number of kernels
+ * (at most cap).  frame_ms receives the mean event-to-event frame time. */
+int piko_raster_profile_frames(piko_raster_pipe *pipe, int nframes, float *kernel_ms,
+                               const char **names, int cap, int *nk, float *frame_ms);
+
+/* PikoScreen::getData (stage.h:108-117): colour buffer, W*H ABGR8 words,
+ * row 0 at the bottom (builtinTypes.h:54-55). */
+int piko_raster_get_data(piko_raster_pipe *pipe, uint32_t *color);
+/* Depth buffer, W*H floats (MutableState::zBuffer, builtinTypes.h:35-40). */
+int piko_raster_get_depth(piko_raster_pipe *pipe, float *depth);
+
+/* Raster-stage bin queues of the last frame (Stage::getData, stage.h:
+ * 331-333): CSR, bin_start[nbins+1], bin_tris[pairs] in submission order.
+ * Returns the pair count (>=0) or a piko_status; writes at most cap entries. */
+long long piko_raster_get_bins(piko_raster_pipe *pipe, int64_t *bin_start, int32_t *bin_tris,
+                               long long cap);
+
+typedef struct {
+  long long triangles;       /* input count */
+  long long emitted;         /* triangles the vertex stage emitted */
+  long long pairs;           /* (triangle, bin) queue entries */
+  long long pair_capacity;
+  int bins_x, bins_y;
+  long long device_bytes;    /* device memory held by the pipe */
+} piko_raster_stats;
+
+int piko_raster_get_stats(piko_raster_pipe *pipe, piko_raster_stats *st);
+
+/* RasterPipe::destroy (PTXBackend.cpp:831-861). */
+int piko_raster_destroy(piko_raster_pipe *pipe);
+
+/* ---- runtime shim (generated-code layer) ------------------------------- */
+
+typedef struct piko_rt_module *piko_module_t;
+typedef struct piko_rt_function *piko_fn_t;
+typedef struct piko_rt_stream *piko_stream_t;   /* NULL = default stream */
+typedef struct piko_rt_event *piko_event_t;
+
+int piko_rt_init(int device);                                     /* cuInit..cuCtxCreate, PTXBackend.cpp:112-130 */
+int piko_rt_module_load(const char *path, piko_module_t *mod);    /* cuModuleLoad, PTXBackend.cpp:127 */
+int piko_rt_module_unload(piko_module_t mod);
+int piko_rt_get_function(piko_module_t mod, const char *name, piko_fn_t *fn); /* cuModuleGetFunction, :870 */
+int piko_rt_get_global(piko_module_t mod, const char *name, void **dptr, size_t *bytes); /* cuModuleGetGlobal, :244-247 */
+int piko_rt_malloc(void **dptr, size_t bytes);                    /* cuMemAlloc, :143 ff. */
+int piko_rt_free(void *dptr);                                     /* cuMemFree, :294-320 */
+int piko_rt_memcpy_htod(void *dst, const void *src, size_t bytes);/* cuMemcpyHtoD */
+int piko_rt_memcpy_dtoh(void *dst, const void *src, size_t bytes);/* cuMemcpyDtoH, :971 */
+int piko_rt_memset_d32(void *dst, uint32_t value, size_t count);
+int piko_rt_launch(piko_fn_t fn, unsigned gx, unsigned bx, unsigned shmem, piko_stream_t s,
+                   void **args);                                  /* cuLaunchKernel, :996 */
+int piko_rt_sync(piko_stream_t s);                                /* cuCtxSynchronize, :251 */
+int piko_rt_stream_create(piko_stream_t *s);
+int piko_rt_stream_destroy(piko_stream_t s);
+int piko_rt_event_create(piko_event_t *e);
+int piko_rt_event_destroy(piko_event_t e);
+int piko_rt_event_record(piko_event_t e, piko_stream_t s);
+int piko_rt_event_elapsed(piko_event_t start, piko_event_t stop, float *ms);
+int piko_rt_mem_info(size_t *free_bytes, size_t *total_bytes);
+int piko_rt_shutdown(void);                                       /* cuCtxDestroy, :840-854 */
+const char *piko_rt_error_string(int status);                     /* getCudaDrvErrorString */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,221 @@
+"""Python host mirror of Piko's RasterPipe over the MI355X C-ABI (libpiko_hip.so).
+
+The reference's user surface is the generated C++ pipe
+(pikoc/samples/rasterPipelineFixPt/rasterPipe.h:16-45):
+
+    RasterPipe pipe;
+    pipe.allocate(constState, mutableState, triangles, count);
+    pipe.prepare(); pipe.run_single();
+    unsigned* fb = pipe.pikoScreen.getData();
+    pipe.destroy();
+
+This module keeps those names and argument meanings.  Every call goes to the
+HIP library; there is no CPU path: if libpiko_hip.so is missing, or no GPU is
+present, the call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import scenes  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_lib = None
+
+PIKO_INPUT_PRETRANSFORMED = 0
+PIKO_INPUT_WORLD = 1
+
+STATUS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "no HIP device",
+          -4: "queue capacity exceeded", -5: "device out of memory", -6: "admission refused"}
+
+
+class PikoError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        super().__init__("%s: %s (%d)" % (what, STATUS.get(status, "unknown"), status))
+
+
+class ConstantState(ctypes.Structure):
+    """piko_constant_state (include/piko_hip.h); ConstantState of
+    api/include/piko/builtinTypes.h:20-33 plus the lighting constants."""
+    _fields_ = [("screenSizeX", ctypes.c_int), ("screenSizeY", ctypes.c_int),
+                ("halfW", ctypes.c_float), ("halfH", ctypes.c_float),
+                ("viewProjMatrix", ctypes.c_float * 16),
+                ("light", ctypes.c_float * 3), ("matcol", ctypes.c_float * 3),
+                ("ambient", ctypes.c_float * 3)]
+
+    @classmethod
+    def for_scene(cls, scene):
+        cs = cls()
+        cs.screenSizeX, cs.screenSizeY = scene.W, scene.H
+        # initScreen, rasterPipelineFixPt/main.cpp:104-105
+        cs.halfW = np.float32(0.5) * np.float32(scene.W)
+        cs.halfH = np.float32(0.5) * np.float32(scene.H)
+        vp = getattr(scene, "view_proj", None)
+        if vp is not None:
+            for k in range(16):
+                cs.viewProjMatrix[k] = float(vp[k])
+        for name, default in (("light", scenes.DEFAULT_LIGHT), ("matcol", scenes.DEFAULT_MATCOL),
+                              ("ambient", scenes.DEFAULT_AMBIENT)):
+            v = getattr(scene, name, None)
+            v = default if v is None else v
+            arr = getattr(cs, name)
+            for k in range(3):
+                arr[k] = float(np.float32(v[k]))
+        return cs
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("bin_size", ctypes.c_int), ("subpixel_bits", ctypes.c_int),
+                ("input_kind", ctypes.c_int), ("pair_capacity", ctypes.c_longlong)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("triangles", ctypes.c_longlong), ("emitted", ctypes.c_longlong),
+                ("pairs", ctypes.c_longlong), ("pair_capacity", ctypes.c_longlong),
+                ("bins_x", ctypes.c_int), ("bins_y", ctypes.c_int),
+                ("device_bytes", ctypes.c_longlong)]
+
+
+def lib_path():
+    return os.path.join(_HERE, "libpiko_hip.so")
+
+
+def lib():
+    """Loads libpiko_hip.so; raises if it is not built (no fallback)."""
+    global _lib
+    if _lib is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            raise RuntimeError("libpiko_hip.so is not built (run __graft_entry__.build()); "
+                               "the raster path has no CPU fallback")
+        L = ctypes.CDLL(path)
+        P, I, LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong
+        pp = ctypes.POINTER(ctypes.c_void_p)
+        L.piko_status_string.restype = ctypes.c_char_p
+        L.piko_raster_allocate.argtypes = [pp, ctypes.POINTER(ConstantState), P, P, I, ctypes.POINTER(Options)]
+        L.piko_raster_allocate_device.argtypes = [pp, ctypes.POINTER(ConstantState), P, P, I,
+                                                  ctypes.POINTER(Options)]
+        L.piko_raster_prepare.argtypes = [P, ctypes.POINTER(ConstantState)]
+        L.piko_raster_run_single.argtypes = [P]
+        L.piko_raster_run_frames.argtypes = [P, I]
+        L.piko_raster_profile_frames.argtypes = [P, I, P, P, I, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_float)]
+        L.piko_raster_get_data.argtypes = [P, P]
+        L.piko_raster_get_depth.argtypes = [P, P]
+        L.piko_raster_get_bins.argtypes = [P, P, P, LL]
+        L.piko_raster_get_bins.restype = LL
+        L.piko_raster_get_stats.argtypes = [P, ctypes.POINTER(Stats)]
+        L.piko_raster_destroy.argtypes = [P]
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise PikoError(rc, what)
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class PikoScreen:
+    """PikoScreen (api/include/piko/stage.h:53-136): the colour target."""
+
+    def __init__(self, pipe):
+        self._pipe = pipe
+
+    def getData(self):
+        """Colour buffer as uint32 ABGR8 [H, W], row 0 at the bottom (stage.h:108-117)."""
+        p = self._pipe
+        out = np.empty(p.W * p.H, np.uint32)
+        _check(lib().piko_raster_get_data(p._h, _p(out)), "getData")
+        return out.reshape(p.H, p.W)
+
+
+class RasterPipe:
+    """VertexShade -> Raster(bin T) -> depth -> Gouraud -> PikoScreen on the GPU."""
+
+    def __init__(self):
+        self._h = None
+        self.pikoScreen = PikoScreen(self)
+
+    def allocate(self, scene, bin_size=16, subpixel_bits=8, pair_capacity=0):
+        """RasterPipe::allocate (PTXBackend.cpp:649-789): uploads the input and
+        sizes every stage.  `scene` is a scenes.TriScene or scenes.WorldScene."""
+        if self._h is not None:
+            raise RuntimeError("pipe already allocated")
+        self.W, self.H, self.T, self.S = scene.W, scene.H, bin_size, subpixel_bits
+        self.kind = PIKO_INPUT_PRETRANSFORMED if scene.kind == "pretransformed" else PIKO_INPUT_WORLD
+        self.cs = ConstantState.for_scene(scene)
+        opt = Options(bin_size, subpixel_bits, self.kind, pair_capacity)
+        pos = np.ascontiguousarray(scene.pos, np.float32)
+        attr = np.ascontiguousarray(scene.col if self.kind == 0 else scene.nrm)
+        h = ctypes.c_void_p()
+        _check(lib().piko_raster_allocate(ctypes.byref(h), ctypes.byref(self.cs), _p(pos), _p(attr),
+                                          int(scene.n), ctypes.byref(opt)), "allocate")
+        self._h = h
+        return self
+
+    def prepare(self, scene=None):
+        """RasterPipe::prepare (PTXBackend.cpp:791-809): per-frame constant state."""
+        if scene is not None:
+            self.cs = ConstantState.for_scene(scene)
+        _check(lib().piko_raster_prepare(self._h, ctypes.byref(self.cs)), "prepare")
+
+    def run_single(self):
+        _check(lib().piko_raster_run_single(self._h), "run_single")
+
+    def run_frames(self, n):
+        _check(lib().piko_raster_run_frames(self._h, int(n)), "run_frames")
+
+    def profile_frames(self, n, cap=64):
+        ms = (ctypes.c_float * cap)()
+        names = (ctypes.c_char_p * cap)()
+        nk = ctypes.c_int()
+        fm = ctypes.c_float()
+        _check(lib().piko_raster_profile_frames(self._h, int(n), ctypes.cast(ms, ctypes.c_void_p),
+                                                ctypes.cast(names, ctypes.c_void_p), cap, ctypes.byref(nk),
+                                                ctypes.byref(fm)), "profile_frames")
+        return [(names[k].decode(), float(ms[k])) for k in range(nk.value)], float(fm.value)
+
+    def get_depth(self):
+        out = np.empty(self.W * self.H, np.float32)
+        _check(lib().piko_raster_get_depth(self._h, _p(out)), "get_depth")
+        return out.reshape(self.H, self.W)
+
+    def get_bins(self):
+        """Raster-stage bin queues (Stage::getData, stage.h:331-333) as CSR."""
+        nb = ((self.W + self.T - 1) // self.T) * ((self.H + self.T - 1) // self.T)
+        start = np.zeros(nb + 1, np.int64)
+        total = lib().piko_raster_get_bins(self._h, _p(start), None, 0)
+        if total < 0:
+            raise PikoError(int(total), "get_bins")
+        tris = np.zeros(max(int(total), 1), np.int32)
+        total = lib().piko_raster_get_bins(self._h, _p(start), _p(tris), int(total))
+        return start, tris[: int(total)]
+
+    def stats(self):
+        st = Stats()
+        _check(lib().piko_raster_get_stats(self._h, ctypes.byref(st)), "stats")
+        return {k: getattr(st, k) for k, _ in Stats._fields_}
+
+    def destroy(self):
+        if self._h is not None:
+            lib().piko_raster_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.destroy()
+
+
+def render(scene, bin_size=16, subpixel_bits=8):
+    """One frame; returns (color[H,W] uint32, depth[H,W] float32, pipe stats)."""
+    with RasterPipe().allocate(scene, bin_size, subpixel_bits) as pipe:
+        pipe.prepare()
+        pipe.run_single()
+        return pipe.pikoScreen.getData(), pipe.get_depth(), pipe.stats()
