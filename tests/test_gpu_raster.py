@@ -78,7 +78,7 @@ def test_large_triangles(oracle_mod):
 
 def test_ties_and_full_screen(oracle_mod):
     W, H = 130, 70
-    big = [(-10.0, -10.0), (400.0, -10.0), (-10.0, 300.0)]
+    big = [(-10.0, -10.0), (300.0, -10.0), (-10.0, 250.0)]  # inside the 225-px guard band
     pos = np.zeros((4, 9), np.float32)
     for i, z in enumerate([0.5, 0.5, 0.25, 0.25]):
         pos[i, 0::3] = [p[0] for p in big]
