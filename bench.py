@@ -58,6 +58,29 @@ def cpu_baseline(scene, T, S, budget_s=12.0):
                       "single-threaded oracle, host %s" % (len(times), T, med, cpu_model())}
 
 
+def copy_calibration(mb):
+    """A plain device-to-device copy of known size through the runtime shim:
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this run give the counters'
+    reading for a known byte count."""
+    import ctypes
+    import piko_amd
+    L = piko_amd.lib()
+    n = mb << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (a, b):
+        if L.piko_rt_malloc(ctypes.byref(ptr), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("malloc failed")
+    L.piko_rt_memset_d32.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_size_t]
+    L.piko_rt_memcpy_dtod.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    L.piko_rt_memset_d32(a, 7, n // 4)
+    for _ in range(5):
+        L.piko_rt_memcpy_dtod(b, a, n)
+    L.piko_rt_sync(None)
+    L.piko_rt_free(a)
+    L.piko_rt_free(b)
+    print(json.dumps({"copy_bytes": n, "copies": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,7 +88,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--bins", default="8,16,32,64")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_mb:
+        copy_calibration(args.copy_mb)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs a single MI355X (no multi-GPU path); "
               "use --gpus 1", file=sys.stderr)

@@ -131,6 +131,44 @@ int piko_raster_get_stats(piko_raster_pipe *pipe, piko_raster_stats *st);
 /* RasterPipe::destroy (PTXBackend.cpp:831-861). */
 int piko_raster_destroy(piko_raster_pipe *pipe);
 
+/* ---- batches (many screens in one launch sequence) --------------------- */
+
+/* One job of a batch: a screen (size, camera, lighting in `cs`) rendering
+ * `count` triangles starting at triangle `src_base` of the batch input, with
+ * raster bin size `bin_size`.  A scene of the batched configuration, or a
+ * request on a resident scene when serving. */
+typedef struct {
+  int count;
+  long long src_base;
+  int bin_size;
+  piko_constant_state cs;
+} piko_raster_job;
+
+typedef struct {
+  int input_kind;          /* PIKO_INPUT_* for every job */
+  int subpixel_bits;       /* S for every job */
+  int max_jobs;            /* job-table capacity */
+  long long max_tris;      /* initial per-batch triangle capacity (grows) */
+  long long max_pixels;    /* initial per-batch pixel capacity (grows) */
+  long long pair_capacity; /* 0 = automatic */
+} piko_batch_options;
+
+/* Creates a batch pipe over `input_tris` resident triangles (pos/attr as for
+ * piko_raster_allocate; on_device != 0: device pointers, not copied). */
+int piko_batch_create(piko_raster_pipe **pipe, const float *pos, const void *attr, long long input_tris,
+                      int on_device, const piko_batch_options *opt);
+/* Enqueues one frame of every job (asynchronous; waits for the pipe's
+ * previous batch first).  All jobs run through one kernel sequence: vertex
+ * stage, binning and sort over the batch, one raster launch per bin size. */
+int piko_batch_submit(piko_raster_pipe *pipe, const piko_raster_job *jobs, int njobs);
+/* 1 if the last batch has finished, 0 if it is still running. */
+int piko_batch_query(piko_raster_pipe *pipe);
+/* Waits for the last batch (re-running it if its queues had to grow). */
+int piko_batch_wait(piko_raster_pipe *pipe);
+/* Framebuffers of job `job` of the last batch (W*H words / floats). */
+int piko_batch_get_color(piko_raster_pipe *pipe, int job, uint32_t *color);
+int piko_batch_get_depth(piko_raster_pipe *pipe, int job, float *depth);
+
 /* ---- runtime shim (generated-code layer) ------------------------------- */
 
 typedef struct piko_rt_module *piko_module_t;
@@ -147,6 +185,7 @@ int piko_rt_malloc(void **dptr, size_t bytes);                    /* cuMemAlloc,
 int piko_rt_free(void *dptr);                                     /* cuMemFree, :294-320 */
 int piko_rt_memcpy_htod(void *dst, const void *src, size_t bytes);/* cuMemcpyHtoD */
 int piko_rt_memcpy_dtoh(void *dst, const void *src, size_t bytes);/* cuMemcpyDtoH, :971 */
+int piko_rt_memcpy_dtod(void *dst, const void *src, size_t bytes);/* device-to-device copy */
 int piko_rt_memset_d32(void *dst, uint32_t value, size_t count);
 int piko_rt_launch(piko_fn_t fn, unsigned gx, unsigned bx, unsigned shmem, piko_stream_t s,
                    void **args);                                  /* cuLaunchKernel, :996 */

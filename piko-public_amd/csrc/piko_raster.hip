@@ -29,6 +29,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "piko_hip.h"
 #include "raster_kernels.h"
@@ -75,95 +76,111 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh, ui
   return r;
 }
 
-// ---- generic exclusive scan (reduce, top, down) -------------------------
+// ---- frame state ----------------------------------------------------------
 
-__global__ void __launch_bounds__(kThreads) k_scan_reduce(const uint32_t *__restrict__ in, long long n,
-                                                           uint32_t *__restrict__ blk) {
-  __shared__ uint32_t sh[kThreads / 64 + 1];
-  const long long base = (long long)blockIdx.x * kTile + (long long)threadIdx.x * kItems;
+struct FrameState {
+  uint32_t total;     // (triangle, bin) pairs of this frame
+  uint32_t overflow;  // sticky: pairs exceeded the queue capacity
+  uint32_t pad[2];
+};
+
+// One 1024-thread block scans n values in place (exclusive) and returns the
+// total.  Element i lives at addr(i).  Each thread owns a contiguous run
+// (sum, one block-wide scan of the sums, rescan), so the block synchronises
+// once however large n is.
+template <typename Addr>
+__device__ uint32_t block_scan_inplace(uint32_t *__restrict__ v, int n, Addr addr, uint32_t *sh) {
+  const int per = (n + 1023) / 1024;
+  const int b = threadIdx.x * per, e = min(n, b + per);
   uint32_t s = 0;
-#pragma unroll
-  for (int k = 0; k < kItems; ++k)
-    if (base + k < n) s += in[base + k];
-  uint32_t total;
-  block_excl_scan<kThreads>(s, sh, total);
-  if (threadIdx.x == 0) blk[blockIdx.x] = total;
-}
-
-// One block scans the block sums in place (exclusive) and stores the total
-// at out[n].
-__global__ void __launch_bounds__(1024) k_scan_top(uint32_t *__restrict__ blk, int nblk,
-                                                   uint32_t *__restrict__ out, long long n) {
-  __shared__ uint32_t sh[1024 / 64 + 1];
-  uint32_t carry = 0;
-  for (int base = 0; base < nblk; base += 1024) {
-    const int i = base + threadIdx.x;
-    const uint32_t v = i < nblk ? blk[i] : 0u;
-    uint32_t total;
-    const uint32_t e = block_excl_scan<1024>(v, sh, total);
-    if (i < nblk) blk[i] = carry + e;
-    carry += total;
+  for (int i = b; i < e; ++i) s += v[addr(i)];
+  uint32_t tot;
+  uint32_t run = block_excl_scan<1024>(s, sh, tot);
+  for (int i = b; i < e; ++i) {
+    const size_t a = addr(i);
+    const uint32_t x = v[a];
+    v[a] = run;
+    run += x;
   }
-  if (threadIdx.x == 0) out[n] = carry;
+  return tot;
 }
 
-__global__ void __launch_bounds__(kThreads) k_scan_down(const uint32_t *__restrict__ in, long long n,
-                                                         const uint32_t *__restrict__ blk,
-                                                         uint32_t *__restrict__ out) {
-  __shared__ uint32_t sh[kThreads / 64 + 1];
-  const long long base = (long long)blockIdx.x * kTile + (long long)threadIdx.x * kItems;
-  uint32_t v[kItems];
-  uint32_t s = 0;
-#pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    v[k] = base + k < n ? in[base + k] : 0u;
-    s += v[k];
+// ---- vertex stage + bin count ---------------------------------------------
+
+constexpr int kVsThreads = 512;
+constexpr int kVsPer = 4;                      // triangles per thread
+constexpr int kVsTile = kVsThreads * kVsPer;   // triangles per block
+
+// Job of virtual triangle i: the last job whose vbase <= i (jobs sorted by
+// vbase).  `hint` is a job index known to be <= the answer.
+__device__ __forceinline__ int job_of_tri(const FrameArgs *__restrict__ jobs, int njobs, int hint, int i) {
+  int j = hint;
+  while (j + 1 < njobs && jobs[j + 1].vbase <= (uint32_t)i) ++j;
+  return j;
+}
+__device__ __forceinline__ int job_search(const FrameArgs *__restrict__ jobs, int njobs, uint32_t key,
+                                          bool by_bin) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const uint32_t v = by_bin ? jobs[mid].bin_base : jobs[mid].vbase;
+    if (v <= key) lo = mid;
+    else hi = mid - 1;
   }
-  uint32_t total;
-  uint32_t run = block_excl_scan<kThreads>(s, sh, total) + blk[blockIdx.x];
-#pragma unroll
-  for (int k = 0; k < kItems; ++k)
-    if (base + k < n) {
-      out[base + k] = run;
-      run += v[k];
-    }
+  return lo;
 }
 
-// ---- vertex stage + bin count -------------------------------------------
-
-template <int KIND>
-__global__ void __launch_bounds__(kThreads) k_vs_count(const float *__restrict__ pos,
-                                                        const void *__restrict__ attr, int n, FrameArgs fa,
-                                                        int4 *__restrict__ geo, uint4 *__restrict__ colr,
-                                                        uint32_t *__restrict__ cnt,
-                                                        uint32_t *__restrict__ keep_blk) {
-  __shared__ uint32_t sh[kThreads / 64 + 1];
-  const int i = blockIdx.x * kThreads + threadIdx.x;
-  uint32_t count = 0, keep = 0;
-  if (i < n) {
+// VertexShaderStage::process (vertexShader.pikostage:36-109) and the number of
+// bins RasterStage::assignBin gives each emitted triangle (raster.pikostage:
+// 40-117).  Triangle i of tile b is lane (i % 512) of round (i / 512) % 4.
+// MULTI: triangles belong to the jobs of a batch (per-job camera, screen,
+// bins); otherwise `fa` describes the single job.
+template <int KIND, bool MULTI>
+__global__ void __launch_bounds__(kVsThreads) k_vs_count(const float *__restrict__ pos, const void *__restrict__ attr,
+                                                          int n, FrameArgs fa, const FrameArgs *__restrict__ jobs,
+                                                          int njobs, int4 *__restrict__ geo,
+                                                          uint4 *__restrict__ colr, uint32_t *__restrict__ cnt,
+                                                          uint32_t *__restrict__ blk_pairs,
+                                                          uint32_t *__restrict__ blk_keep) {
+  __shared__ uint32_t sh[kVsThreads / 64 + 1];
+  __shared__ int s_job0;
+  const int t = threadIdx.x;
+  if (MULTI) {
+    if (t == 0) s_job0 = job_search(jobs, njobs, (uint32_t)(blockIdx.x * kVsTile), false);
+    __syncthreads();
+  }
+  uint32_t pairs = 0, keepc = 0;
+#pragma unroll
+  for (int k = 0; k < kVsPer; ++k) {
+    const int i = blockIdx.x * kVsTile + k * kVsThreads + t;
+    if (i >= n) continue;
+    const FrameArgs &ja = MULTI ? jobs[job_of_tri(jobs, njobs, s_job0, i)] : fa;
+    const int bsb = ja.S + ja.lgT;
+    const size_t src = MULTI ? (size_t)ja.src_base + (size_t)(i - (int)ja.vbase) : (size_t)i;
     int p[6];
     float z[3];
     uint32_t c[3];
-    const float *v = pos + 9 * (size_t)i;
+    const float *v = pos + 9 * src;
     if (KIND == PIKO_INPUT_PRETRANSFORMED) {
-      const uint32_t *cc = (const uint32_t *)attr + 3 * (size_t)i;
+      const uint32_t *cc = (const uint32_t *)attr + 3 * src;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        p[2 * k] = fixpt(v[3 * k], fa.S);
-        p[2 * k + 1] = fixpt(v[3 * k + 1], fa.S);
-        z[k] = v[3 * k + 2];
-        c[k] = cc[k];
+      for (int q = 0; q < 3; ++q) {
+        p[2 * q] = fixpt(v[3 * q], ja.S);
+        p[2 * q + 1] = fixpt(v[3 * q + 1], ja.S);
+        z[q] = v[3 * q + 2];
+        c[q] = cc[q];
       }
     } else {
-      const float *nn = (const float *)attr + 9 * (size_t)i;
+      const float *nn = (const float *)attr + 9 * src;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        transform_vertex(v + 3 * k, fa, p[2 * k], p[2 * k + 1], z[k]);
-        c[k] = light_vertex(nn[3 * k], nn[3 * k + 1], nn[3 * k + 2], fa);
+      for (int q = 0; q < 3; ++q) {
+        transform_vertex(v + 3 * q, ja, p[2 * q], p[2 * q + 1], z[q]);
+        c[q] = light_vertex(nn[3 * q], nn[3 * q + 1], nn[3 * q + 2], ja);
       }
     }
-    if (vs_keep(p, fa.W, fa.H, fa.S)) {
-      keep = 1;
+    uint32_t m = 0;
+    if (vs_keep(p, ja.W, ja.H, ja.S)) {
+      ++keepc;
       Edges e;
       e.set(p);
       const float rcp = 1.0f / (float)e.denom();  // raster.pikostage:174
@@ -172,98 +189,179 @@ __global__ void __launch_bounds__(kThreads) k_vs_count(const float *__restrict__
       geo[3 * (size_t)i + 1] = make_int4(p[4], p[5], __float_as_int(z[2]), __float_as_int(z0mz2));
       geo[3 * (size_t)i + 2] = make_int4(__float_as_int(z1mz2), __float_as_int(rcp), 0, 0);
       colr[i] = make_uint4(c[0], c[1], c[2], 0u);
-      const int bsb = fa.S + fa.lgT;
       int bx1, by1, bx2, by2;
-      bin_rect(p, bsb, fa.nBx, fa.nBy, bx1, by1, bx2, by2);
+      bin_rect(p, bsb, ja.nBx, ja.nBy, bx1, by1, bx2, by2);
       for (int by = by1; by <= by2; ++by)
-        for (int bx = bx1; bx <= bx2; ++bx) count += tr_accept(e, bx, by, bsb) ? 1u : 0u;
+        for (int bx = bx1; bx <= bx2; ++bx) m += tr_accept(e, bx, by, bsb) ? 1u : 0u;
     }
-    cnt[i] = count;
+    cnt[i] = m;
+    pairs += m;
   }
-  uint32_t total;
-  block_excl_scan<kThreads>(keep, sh, total);
-  if (threadIdx.x == 0) keep_blk[blockIdx.x] = total;
+  uint32_t tp, tk;
+  block_excl_scan<kVsThreads>(pairs, sh, tp);
+  block_excl_scan<kVsThreads>(keepc, sh, tk);
+  if (t == 0) {
+    blk_pairs[blockIdx.x] = tp;
+    blk_keep[blockIdx.x] = tk;
+  }
 }
 
-// (bin, triangle) pairs at the scanned offsets, walk order within a triangle.
-__global__ void __launch_bounds__(kThreads) k_emit_pairs(const int4 *__restrict__ geo,
-                                                          const uint32_t *__restrict__ off, int n, FrameArgs fa,
-                                                          long long cap, uint32_t *__restrict__ keys,
-                                                          uint32_t *__restrict__ vals,
-                                                          uint32_t *__restrict__ flags) {
-  const int i = blockIdx.x * kThreads + threadIdx.x;
-  if (i == 0 && (long long)off[n] > cap) flags[0] = 1u;  // queue overflow: reported
-  if (i >= n) return;
-  uint32_t o = off[i];
-  const uint32_t e_ = off[i + 1];
-  if (o == e_) return;
-  const int4 a = geo[3 * (size_t)i + 0], b = geo[3 * (size_t)i + 1];
-  const int p[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
-  Edges e;
-  e.set(p);
-  const int bsb = fa.S + fa.lgT;
-  int bx1, by1, bx2, by2;
-  bin_rect(p, bsb, fa.nBx, fa.nBy, bx1, by1, bx2, by2);
-  for (int by = by1; by <= by2; ++by)
-    for (int bx = bx1; bx <= bx2; ++bx)
-      if (tr_accept(e, bx, by, bsb)) {
-        if ((long long)o < cap) {
-          keys[o] = (uint32_t)(by * fa.nBx + bx);
-          vals[o] = (uint32_t)i;
+// Exclusive scan of the tiles' pair counts; total and overflow flag.
+__global__ void __launch_bounds__(1024) k_scan_tiles(uint32_t *__restrict__ blk_pairs, int ntiles,
+                                                     FrameState *__restrict__ fs, long long cap) {
+  __shared__ uint32_t sh[1024 / 64 + 1];
+  const uint32_t tot = block_scan_inplace(blk_pairs, ntiles, [](int i) { return (size_t)i; }, sh);
+  if (threadIdx.x == 0) {
+    fs->total = tot;
+    if ((long long)tot > cap) fs->overflow = 1u;
+  }
+}
+
+// (bin, triangle) pairs at their offsets: submission order across triangles,
+// walk order (rows, then columns) within one.  Bins are numbered per batch
+// (job bin_base + local bin).
+template <bool MULTI>
+__global__ void __launch_bounds__(kVsThreads) k_emit_pairs(const int4 *__restrict__ geo,
+                                                            const uint32_t *__restrict__ cnt, int n, FrameArgs fa,
+                                                            const FrameArgs *__restrict__ jobs, int njobs,
+                                                            const uint32_t *__restrict__ blk_off, long long cap,
+                                                            uint32_t *__restrict__ keys,
+                                                            uint32_t *__restrict__ vals) {
+  __shared__ uint32_t sh[kVsThreads / 64 + 1];
+  __shared__ int s_job0;
+  const int t = threadIdx.x;
+  if (MULTI) {
+    if (t == 0) s_job0 = job_search(jobs, njobs, (uint32_t)(blockIdx.x * kVsTile), false);
+    __syncthreads();
+  }
+  uint32_t c[kVsPer], off[kVsPer];
+#pragma unroll
+  for (int k = 0; k < kVsPer; ++k) {
+    const int i = blockIdx.x * kVsTile + k * kVsThreads + t;
+    c[k] = i < n ? cnt[i] : 0u;
+  }
+  uint32_t run = blk_off[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kVsPer; ++k) {
+    uint32_t tot;
+    off[k] = run + block_excl_scan<kVsThreads>(c[k], sh, tot);
+    run += tot;
+  }
+#pragma unroll
+  for (int k = 0; k < kVsPer; ++k) {
+    if (!c[k]) continue;
+    const int i = blockIdx.x * kVsTile + k * kVsThreads + t;
+    const FrameArgs &ja = MULTI ? jobs[job_of_tri(jobs, njobs, s_job0, i)] : fa;
+    const int bsb = ja.S + ja.lgT;
+    const uint32_t bbase = MULTI ? ja.bin_base : 0u;
+    const int4 a = geo[3 * (size_t)i + 0], b = geo[3 * (size_t)i + 1];
+    const int p[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
+    Edges e;
+    e.set(p);
+    int bx1, by1, bx2, by2;
+    bin_rect(p, bsb, ja.nBx, ja.nBy, bx1, by1, bx2, by2);
+    uint32_t o = off[k];
+    for (int by = by1; by <= by2; ++by)
+      for (int bx = bx1; bx <= bx2; ++bx)
+        if (tr_accept(e, bx, by, bsb)) {
+          if ((long long)o < cap) {
+            keys[o] = bbase + (uint32_t)(by * ja.nBx + bx);
+            vals[o] = (uint32_t)i;
+          }
+          ++o;
         }
-        ++o;
-      }
+  }
 }
 
-// ---- stable LSD radix sort of the pairs by bin ---------------------------
+// ---- stable LSD radix sort of the pairs by bin ----------------------------
+//
+// Per 8-bit digit: k_sort_count writes each tile's digit histogram into a
+// digit-major matrix, k_sort_rows scans each digit row (one block per digit),
+// k_sort_scatter ranks its tile stably (wave ballots + per-wave digit counts
+// in LDS), stages it in LDS in digit order and writes it out coalesced.  The
+// bins' queues end up in submission order; no global atomics touch a bin.
 
-__global__ void __launch_bounds__(kThreads) k_radix_hist(const uint32_t *__restrict__ keys,
-                                                          const uint32_t *__restrict__ total, long long cap,
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kThreads * kSortItems;  // keys per block
+
+__global__ void __launch_bounds__(kThreads) k_sort_count(const uint32_t *__restrict__ keys,
+                                                          const FrameState *__restrict__ fs, long long cap,
                                                           uint32_t *__restrict__ hist, int nblk, int shift) {
   __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
+  const int t = threadIdx.x;
+  h[t] = 0;
   __syncthreads();
-  const long long P = min((long long)*total, cap);
-  const long long base = (long long)blockIdx.x * kTile;
+  const long long P = min((long long)fs->total, cap);
+  const long long base = (long long)blockIdx.x * kSortTile;
+  if (base < P) {
 #pragma unroll 4
-  for (int r = 0; r < kItems; ++r) {
-    const long long idx = base + r * kThreads + threadIdx.x;
-    if (idx < P) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+    for (int r = 0; r < kSortItems; ++r) {
+      const long long idx = base + r * kThreads + t;
+      if (idx < P) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+    }
   }
   __syncthreads();
-  hist[(size_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+  hist[(size_t)t * nblk + blockIdx.x] = h[t];
 }
 
-// Stable scatter: rounds of 256 consecutive keys; within a wave the rank of a
-// key among equal digits comes from 9 ballots (8 digit bits + validity),
-// across the block's 4 waves from per-wave digit counts in LDS.
-__global__ void __launch_bounds__(kThreads) k_radix_scatter(const uint32_t *__restrict__ kin,
-                                                             const uint32_t *__restrict__ vin,
-                                                             uint32_t *__restrict__ kout,
-                                                             uint32_t *__restrict__ vout,
-                                                             const uint32_t *__restrict__ total, long long cap,
-                                                             const uint32_t *__restrict__ hist, int nblk,
-                                                             int shift) {
+// Exclusive scan of each digit's row of the matrix (tiles at or beyond the
+// pair total are empty) and the row totals; block d handles digit d.
+__global__ void __launch_bounds__(kThreads) k_sort_rows(uint32_t *__restrict__ hist, const FrameState *__restrict__ fs,
+                                                         long long cap, int nblk, uint32_t *__restrict__ rowtot) {
+  __shared__ uint32_t sh[kThreads / 64 + 1];
+  const long long P = min((long long)fs->total, cap);
+  const int used = (int)((P + kSortTile - 1) / kSortTile);
+  uint32_t *row = hist + (size_t)blockIdx.x * nblk;
+  uint32_t carry = 0;
+  for (int base = 0; base < used; base += kThreads) {
+    const int i = base + threadIdx.x;
+    const uint32_t x = i < used ? row[i] : 0u;
+    uint32_t tot;
+    const uint32_t e = block_excl_scan<kThreads>(x, sh, tot);
+    if (i < used) row[i] = carry + e;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) rowtot[blockIdx.x] = carry;
+}
+
+__global__ void __launch_bounds__(kThreads) k_sort_scatter(const uint32_t *__restrict__ kin,
+                                                            const uint32_t *__restrict__ vin,
+                                                            uint32_t *__restrict__ kout,
+                                                            uint32_t *__restrict__ vout,
+                                                            const FrameState *__restrict__ fs, long long cap,
+                                                            const uint32_t *__restrict__ hist,
+                                                            const uint32_t *__restrict__ rowtot, int nblk, int shift) {
   __shared__ uint32_t run[256];
   __shared__ uint32_t wcnt[kThreads / 64][256];
+  __shared__ uint32_t s_gb[256];
+  __shared__ uint32_t s_loff[256];
+  __shared__ uint32_t s_k[kSortTile], s_v[kSortTile];
+  __shared__ uint32_t sh[kThreads / 64 + 1];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const long long P = min((long long)*total, cap);
-  const long long base = (long long)blockIdx.x * kTile;
+  const long long P = min((long long)fs->total, cap);
+  const long long base = (long long)blockIdx.x * kSortTile;
   if (base >= P) return;
-  run[t] = hist[(size_t)t * nblk + blockIdx.x];
+  run[t] = 0;
+  {
+    uint32_t tot;  // digit base = totals of the smaller digits + this tile's row prefix
+    s_gb[t] = block_excl_scan<kThreads>(rowtot[t], sh, tot) + hist[(size_t)t * nblk + blockIdx.x];
+  }
   const unsigned long long lt = (1ull << lane) - 1ull;
-  for (int r = 0; r < kItems; ++r) {
+  uint32_t mk[kSortItems], mv[kSortItems], mr[kSortItems];
 #pragma unroll
-    for (int k = 0; k < kThreads / 64; ++k) wcnt[k][t] = 0;
-    __syncthreads();
+  for (int r = 0; r < kSortItems; ++r) {
     const long long idx = base + r * kThreads + t;
     const bool valid = idx < P;
-    uint32_t key = 0, val = 0;
-    if (valid) {
-      key = kin[idx];
-      val = vin[idx];
-    }
-    const uint32_t d = valid ? ((key >> shift) & 255u) : 256u;
+    mk[r] = valid ? kin[idx] : 0u;
+    mv[r] = valid ? vin[idx] : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+#pragma unroll
+    for (int q = 0; q < kThreads / 64; ++q) wcnt[q][t] = 0;
+    __syncthreads();
+    const bool valid = base + r * kThreads + t < P;
+    const uint32_t d = valid ? ((mk[r] >> shift) & 255u) : 256u;
     unsigned long long peers = ~0ull;
 #pragma unroll
     for (int b = 0; b < 9; ++b) {
@@ -276,159 +374,328 @@ __global__ void __launch_bounds__(kThreads) k_radix_scatter(const uint32_t *__re
     __syncthreads();
     if (valid) {
       uint32_t o = run[d] + rank;
-      for (int k = 0; k < w; ++k) o += wcnt[k][d];
-      kout[o] = key;
-      vout[o] = val;
+      for (int q = 0; q < w; ++q) o += wcnt[q][d];
+      mr[r] = o;
+    } else {
+      mr[r] = 0xFFFFFFFFu;
     }
     __syncthreads();
     uint32_t add = 0;
 #pragma unroll
-    for (int k = 0; k < kThreads / 64; ++k) add += wcnt[k][t];
+    for (int q = 0; q < kThreads / 64; ++q) add += wcnt[q][t];
     run[t] += add;
+  }
+  __syncthreads();
+  uint32_t tot;
+  s_loff[t] = block_excl_scan<kThreads>(run[t], sh, tot);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    if (mr[r] == 0xFFFFFFFFu) continue;
+    const uint32_t lp = s_loff[(mk[r] >> shift) & 255u] + mr[r];
+    s_k[lp] = mk[r];
+    s_v[lp] = mv[r];
+  }
+  __syncthreads();
+  const int ntile = (int)min((long long)kSortTile, P - base);
+  for (int i = t; i < ntile; i += kThreads) {
+    const uint32_t k = s_k[i];
+    const uint32_t d = (k >> shift) & 255u;
+    const uint32_t g = s_gb[d] + ((uint32_t)i - s_loff[d]);
+    kout[g] = k;
+    vout[g] = s_v[i];
   }
 }
 
-// bin_start[b] = first queue index with key >= b; bin_start[nbins] = P.
-__global__ void __launch_bounds__(kThreads) k_bin_bounds(const uint32_t *__restrict__ keys,
-                                                          const uint32_t *__restrict__ total, long long cap,
-                                                          uint32_t *__restrict__ start, int nbins) {
-  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
-  const long long P = min((long long)*total, cap);
-  if (P == 0) {
-    if (i <= nbins) start[i] = 0;
-    return;
+// Queue bounds from the sorted keys: bstart[b] / bend[b] for every non-empty
+// bin (empty bins keep 0/0; the raster kernel clears each bin's pair after use).
+__global__ void __launch_bounds__(kThreads) k_bounds(const uint32_t *__restrict__ keys,
+                                                      const FrameState *__restrict__ fs, long long cap,
+                                                      uint32_t *__restrict__ bstart, uint32_t *__restrict__ bend) {
+  const long long P = min((long long)fs->total, cap);
+  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < P; i += (long long)gridDim.x * kThreads) {
+    const uint32_t k = keys[i];
+    if (i == 0 || keys[i - 1] != k) bstart[k] = (uint32_t)i;
+    if (i == P - 1 || keys[i + 1] != k) bend[k] = (uint32_t)(i + 1);
   }
-  if (i >= P) return;
-  const int k = (int)keys[i];
-  const int kp = i == 0 ? -1 : (int)keys[i - 1];
-  for (int b = kp + 1; b <= k; ++b) start[b] = (uint32_t)i;
-  if (i == P - 1)
-    for (int b = k + 1; b <= nbins; ++b) start[b] = (uint32_t)P;
 }
 
 // ---- tile raster ----------------------------------------------------------
+//
+// One wave per bin (bins laid out so that the blocks an XCD receives cover a
+// contiguous band of bins).  The bin's queue is processed in batches of 64
+// triangles staged in LDS.  Every (triangle, pixel) candidate -- the pixels
+// of the triangle's rounded bbox inside the bin -- is spread evenly over the
+// lanes (exclusive scan of per-triangle candidate counts + binary search);
+// covered candidates min-reduce a visibility key (z bits, 0xFFFFFFFE - queue
+// position) into LDS.  Queue position orders like submission order because
+// the queues are stably sorted.  The winner of each pixel is then shaded from
+// the staged batch (single-batch bins, the common case) or from global memory.
 
-template <int T, int NT>
-__global__ void __launch_bounds__(NT) k_raster(const int4 *__restrict__ geo, const uint4 *__restrict__ colr,
-                                               const uint32_t *__restrict__ bin_start,
-                                               const uint32_t *__restrict__ bin_tris, FrameArgs fa,
-                                               uint32_t *__restrict__ color, float *__restrict__ depth) {
+constexpr int kRT = 64;  // raster lanes per bin
+
+// fromABGR's byte/255.0f (math.h:323-330) for every byte, folded at compile
+// time: constant evaluation rounds the division exactly like IEEE runtime
+// division, so the table holds the same floats the reference computes.
+struct U8Table {
+  float v[256];
+  constexpr U8Table() : v() {
+    for (int i = 0; i < 256; ++i) v[i] = (float)i / 255.0f;
+  }
+};
+__constant__ U8Table kU8Table = U8Table();
+
+struct RasterBatch {
+  uint32_t coff[kRT + 1];
+  uint32_t rect[kRT];  // x | y << 12 | w << 24
+  int xy[6][kRT];
+  float zr[4][kRT];    // z2, z0mz2, z1mz2, onebybary
+  uint32_t col[3][kRT];
+  uint32_t small[kRT];
+};
+
+// Edge functions at the centre of (px, py); int32 when the triangle's rounded
+// bbox is below 2^(30-2S) pixels (then |e| < 2^31 everywhere inside it, so the
+// values equal the int64 ones), else int64.
+__device__ __forceinline__ bool covered_at(const int *xy, bool small, int px, int py, int S, float rcp, float &alpha,
+                                           float &beta) {
+  const int half = 1 << (S - 1);
+  const int cx = (px << S) + half, cy = (py << S) + half;
+  if (small) {
+    const int e0 = (cy - xy[1]) * (xy[2] - xy[0]) - (cx - xy[0]) * (xy[3] - xy[1]);
+    const int e1 = (cy - xy[3]) * (xy[4] - xy[2]) - (cx - xy[2]) * (xy[5] - xy[3]);
+    const int e2 = (cy - xy[5]) * (xy[0] - xy[4]) - (cx - xy[4]) * (xy[1] - xy[5]);
+    if ((e0 | e1 | e2) < 0) return false;
+    alpha = (float)e1 * rcp;
+    beta = (float)e2 * rcp;
+    return true;
+  }
+  Edges e;
+  e.set(xy);
+  long long e0, e1, e2;
+  pixel_edges(e, px, py, S, e0, e1, e2);
+  if ((e0 | e1 | e2) < 0) return false;
+  alpha = (float)e1 * rcp;  // raster.pikostage:269-270
+  beta = (float)e2 * rcp;
+  return true;
+}
+
+__device__ __forceinline__ bool small_bbox(const int *p, int S) {
+  const int lox = (imin3(p[0], p[2], p[4]) & (int)0xffffff00) >> S;
+  const int loy = (imin3(p[1], p[3], p[5]) & (int)0xffffff00) >> S;
+  const int hix = ((imax3(p[0], p[2], p[4]) + 0xff) & (int)0xffffff00) >> S;
+  const int hiy = ((imax3(p[1], p[3], p[5]) + 0xff) & (int)0xffffff00) >> S;
+  return (long long)(hix - lox) * (long long)(hiy - loy) < (1ll << (30 - 2 * S));
+}
+
+__device__ __forceinline__ uint32_t gouraud(const float *u8, uint32_t c0, uint32_t c1, uint32_t c2, float alpha,
+                                            float beta) {
+  float rgb[3];
+#pragma unroll
+  for (int ch = 0; ch < 3; ++ch) {  // raster.pikostage:183-185, 254-255, 291-293
+    const float v0 = u8[(c0 >> (8 * ch)) & 255u];
+    const float v1 = u8[(c1 >> (8 * ch)) & 255u];
+    const float v2 = u8[(c2 >> (8 * ch)) & 255u];
+    rgb[ch] = interp(v0 - v2, v1 - v2, v2, alpha, beta);
+  }
+  return to_abgr(rgb[0], rgb[1], rgb[2]);
+}
+
+// Setup record of one triangle as the raster kernel consumes it.
+struct TriRec {
+  int4 a, b, c;  // x0 y0 x1 y1 | x2 y2 z2 z0mz2 | z1mz2 rcp - -
+  uint4 col;     // c0 c1 c2 -
+};
+
+__device__ __forceinline__ TriRec load_tri(const int4 *__restrict__ geo, const uint4 *__restrict__ colr,
+                                           uint32_t tri) {
+  TriRec r;
+  r.a = geo[3 * (size_t)tri + 0];
+  r.b = geo[3 * (size_t)tri + 1];
+  r.c = geo[3 * (size_t)tri + 2];
+  r.col = colr[tri];
+  return r;
+}
+
+// Stages lane t's triangle (if `valid`) into the batch arrays and returns its
+// candidate count: pixels of its rounded bbox (rasterMacros.h:58-63) inside
+// [xlo,xhi) x [ylo,yhi).
+__device__ __forceinline__ uint32_t stage_tri(RasterBatch &bt, int t, bool valid, const TriRec &r, int S, int xlo,
+                                              int xhi, int ylo, int yhi) {
+  if (!valid) return 0;
+  const int p[6] = {r.a.x, r.a.y, r.a.z, r.a.w, r.b.x, r.b.y};
+  const int lox = (imin3(p[0], p[2], p[4]) & (int)0xffffff00) >> S;
+  const int loy = (imin3(p[1], p[3], p[5]) & (int)0xffffff00) >> S;
+  const int hix = ((imax3(p[0], p[2], p[4]) + 0xff) & (int)0xffffff00) >> S;
+  const int hiy = ((imax3(p[1], p[3], p[5]) + 0xff) & (int)0xffffff00) >> S;
+  const int x0 = max(lox, xlo), x1 = min(hix, xhi);
+  const int y0 = max(loy, ylo), y1 = min(hiy, yhi);
+  const int w = max(0, x1 - x0), h = max(0, y1 - y0);
+  bt.rect[t] = (uint32_t)(x0 & 0xfff) | ((uint32_t)(y0 & 0xfff) << 12) | ((uint32_t)w << 24);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) bt.xy[k][t] = p[k];
+  bt.zr[0][t] = __int_as_float(r.b.z);
+  bt.zr[1][t] = __int_as_float(r.b.w);
+  bt.zr[2][t] = __int_as_float(r.c.x);
+  bt.zr[3][t] = __int_as_float(r.c.y);
+  bt.col[0][t] = r.col.x;
+  bt.col[1][t] = r.col.y;
+  bt.col[2][t] = r.col.z;
+  bt.small[t] = (long long)(hix - lox) * (long long)(hiy - loy) < (1ll << (30 - 2 * S));
+  return (uint32_t)(w * h);
+}
+
+// Candidates of the staged batch: covered ones min-reduce their key into vis.
+template <int T>
+__device__ __forceinline__ void raster_batch(RasterBatch &bt, unsigned long long *vis, uint32_t nc, int t, int S,
+                                             int xlo, int ylo, uint32_t pos0) {
+  const uint32_t inc = wave_incl_scan(nc);
+  bt.coff[t] = inc - nc;
+  const uint32_t tot = __shfl(inc, 63, 64);
+  __syncthreads();
+  for (uint32_t cidx = t; cidx < tot; cidx += kRT) {
+    int lo = 0, hi = kRT - 1;  // owner: largest o with coff[o] <= cidx
+#pragma unroll
+    for (int step = 0; step < 6; ++step) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (bt.coff[mid] <= cidx) lo = mid;
+      else hi = mid - 1;
+    }
+    const int o = lo;
+    const uint32_t k = cidx - bt.coff[o];
+    const uint32_t rect = bt.rect[o];
+    const int w = (int)(rect >> 24);
+    const int q = (int)(((float)k + 0.5f) * __builtin_amdgcn_rcpf((float)w));
+    const int r = (int)k - q * w;
+    const int px = (int)(rect & 0xfff) + r, py = (int)((rect >> 12) & 0xfff) + q;
+    const int xy[6] = {bt.xy[0][o], bt.xy[1][o], bt.xy[2][o], bt.xy[3][o], bt.xy[4][o], bt.xy[5][o]};
+    float alpha, beta;
+    if (covered_at(xy, bt.small[o] != 0, px, py, S, bt.zr[3][o], alpha, beta)) {
+      const float z = interp(bt.zr[1][o], bt.zr[2][o], bt.zr[0][o], alpha, beta);  // raster.pikostage:272
+      atomicMin((long long *)&vis[(py - ylo) * T + (px - xlo)], (long long)vis_key(z, pos0 + (uint32_t)o));
+    }
+  }
+  __syncthreads();
+}
+
+// Software pipeline per wave: while bin j is rasterised, the setup records of
+// bin j+1, the queue entries of bin j+2 and the queue range of bin j+3 are
+// already in flight.
+template <int T, bool MULTI>
+__global__ void __launch_bounds__(kRT) k_raster(const int4 *__restrict__ geo, const uint4 *__restrict__ colr,
+                                                uint32_t *__restrict__ bstart, uint32_t *__restrict__ bend,
+                                                const uint32_t *__restrict__ bin_tris, FrameArgs fa,
+                                                const FrameArgs *__restrict__ jobs, int njobs, int bin_lo,
+                                                int bin_hi, uint32_t *__restrict__ color, float *__restrict__ depth,
+                                                int persistent) {
   __shared__ unsigned long long vis[T * T];
-  __shared__ uint32_t coff[NT + 1];
-  __shared__ uint32_t s_rect[NT];  // rx | ry << 12 | w << 24
-  __shared__ float s_rcpw[NT];
-  __shared__ int s_xy[6][NT];
-  __shared__ float s_zr[4][NT];    // z2, z0mz2, z1mz2, onebybary
-  __shared__ uint32_t s_tri[NT];
+  __shared__ RasterBatch bt;
   __shared__ float s_u8[256];
-  __shared__ uint32_t sh[NT / 64 + 1];
 
   const int t = threadIdx.x;
-  const int bin = blockIdx.x;
-  const int bx = bin % fa.nBx, by = bin / fa.nBx;
-  const int xlo = bx * T, ylo = by * T;
-  const int xhi = min(xlo + T, fa.W), yhi = min(ylo + T, fa.H);
-  const int S = fa.S;
-  for (int q = t; q < T * T; q += NT) vis[q] = kVisSentinel;
-  for (int q = t; q < 256; q += NT) s_u8[q] = (float)q / 255.0f;  // fromABGR, math.h:323-330
-  const uint32_t beg = bin_start[bin], end = bin_start[bin + 1];
-  __syncthreads();
+  const int nb = bin_hi - bin_lo;
+  // Bin order over [bin_lo, bin_hi).  One bin per block: block i -> XCD i % 8
+  // gets a contiguous band of bins.  Persistent: wave w takes bins w,
+  // w + grid, w + 2 grid, ... (interleaved across the screen, which evens out
+  // empty and dense regions).
+  int jn = 0;
+  auto next_bin = [&]() -> int {
+    const int j = jn++;
+    if (!persistent) {
+      if (j > 0) return -1;
+      const int per = (nb + 7) / 8;
+      const int b = (blockIdx.x % 8) * per + blockIdx.x / 8;
+      return b < nb ? bin_lo + b : -1;
+    }
+    const int b = blockIdx.x + j * gridDim.x;
+    return b < nb ? bin_lo + b : -1;
+  };
+  for (int q = t; q < 256; q += kRT) s_u8[q] = kU8Table.v[q];
 
-  for (uint32_t b0 = beg; b0 < end; b0 += NT) {
-    const uint32_t j = b0 + t;
-    uint32_t nc = 0;
-    if (j < end) {
-      const uint32_t tri = bin_tris[j];
-      const int4 a = geo[3 * (size_t)tri + 0], b = geo[3 * (size_t)tri + 1], c = geo[3 * (size_t)tri + 2];
-      const int p[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
-      // pixel range of the rounded bbox (rasterMacros.h:58-63) inside the bin
-      const int lox = (imin3(p[0], p[2], p[4]) & (int)0xffffff00) >> S;
-      const int loy = (imin3(p[1], p[3], p[5]) & (int)0xffffff00) >> S;
-      const int hix = ((imax3(p[0], p[2], p[4]) + 0xff) & (int)0xffffff00) >> S;
-      const int hiy = ((imax3(p[1], p[3], p[5]) + 0xff) & (int)0xffffff00) >> S;
-      const int x0 = max(lox, xlo), x1 = min(hix, xhi);
-      const int y0 = max(loy, ylo), y1 = min(hiy, yhi);
-      const int w = max(0, x1 - x0), h = max(0, y1 - y0);
-      nc = (uint32_t)(w * h);
-      s_rect[t] = (uint32_t)(x0 & 0xfff) | ((uint32_t)(y0 & 0xfff) << 12) | ((uint32_t)w << 24);
-      s_rcpw[t] = w ? 1.0f / (float)w : 0.0f;
-#pragma unroll
-      for (int k = 0; k < 6; ++k) s_xy[k][t] = p[k];
-      s_zr[0][t] = __int_as_float(b.z);
-      s_zr[1][t] = __int_as_float(b.w);
-      s_zr[2][t] = __int_as_float(c.x);
-      s_zr[3][t] = __int_as_float(c.y);
-      s_tri[t] = tri;
-    }
-    uint32_t tot;
-    const uint32_t ex = block_excl_scan<NT>(nc, sh, tot);
-    coff[t] = ex;
-    if (t == 0) coff[NT] = tot;
-    __syncthreads();
-    for (uint32_t cidx = t; cidx < tot; cidx += NT) {
-      // owner: largest o with coff[o] <= cidx (coff is non-decreasing)
-      int lo = 0, hi = NT - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (coff[mid] <= cidx) lo = mid;
-        else hi = mid - 1;
-      }
-      const int o = lo;
-      const uint32_t k = cidx - coff[o];
-      const uint32_t rect = s_rect[o];
-      const int w = (int)(rect >> 24);
-      const int q = (int)(((float)k + 0.5f) * s_rcpw[o]);
-      const int r = (int)k - q * w;
-      const int px = (int)(rect & 0xfff) + r, py = (int)((rect >> 12) & 0xfff) + q;
-      Edges e;
-      const int p[6] = {s_xy[0][o], s_xy[1][o], s_xy[2][o], s_xy[3][o], s_xy[4][o], s_xy[5][o]};
-      e.set(p);
-      long long e0, e1, e2;
-      pixel_edges(e, px, py, S, e0, e1, e2);
-      if ((e0 | e1 | e2) >= 0) {
-        const float rcp = s_zr[3][o];
-        const float alpha = (float)e1 * rcp;  // raster.pikostage:269-272
-        const float beta = (float)e2 * rcp;
-        const float z = interp(s_zr[1][o], s_zr[2][o], s_zr[0][o], alpha, beta);
-        atomicMin((long long *)&vis[(py - ylo) * T + (px - xlo)], (long long)vis_key(z, s_tri[o]));
-      }
-    }
-    __syncthreads();
-  }
+  // pipeline prologue
+  int bin0 = next_bin();
+  int bin1 = bin0 >= 0 ? next_bin() : -1;
+  int bin2 = bin1 >= 0 ? next_bin() : -1;
+  uint32_t b0 = 0, e0 = 0, b1 = 0, e1 = 0, b2 = 0, e2 = 0;
+  if (bin0 >= 0) { b0 = bstart[bin0]; e0 = bend[bin0]; }
+  if (bin1 >= 0) { b1 = bstart[bin1]; e1 = bend[bin1]; }
+  if (bin2 >= 0) { b2 = bstart[bin2]; e2 = bend[bin2]; }
+  uint32_t tri0 = b0 + t < e0 ? bin_tris[b0 + t] : 0u;
+  uint32_t tri1 = b1 + t < e1 ? bin_tris[b1 + t] : 0u;
+  TriRec g0;
+  if (b0 + t < e0) g0 = load_tri(geo, colr, tri0);
 
-  // resolve: depth = winning z, colour = Gouraud of the winner (raster.pikostage:289-316)
-  for (int q = t; q < T * T; q += NT) {
-    const int px = xlo + (q % T), py = ylo + (q / T);
-    if (px >= xhi || py >= yhi) continue;
-    const unsigned long long key = vis[q];
-    uint32_t col = kClearColor;
-    float d = 1.0f;
-    if (key != kVisSentinel) {
-      const uint32_t tri = 0xFFFFFFFEu - (uint32_t)(key & 0xFFFFFFFFull);
-      d = __int_as_float((int)(key >> 32));
-      const int4 a = geo[3 * (size_t)tri + 0], b = geo[3 * (size_t)tri + 1], c = geo[3 * (size_t)tri + 2];
-      const int p[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
-      Edges e;
-      e.set(p);
-      long long e0, e1, e2;
-      pixel_edges(e, px, py, S, e0, e1, e2);
-      const float rcp = __int_as_float(c.y);
-      const float alpha = (float)e1 * rcp;
-      const float beta = (float)e2 * rcp;
-      const uint4 cc = colr[tri];
-      float rgb[3];
-#pragma unroll
-      for (int ch = 0; ch < 3; ++ch) {
-        const float v0 = s_u8[(cc.x >> (8 * ch)) & 255u];
-        const float v1 = s_u8[(cc.y >> (8 * ch)) & 255u];
-        const float v2 = s_u8[(cc.z >> (8 * ch)) & 255u];
-        rgb[ch] = interp(v0 - v2, v1 - v2, v2, alpha, beta);
-      }
-      col = to_abgr(rgb[0], rgb[1], rgb[2]);
+  while (bin0 >= 0) {
+    // issue the next stages' loads
+    const int bin3 = bin2 >= 0 ? next_bin() : -1;
+    uint32_t b3 = 0, e3 = 0;
+    if (bin3 >= 0) { b3 = bstart[bin3]; e3 = bend[bin3]; }
+    const uint32_t tri2 = b2 + t < e2 ? bin_tris[b2 + t] : 0u;
+    TriRec g1;
+    if (b1 + t < e1) g1 = load_tri(geo, colr, tri1);
+
+    // rasterise bin0
+    const FrameArgs &ja = MULTI ? jobs[job_search(jobs, njobs, (uint32_t)bin0, true)] : fa;
+    const int S = ja.S;
+    const int lbin = MULTI ? bin0 - (int)ja.bin_base : bin0;
+    const size_t fbb = MULTI ? (size_t)ja.fb_base : 0;
+    const int bx = lbin % ja.nBx, by = lbin / ja.nBx;
+    const int xlo = bx * T, ylo = by * T;
+    const int xhi = min(xlo + T, ja.W), yhi = min(ylo + T, ja.H);
+    if (t == 0) {  // leave the queue range cleared for the next frame
+      bstart[bin0] = 0;
+      bend[bin0] = 0;
     }
-    const size_t pid = (size_t)py * fa.W + px;
-    color[pid] = col;
-    depth[pid] = d;
+    for (int q = t; q < T * T; q += kRT) vis[q] = kVisSentinel;
+    const uint32_t nq = e0 - b0;
+    const uint32_t nc = stage_tri(bt, t, t < (int)nq, g0, S, xlo, xhi, ylo, yhi);
+    raster_batch<T>(bt, vis, nc, t, S, xlo, ylo, 0u);
+    for (uint32_t off = kRT; off < nq; off += kRT) {  // queues longer than one batch
+      const bool v = off + t < nq;
+      TriRec r;
+      if (v) r = load_tri(geo, colr, bin_tris[b0 + off + t]);
+      const uint32_t ncx = stage_tri(bt, t, v, r, S, xlo, xhi, ylo, yhi);
+      raster_batch<T>(bt, vis, ncx, t, S, xlo, ylo, off);
+    }
+    // resolve: depth = winning z, colour = Gouraud of the winner, clear elsewhere
+    for (int q = t; q < T * T; q += kRT) {
+      const int px = xlo + (q % T), py = ylo + (q / T);
+      if (px >= xhi || py >= yhi) continue;
+      const unsigned long long key = vis[q];
+      const size_t pid = fbb + (size_t)py * ja.W + px;
+      uint32_t col = kClearColor;
+      if (key != kVisSentinel) {
+        const uint32_t pos = 0xFFFFFFFEu - (uint32_t)(key & 0xFFFFFFFFull);
+        int xy[6];
+        float rcp;
+        uint32_t c0, c1, c2;
+        if (nq <= kRT) {
+#pragma unroll
+          for (int k = 0; k < 6; ++k) xy[k] = bt.xy[k][pos];
+          rcp = bt.zr[3][pos];
+          c0 = bt.col[0][pos];
+          c1 = bt.col[1][pos];
+          c2 = bt.col[2][pos];
+        } else {
+          const TriRec r = load_tri(geo, colr, bin_tris[b0 + pos]);
+          xy[0] = r.a.x; xy[1] = r.a.y; xy[2] = r.a.z; xy[3] = r.a.w; xy[4] = r.b.x; xy[5] = r.b.y;
+          rcp = __int_as_float(r.c.y);
+          c0 = r.col.x;
+          c1 = r.col.y;
+          c2 = r.col.z;
+        }
+        float alpha = 0.0f, beta = 0.0f;
+        covered_at(xy, small_bbox(xy, S), px, py, S, rcp, alpha, beta);
+        col = gouraud(s_u8, c0, c1, c2, alpha, beta);
+      }
+      depth[pid] = __int_as_float((int)(key >> 32));
+      color[pid] = col;
+    }
+    __syncthreads();
+    // rotate the pipeline
+    bin0 = bin1; b0 = b1; e0 = e1; g0 = g1;
+    bin1 = bin2; b1 = b2; e1 = e2; tri1 = tri2;
+    bin2 = bin3; b2 = b3; e2 = e3;
   }
 }
 
@@ -443,33 +710,59 @@ inline long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
 }  // namespace
 
 // ---- pipe object --------------------------------------------------------
+//
+// One object serves both forms.  A plain pipe is one job described by
+// kernel arguments (RasterPipe of rasterPipe.h:16-45).  A batch pipe holds a
+// device job table: every job is a screen with its own camera, resolution and
+// bin size drawing a range of the resident input triangles; all jobs of a
+// batch run through one kernel sequence (one raster launch per bin size).
+
+struct RasterClass {
+  int T, bin_lo, bin_hi;
+};
 
 struct piko_raster_pipe {
-  int W = 0, H = 0, S = 0, T = 0, lgT = 0, nBx = 0, nBy = 0, nbins = 0, kind = 0, count = 0;
-  FrameArgs fa{};
+  bool multi = false;
+  int kind = 0, S = 0;
+  int W = 0, H = 0, T = 0, lgT = 0, nBx = 0, nBy = 0;  // plain pipe view
+  FrameArgs fa{};                                         // plain pipe job
+  // current frame
+  int ntris = 0, nbins = 0, passes = 0;
+  long long npix = 0;
+  std::vector<FrameArgs> jobs;     // device order: grouped by bin size, then submission order
+  std::vector<int> job_slot;       // submitted job index -> position in `jobs`
+  std::vector<RasterClass> classes;
+  bool in_flight = false;
+  // capacities
+  long long cap_tris = 0, cap_pix = 0;
+  int cap_bins = 0, cap_jobs = 0;
+  long long cap = 0;  // (triangle, bin) pairs
+  int nblk_cap = 0;
+  long long input_tris = 0;
   hipStream_t stream = nullptr;
   const float *d_pos = nullptr;
   const void *d_attr = nullptr;
   bool owns_input = false;
+  FrameArgs *d_jobs = nullptr;
+  FrameArgs *h_jobs = nullptr;  // pinned staging for the job table
   int4 *d_geo = nullptr;
   uint4 *d_col = nullptr;
-  uint32_t *d_cnt = nullptr;   // [n]
-  uint32_t *d_off = nullptr;   // [n+1]
-  uint32_t *d_blk = nullptr;   // scan block sums
-  long long blk_cap = 0;
+  uint32_t *d_cnt = nullptr;         // per triangle: bins assigned
+  uint32_t *d_keep = nullptr;        // per vertex-stage tile: emitted triangles
+  uint32_t *d_tile_pairs = nullptr;  // per vertex-stage tile: pairs, then their offsets
+  FrameState *d_fs = nullptr;
+  uint32_t *d_bstart = nullptr, *d_bend = nullptr;  // [cap_bins]
   uint32_t *d_keys[2] = {nullptr, nullptr};
   uint32_t *d_vals[2] = {nullptr, nullptr};
-  long long cap = 0;
-  uint32_t *d_hist = nullptr;  // [256 * nblk_cap + 1]
-  int nblk_cap = 0;
-  uint32_t *d_start = nullptr; // [nbins+1]
-  uint32_t *d_flags = nullptr; // [0] overflow
-  uint32_t *d_keep = nullptr;  // per vertex-stage block: emitted triangles
+  uint32_t *d_hist = nullptr;    // [256][nblk_cap] digit matrix
+  uint32_t *d_rowtot = nullptr;  // [256] digit totals
   uint32_t *d_color = nullptr;
   float *d_depth = nullptr;
-  int passes = 0;
   int final_buf = 0;
+  int raster_waves = 0;       // persistent raster grid
+  int raster_persistent = 1;  // 1: persistent pipelined waves; 0: one block per bin
   long long device_bytes = 0;
+  long long pair_bytes = 0, tri_bytes = 0, pix_bytes = 0, bin_bytes = 0;
 };
 
 namespace {
@@ -480,220 +773,332 @@ namespace {
   } while (0)
 
 template <typename T>
-int dalloc(piko_raster_pipe *p, T **ptr, long long elems) {
+int dalloc(piko_raster_pipe *p, T **ptr, long long elems, long long *acct = nullptr) {
   const size_t bytes = (size_t)std::max<long long>(elems, 1) * sizeof(T);
   if (hipMalloc((void **)ptr, bytes) != hipSuccess) {
     *ptr = nullptr;
     return PIKO_ERR_OOM;
   }
   p->device_bytes += (long long)bytes;
+  if (acct) *acct += (long long)bytes;
   return PIKO_OK;
+}
+
+template <typename T>
+void dfree(T *&ptr) {
+  if (ptr) (void)hipFree((void *)ptr);
+  ptr = nullptr;
 }
 
 void free_pairs(piko_raster_pipe *p) {
   for (int k = 0; k < 2; ++k) {
-    if (p->d_keys[k]) (void)hipFree(p->d_keys[k]);
-    if (p->d_vals[k]) (void)hipFree(p->d_vals[k]);
-    p->d_keys[k] = p->d_vals[k] = nullptr;
+    dfree(p->d_keys[k]);
+    dfree(p->d_vals[k]);
   }
-  if (p->d_hist) (void)hipFree(p->d_hist);
-  p->d_hist = nullptr;
-  if (p->d_blk) (void)hipFree(p->d_blk);
-  p->d_blk = nullptr;
-  p->device_bytes -= (long long)(4 * p->cap * sizeof(uint32_t)) +
-                     (long long)((256ll * p->nblk_cap + 1) * sizeof(uint32_t)) +
-                     (long long)(std::max<long long>(p->blk_cap, 1) * sizeof(uint32_t));
+  dfree(p->d_hist);
+  p->device_bytes -= p->pair_bytes;
+  p->pair_bytes = 0;
 }
 
 int alloc_pairs(piko_raster_pipe *p, long long cap) {
   p->cap = cap;
-  p->nblk_cap = (int)cdiv(cap, kTile);
+  p->nblk_cap = (int)cdiv(cap, kSortTile);
   int rc;
   for (int k = 0; k < 2; ++k) {
-    if ((rc = dalloc(p, &p->d_keys[k], cap)) != PIKO_OK) return rc;
-    if ((rc = dalloc(p, &p->d_vals[k], cap)) != PIKO_OK) return rc;
+    if ((rc = dalloc(p, &p->d_keys[k], cap, &p->pair_bytes)) != PIKO_OK) return rc;
+    if ((rc = dalloc(p, &p->d_vals[k], cap, &p->pair_bytes)) != PIKO_OK) return rc;
   }
-  if ((rc = dalloc(p, &p->d_hist, 256ll * p->nblk_cap + 1)) != PIKO_OK) return rc;
-  const long long scan_n = std::max<long long>(p->count, 256ll * p->nblk_cap);
-  p->blk_cap = cdiv(scan_n, kTile) + 1;
-  if ((rc = dalloc(p, &p->d_blk, p->blk_cap)) != PIKO_OK) return rc;
+  if ((rc = dalloc(p, &p->d_hist, 256ll * p->nblk_cap, &p->pair_bytes)) != PIKO_OK) return rc;
   return PIKO_OK;
 }
 
-void scan(piko_raster_pipe *p, const uint32_t *in, long long n, uint32_t *out) {
-  const int nb = (int)std::max<long long>(cdiv(n, kTile), 1);
-  k_scan_reduce<<<nb, kThreads, 0, p->stream>>>(in, n, p->d_blk);
-  k_scan_top<<<1, 1024, 0, p->stream>>>(p->d_blk, nb, out, n);
-  k_scan_down<<<nb, kThreads, 0, p->stream>>>(in, n, p->d_blk, out);
+void free_tris(piko_raster_pipe *p) {
+  dfree(p->d_geo);
+  dfree(p->d_col);
+  dfree(p->d_cnt);
+  dfree(p->d_keep);
+  dfree(p->d_tile_pairs);
+  p->device_bytes -= p->tri_bytes;
+  p->tri_bytes = 0;
 }
 
-// Kernel names in launch order, for profiling.
-const char *kNames[] = {"k_vs_count", "k_scan_reduce", "k_scan_top", "k_scan_down", "k_emit_pairs",
-                        "k_radix_hist", "k_scan_reduce", "k_scan_top", "k_scan_down", "k_radix_scatter",
-                        "k_radix_hist", "k_scan_reduce", "k_scan_top", "k_scan_down", "k_radix_scatter",
-                        "k_radix_hist", "k_scan_reduce", "k_scan_top", "k_scan_down", "k_radix_scatter",
-                        "k_bin_bounds", "k_raster"};
+int alloc_tris(piko_raster_pipe *p, long long ntris) {
+  p->cap_tris = std::max<long long>(ntris, 1);
+  const long long tiles = cdiv(p->cap_tris, kVsTile);
+  int rc;
+  if ((rc = dalloc(p, &p->d_geo, 3 * p->cap_tris, &p->tri_bytes)) != PIKO_OK) return rc;
+  if ((rc = dalloc(p, &p->d_col, p->cap_tris, &p->tri_bytes)) != PIKO_OK) return rc;
+  if ((rc = dalloc(p, &p->d_cnt, p->cap_tris, &p->tri_bytes)) != PIKO_OK) return rc;
+  if ((rc = dalloc(p, &p->d_keep, tiles, &p->tri_bytes)) != PIKO_OK) return rc;
+  if ((rc = dalloc(p, &p->d_tile_pairs, tiles, &p->tri_bytes)) != PIKO_OK) return rc;
+  return PIKO_OK;
+}
 
-// Enqueues one frame.  ev (optional) receives an event after each kernel.
+void free_pix(piko_raster_pipe *p) {
+  dfree(p->d_color);
+  dfree(p->d_depth);
+  p->device_bytes -= p->pix_bytes;
+  p->pix_bytes = 0;
+}
+
+int alloc_pix(piko_raster_pipe *p, long long npix) {
+  p->cap_pix = std::max<long long>(npix, 1);
+  int rc;
+  if ((rc = dalloc(p, &p->d_color, p->cap_pix, &p->pix_bytes)) != PIKO_OK) return rc;
+  if ((rc = dalloc(p, &p->d_depth, p->cap_pix, &p->pix_bytes)) != PIKO_OK) return rc;
+  return PIKO_OK;
+}
+
+void free_bins(piko_raster_pipe *p) {
+  dfree(p->d_bstart);
+  dfree(p->d_bend);
+  p->device_bytes -= p->bin_bytes;
+  p->bin_bytes = 0;
+}
+
+int alloc_bins(piko_raster_pipe *p, int nbins) {
+  p->cap_bins = std::max(nbins, 1);
+  int rc;
+  if ((rc = dalloc(p, &p->d_bstart, (long long)p->cap_bins, &p->bin_bytes)) != PIKO_OK) return rc;
+  if ((rc = dalloc(p, &p->d_bend, (long long)p->cap_bins, &p->bin_bytes)) != PIKO_OK) return rc;
+  // queue ranges start cleared; the raster kernel clears each one after use
+  HIPOK(hipMemsetAsync(p->d_bstart, 0, (size_t)p->cap_bins * 4, p->stream));
+  HIPOK(hipMemsetAsync(p->d_bend, 0, (size_t)p->cap_bins * 4, p->stream));
+  return PIKO_OK;
+}
+
+int radix_passes(int nbins) {
+  int bits = 0;
+  while ((1ll << bits) < (long long)nbins) ++bits;
+  return (bits + 7) / 8;
+}
+
+// Kernel names in launch order (the sort section repeats once per pass, the
+// raster once per bin-size class).
+const char *kernel_name(const piko_raster_pipe *p, int k) {
+  static const char *head[] = {"k_vs_count", "k_scan_tiles", "k_emit_pairs"};
+  static const char *pass[] = {"k_sort_count", "k_sort_rows", "k_sort_scatter"};
+  if (k < 3) return head[k];
+  if (k < 3 + 3 * p->passes) return pass[(k - 3) % 3];
+  if (k == 3 + 3 * p->passes) return "k_bounds";
+  return "k_raster";
+}
+
+template <bool MULTI>
+void launch_raster(piko_raster_pipe *p, int T, int bin_lo, int bin_hi, const uint32_t *bin_tris) {
+  const int nb = bin_hi - bin_lo;
+  if (nb <= 0) return;
+  const int rgrid = p->raster_persistent ? std::max(1, std::min(p->raster_waves, nb)) : 8 * ((nb + 7) / 8);
+#define PIKO_RASTER_LAUNCH(TT)                                                                                   \
+  k_raster<TT, MULTI><<<rgrid, kRT, 0, p->stream>>>(p->d_geo, p->d_col, p->d_bstart, p->d_bend, bin_tris, p->fa, \
+                                                    p->d_jobs, (int)p->jobs.size(), bin_lo, bin_hi, p->d_color,  \
+                                                    p->d_depth, p->raster_persistent)
+  switch (T) {
+    case 8: PIKO_RASTER_LAUNCH(8); break;
+    case 16: PIKO_RASTER_LAUNCH(16); break;
+    case 32: PIKO_RASTER_LAUNCH(32); break;
+    default: PIKO_RASTER_LAUNCH(64); break;
+  }
+#undef PIKO_RASTER_LAUNCH
+}
+
+// Enqueues one frame of the current job set.  ev (optional) receives an event
+// after each kernel.
 int enqueue_frame(piko_raster_pipe *p, hipEvent_t *ev, int *nev) {
   int e = 0;
   auto mark = [&]() {
     if (ev) (void)hipEventRecord(ev[e], p->stream);
     ++e;
   };
-  const int n = p->count;
-  const int gv = (int)std::max<long long>(cdiv(n, kThreads), 1);
-  if (p->kind == PIKO_INPUT_PRETRANSFORMED)
-    k_vs_count<PIKO_INPUT_PRETRANSFORMED><<<gv, kThreads, 0, p->stream>>>(
-        p->d_pos, p->d_attr, n, p->fa, p->d_geo, p->d_col, p->d_cnt, p->d_keep);
-  else
-    k_vs_count<PIKO_INPUT_WORLD><<<gv, kThreads, 0, p->stream>>>(p->d_pos, p->d_attr, n, p->fa, p->d_geo,
-                                                                 p->d_col, p->d_cnt, p->d_keep);
-  mark();
-  {
-    const int nb = (int)std::max<long long>(cdiv(n, kTile), 1);
-    k_scan_reduce<<<nb, kThreads, 0, p->stream>>>(p->d_cnt, n, p->d_blk);
-    mark();
-    k_scan_top<<<1, 1024, 0, p->stream>>>(p->d_blk, nb, p->d_off, n);
-    mark();
-    k_scan_down<<<nb, kThreads, 0, p->stream>>>(p->d_cnt, n, p->d_blk, p->d_off);
-    mark();
+  const int n = p->ntris;
+  const int tiles = (int)std::max<long long>(cdiv(n, kVsTile), 1);
+  const int nj = (int)p->jobs.size();
+#define PIKO_VS_LAUNCH(KIND, MULTI)                                                                              \
+  k_vs_count<KIND, MULTI><<<tiles, kVsThreads, 0, p->stream>>>(p->d_pos, p->d_attr, n, p->fa, p->d_jobs, nj,     \
+                                                               p->d_geo, p->d_col, p->d_cnt, p->d_tile_pairs,    \
+                                                               p->d_keep)
+  if (p->kind == PIKO_INPUT_PRETRANSFORMED) {
+    if (p->multi) PIKO_VS_LAUNCH(PIKO_INPUT_PRETRANSFORMED, true);
+    else PIKO_VS_LAUNCH(PIKO_INPUT_PRETRANSFORMED, false);
+  } else {
+    if (p->multi) PIKO_VS_LAUNCH(PIKO_INPUT_WORLD, true);
+    else PIKO_VS_LAUNCH(PIKO_INPUT_WORLD, false);
   }
-  k_emit_pairs<<<gv, kThreads, 0, p->stream>>>(p->d_geo, p->d_off, n, p->fa, p->cap, p->d_keys[0],
-                                               p->d_vals[0], p->d_flags);
+#undef PIKO_VS_LAUNCH
   mark();
-  const uint32_t *total = p->d_off + n;
+  k_scan_tiles<<<1, 1024, 0, p->stream>>>(p->d_tile_pairs, tiles, p->d_fs, p->cap);
+  mark();
+  if (p->multi)
+    k_emit_pairs<true><<<tiles, kVsThreads, 0, p->stream>>>(p->d_geo, p->d_cnt, n, p->fa, p->d_jobs, nj,
+                                                           p->d_tile_pairs, p->cap, p->d_keys[0], p->d_vals[0]);
+  else
+    k_emit_pairs<false><<<tiles, kVsThreads, 0, p->stream>>>(p->d_geo, p->d_cnt, n, p->fa, p->d_jobs, nj,
+                                                            p->d_tile_pairs, p->cap, p->d_keys[0], p->d_vals[0]);
+  mark();
   int cur = 0;
-  const long long hn = 256ll * p->nblk_cap;
   for (int pass = 0; pass < p->passes; ++pass) {
-    k_radix_hist<<<p->nblk_cap, kThreads, 0, p->stream>>>(p->d_keys[cur], total, p->cap, p->d_hist,
+    k_sort_count<<<p->nblk_cap, kThreads, 0, p->stream>>>(p->d_keys[cur], p->d_fs, p->cap, p->d_hist,
                                                           p->nblk_cap, 8 * pass);
     mark();
-    const int nb = (int)std::max<long long>(cdiv(hn, kTile), 1);
-    k_scan_reduce<<<nb, kThreads, 0, p->stream>>>(p->d_hist, hn, p->d_blk);
+    k_sort_rows<<<256, kThreads, 0, p->stream>>>(p->d_hist, p->d_fs, p->cap, p->nblk_cap, p->d_rowtot);
     mark();
-    k_scan_top<<<1, 1024, 0, p->stream>>>(p->d_blk, nb, p->d_hist, hn);
-    mark();
-    k_scan_down<<<nb, kThreads, 0, p->stream>>>(p->d_hist, hn, p->d_blk, p->d_hist);
-    mark();
-    k_radix_scatter<<<p->nblk_cap, kThreads, 0, p->stream>>>(p->d_keys[cur], p->d_vals[cur], p->d_keys[cur ^ 1],
-                                                             p->d_vals[cur ^ 1], total, p->cap, p->d_hist,
-                                                             p->nblk_cap, 8 * pass);
+    k_sort_scatter<<<p->nblk_cap, kThreads, 0, p->stream>>>(p->d_keys[cur], p->d_vals[cur], p->d_keys[cur ^ 1],
+                                                            p->d_vals[cur ^ 1], p->d_fs, p->cap, p->d_hist,
+                                                            p->d_rowtot, p->nblk_cap, 8 * pass);
     mark();
     cur ^= 1;
   }
   p->final_buf = cur;
-  {
-    const long long nthreads = std::max<long long>(p->cap, p->nbins + 1);
-    k_bin_bounds<<<(int)cdiv(nthreads, kThreads), kThreads, 0, p->stream>>>(p->d_keys[cur], total, p->cap,
-                                                                           p->d_start, p->nbins);
-    mark();
-  }
-  switch (p->T) {
-    case 8:
-      k_raster<8, 64><<<p->nbins, 64, 0, p->stream>>>(p->d_geo, p->d_col, p->d_start, p->d_vals[cur], p->fa,
-                                                      p->d_color, p->d_depth);
-      break;
-    case 16:
-      k_raster<16, 256><<<p->nbins, 256, 0, p->stream>>>(p->d_geo, p->d_col, p->d_start, p->d_vals[cur], p->fa,
-                                                         p->d_color, p->d_depth);
-      break;
-    case 32:
-      k_raster<32, 256><<<p->nbins, 256, 0, p->stream>>>(p->d_geo, p->d_col, p->d_start, p->d_vals[cur], p->fa,
-                                                         p->d_color, p->d_depth);
-      break;
-    default:
-      k_raster<64, 256><<<p->nbins, 256, 0, p->stream>>>(p->d_geo, p->d_col, p->d_start, p->d_vals[cur], p->fa,
-                                                         p->d_color, p->d_depth);
-      break;
-  }
+  const int gstride = (int)std::min<long long>(std::max<long long>(cdiv(p->cap, kThreads * 8), 1), 1024);
+  k_bounds<<<gstride, kThreads, 0, p->stream>>>(p->d_keys[cur], p->d_fs, p->cap, p->d_bstart, p->d_bend);
   mark();
+  for (const RasterClass &c : p->classes) {
+    if (p->multi) launch_raster<true>(p, c.T, c.bin_lo, c.bin_hi, p->d_vals[cur]);
+    else launch_raster<false>(p, c.T, c.bin_lo, c.bin_hi, p->d_vals[cur]);
+    if (c.bin_hi > c.bin_lo) mark();
+  }
   if (nev) *nev = e;
   return hipGetLastError() == hipSuccess ? PIKO_OK : PIKO_ERR_HIP;
 }
 
-// Reads the overflow flag and the pair total; grows the queues when they
-// overflowed.  Returns 1 if the frame must be re-run.
-int check_capacity(piko_raster_pipe *p, int *rerun) {
-  uint32_t flags[1], total;
-  HIPOK(hipMemcpyAsync(flags, p->d_flags, sizeof(flags), hipMemcpyDeviceToHost, p->stream));
-  HIPOK(hipMemcpyAsync(&total, p->d_off + p->count, sizeof(total), hipMemcpyDeviceToHost, p->stream));
+int read_frame_state(piko_raster_pipe *p, FrameState *fs) {
+  HIPOK(hipMemcpyAsync(fs, p->d_fs, sizeof(FrameState), hipMemcpyDeviceToHost, p->stream));
   HIPOK(hipStreamSynchronize(p->stream));
+  return PIKO_OK;
+}
+
+// Reads the overflow flag and the pair total; grows the queues when they
+// overflowed.  *rerun = 1 if the frames must be re-run.
+int check_capacity(piko_raster_pipe *p, int *rerun) {
+  FrameState fs;
+  int rc = read_frame_state(p, &fs);
+  if (rc != PIKO_OK) return rc;
   *rerun = 0;
-  if (flags[0] || (long long)total > p->cap) {
+  if (fs.overflow || (long long)fs.total > p->cap) {
     free_pairs(p);
-    int rc = alloc_pairs(p, (long long)total + (long long)total / 4 + 4096);
+    rc = alloc_pairs(p, (long long)fs.total + (long long)fs.total / 4 + 4096);
     if (rc != PIKO_OK) return rc;
-    HIPOK(hipMemsetAsync(p->d_flags, 0, sizeof(uint32_t), p->stream));
+    const uint32_t zero = 0;
+    HIPOK(hipMemcpyAsync(&p->d_fs->overflow, &zero, 4, hipMemcpyHostToDevice, p->stream));
+    HIPOK(hipStreamSynchronize(p->stream));
     *rerun = 1;
   }
   return PIKO_OK;
 }
 
+bool valid_bin(int T) { return T == 8 || T == 16 || T == 32 || T == 64; }
+
+bool valid_screen(int W, int H) { return W > 0 && H > 0 && W <= 4095 && H <= 4095; }
+
 int validate_opts(const piko_constant_state *cs, int count, const piko_raster_options *opt) {
   if (!cs || !opt || count < 0) return PIKO_ERR_INVALID;
-  if (cs->screenSizeX <= 0 || cs->screenSizeY <= 0 || cs->screenSizeX > 4095 || cs->screenSizeY > 4095)
-    return PIKO_ERR_INVALID;
-  const int T = opt->bin_size;
-  if (T != 8 && T != 16 && T != 32 && T != 64) return PIKO_ERR_INVALID;
+  if (!valid_screen(cs->screenSizeX, cs->screenSizeY)) return PIKO_ERR_INVALID;
+  if (!valid_bin(opt->bin_size)) return PIKO_ERR_INVALID;
   if (opt->subpixel_bits < 1 || opt->subpixel_bits > 8) return PIKO_ERR_INVALID;
   if (opt->input_kind != PIKO_INPUT_PRETRANSFORMED && opt->input_kind != PIKO_INPUT_WORLD)
     return PIKO_ERR_INVALID;
   return PIKO_OK;
 }
 
-void set_frame_args(piko_raster_pipe *p, const piko_constant_state *cs) {
-  FrameArgs &fa = p->fa;
-  fa.W = p->W;
-  fa.H = p->H;
-  fa.S = p->S;
-  fa.lgT = p->lgT;
-  fa.nBx = p->nBx;
-  fa.nBy = p->nBy;
+FrameArgs make_args(const piko_constant_state *cs, int S, int T) {
+  FrameArgs fa{};
+  fa.W = cs->screenSizeX;
+  fa.H = cs->screenSizeY;
+  fa.S = S;
+  fa.lgT = ilog2i(T);
+  fa.nBx = (fa.W + T - 1) / T;
+  fa.nBy = (fa.H + T - 1) / T;
   fa.halfW = cs->halfW;
   fa.halfH = cs->halfH;
   std::memcpy(fa.vp, cs->viewProjMatrix, sizeof(fa.vp));
   std::memcpy(fa.light, cs->light, sizeof(fa.light));
   std::memcpy(fa.matcol, cs->matcol, sizeof(fa.matcol));
   std::memcpy(fa.ambient, cs->ambient, sizeof(fa.ambient));
+  fa.nbins = (uint32_t)(fa.nBx * fa.nBy);
+  return fa;
 }
 
-int allocate_common(piko_raster_pipe **out, const piko_constant_state *cs, int count,
-                    const piko_raster_options *opt, piko_raster_pipe *p) {
+int create_common(piko_raster_pipe *p, int kind, int S, long long tris, long long pix, int bins,
+                  long long pair_capacity) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PIKO_ERR_NO_DEVICE;
-  p->W = cs->screenSizeX;
-  p->H = cs->screenSizeY;
-  p->S = opt->subpixel_bits;
-  p->T = opt->bin_size;
-  p->lgT = ilog2i(p->T);
-  p->nBx = (p->W + p->T - 1) / p->T;
-  p->nBy = (p->H + p->T - 1) / p->T;
-  p->nbins = p->nBx * p->nBy;
-  p->kind = opt->input_kind;
-  p->count = count;
-  int bits = 0;
-  while ((1ll << bits) < (long long)p->nbins) ++bits;
-  p->passes = (bits + 7) / 8;
-  set_frame_args(p, cs);
+  p->kind = kind;
+  p->S = S;
+  {
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    p->raster_waves = cus * 20;  // ~20 resident waves per CU
+    const char *m = std::getenv("PIKO_RASTER_PERSISTENT");
+    if (m) p->raster_persistent = m[0] == '1';
+  }
   HIPOK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
   int rc;
-  if ((rc = dalloc(p, &p->d_geo, 3ll * count)) != PIKO_OK) return rc;
-  if ((rc = dalloc(p, &p->d_col, (long long)count)) != PIKO_OK) return rc;
-  if ((rc = dalloc(p, &p->d_cnt, (long long)count)) != PIKO_OK) return rc;
-  if ((rc = dalloc(p, &p->d_off, (long long)count + 1)) != PIKO_OK) return rc;
-  if ((rc = dalloc(p, &p->d_start, (long long)p->nbins + 1)) != PIKO_OK) return rc;
-  if ((rc = dalloc(p, &p->d_flags, 4)) != PIKO_OK) return rc;
-  if ((rc = dalloc(p, &p->d_keep, std::max<long long>(cdiv(count, kThreads), 1))) != PIKO_OK) return rc;
-  if ((rc = dalloc(p, &p->d_color, (long long)p->W * p->H)) != PIKO_OK) return rc;
-  if ((rc = dalloc(p, &p->d_depth, (long long)p->W * p->H)) != PIKO_OK) return rc;
-  const long long cap = opt->pair_capacity > 0 ? opt->pair_capacity
-                                               : std::max<long long>(2ll * count + 1024, 1 << 16);
+  if ((rc = dalloc(p, &p->d_rowtot, 256)) != PIKO_OK) return rc;
+  if ((rc = dalloc(p, &p->d_fs, 1)) != PIKO_OK) return rc;
+  if ((rc = alloc_tris(p, tris)) != PIKO_OK) return rc;
+  if ((rc = alloc_pix(p, pix)) != PIKO_OK) return rc;
+  if ((rc = alloc_bins(p, bins)) != PIKO_OK) return rc;
+  const long long cap = pair_capacity > 0 ? pair_capacity : std::max<long long>(2ll * tris + 1024, 1 << 16);
   if ((rc = alloc_pairs(p, cap)) != PIKO_OK) return rc;
-  HIPOK(hipMemsetAsync(p->d_flags, 0, 4 * sizeof(uint32_t), p->stream));
-  HIPOK(hipMemsetAsync(p->d_start, 0, ((size_t)p->nbins + 1) * sizeof(uint32_t), p->stream));
-  HIPOK(hipMemsetAsync(p->d_off, 0, ((size_t)count + 1) * sizeof(uint32_t), p->stream));
+  HIPOK(hipMemsetAsync(p->d_fs, 0, sizeof(FrameState), p->stream));
   HIPOK(hipStreamSynchronize(p->stream));
+  return PIKO_OK;
+}
+
+int upload_input(piko_raster_pipe *p, const float *pos, const void *attr, long long tris, int kind) {
+  const size_t pb = (size_t)tris * 9 * sizeof(float);
+  const size_t ab = (size_t)tris * (kind == PIKO_INPUT_WORLD ? 9 * sizeof(float) : 3 * sizeof(uint32_t));
+  float *dp = nullptr;
+  void *da = nullptr;
+  p->owns_input = true;
+  if (hipMalloc((void **)&dp, std::max<size_t>(pb, 4)) != hipSuccess) return PIKO_ERR_OOM;
+  p->d_pos = dp;
+  if (hipMalloc(&da, std::max<size_t>(ab, 4)) != hipSuccess) return PIKO_ERR_OOM;
+  p->d_attr = da;
+  p->device_bytes += (long long)(pb + ab);
+  if (tris > 0 && (hipMemcpy(dp, pos, pb, hipMemcpyHostToDevice) != hipSuccess ||
+                   hipMemcpy(da, attr, ab, hipMemcpyHostToDevice) != hipSuccess))
+    return PIKO_ERR_HIP;
+  return PIKO_OK;
+}
+
+int plain_pipe(piko_raster_pipe **out, const piko_constant_state *cs, const float *pos, const void *attr, int count,
+               const piko_raster_options *opt, bool on_device) {
+  if (!out) return PIKO_ERR_INVALID;
+  *out = nullptr;
+  int rc = validate_opts(cs, count, opt);
+  if (rc != PIKO_OK) return rc;
+  if (count > 0 && (!pos || !attr)) return PIKO_ERR_INVALID;
+  piko_raster_pipe *p = new (std::nothrow) piko_raster_pipe();
+  if (!p) return PIKO_ERR_OOM;
+  p->W = cs->screenSizeX;
+  p->H = cs->screenSizeY;
+  p->T = opt->bin_size;
+  p->fa = make_args(cs, opt->subpixel_bits, p->T);
+  p->lgT = p->fa.lgT;
+  p->nBx = p->fa.nBx;
+  p->nBy = p->fa.nBy;
+  p->ntris = count;
+  p->input_tris = count;
+  p->nbins = (int)p->fa.nbins;
+  p->npix = (long long)p->W * p->H;
+  p->passes = radix_passes(p->nbins);
+  p->jobs.assign(1, p->fa);
+  p->classes.assign(1, RasterClass{p->T, 0, p->nbins});
+  rc = create_common(p, opt->input_kind, opt->subpixel_bits, count, p->npix, p->nbins, opt->pair_capacity);
+  if (rc == PIKO_OK) {
+    if (on_device) {
+      p->d_pos = pos;
+      p->d_attr = attr;
+    } else {
+      rc = upload_input(p, pos, attr, count, opt->input_kind);
+    }
+  }
+  if (rc != PIKO_OK) {
+    piko_raster_destroy(p);
+    return rc;
+  }
   *out = p;
   return PIKO_OK;
 }
@@ -719,12 +1124,16 @@ int piko_raster_destroy(piko_raster_pipe *p) {
   if (!p) return PIKO_ERR_INVALID;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   free_pairs(p);
-  void *bufs[] = {p->d_geo, p->d_col, p->d_cnt, p->d_off, p->d_start, p->d_flags, p->d_keep, p->d_color, p->d_depth};
-  for (void *b : bufs)
-    if (b) (void)hipFree(b);
+  free_tris(p);
+  free_pix(p);
+  free_bins(p);
+  dfree(p->d_rowtot);
+  dfree(p->d_fs);
+  dfree(p->d_jobs);
+  if (p->h_jobs) (void)hipHostFree(p->h_jobs);
   if (p->owns_input) {
-    if (p->d_pos) (void)hipFree((void *)p->d_pos);
-    if (p->d_attr) (void)hipFree((void *)p->d_attr);
+    dfree(p->d_pos);
+    dfree(p->d_attr);
   }
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
@@ -733,61 +1142,19 @@ int piko_raster_destroy(piko_raster_pipe *p) {
 
 int piko_raster_allocate(piko_raster_pipe **out, const piko_constant_state *cs, const float *pos,
                          const void *attr, int count, const piko_raster_options *opt) {
-  if (!out) return PIKO_ERR_INVALID;
-  *out = nullptr;
-  int rc = validate_opts(cs, count, opt);
-  if (rc != PIKO_OK) return rc;
-  if (count > 0 && (!pos || !attr)) return PIKO_ERR_INVALID;
-  piko_raster_pipe *p = new (std::nothrow) piko_raster_pipe();
-  if (!p) return PIKO_ERR_OOM;
-  rc = allocate_common(out, cs, count, opt, p);
-  if (rc == PIKO_OK) {
-    const size_t pb = (size_t)count * 9 * sizeof(float);
-    const size_t ab = (size_t)count * (opt->input_kind == PIKO_INPUT_WORLD ? 9 * sizeof(float) : 3 * sizeof(uint32_t));
-    float *dp = nullptr;
-    void *da = nullptr;
-    p->owns_input = true;
-    if (hipMalloc((void **)&dp, std::max<size_t>(pb, 4)) != hipSuccess) rc = PIKO_ERR_OOM;
-    p->d_pos = dp;
-    if (rc == PIKO_OK && hipMalloc(&da, std::max<size_t>(ab, 4)) != hipSuccess) rc = PIKO_ERR_OOM;
-    p->d_attr = da;
-    if (rc == PIKO_OK) {
-      p->device_bytes += (long long)(pb + ab);
-      if (count > 0 && (hipMemcpy(dp, pos, pb, hipMemcpyHostToDevice) != hipSuccess ||
-                        hipMemcpy(da, attr, ab, hipMemcpyHostToDevice) != hipSuccess))
-        rc = PIKO_ERR_HIP;
-    }
-  }
-  if (rc != PIKO_OK) {
-    *out = nullptr;
-    piko_raster_destroy(p);
-  }
-  return rc;
+  return plain_pipe(out, cs, pos, attr, count, opt, false);
 }
 
 int piko_raster_allocate_device(piko_raster_pipe **out, const piko_constant_state *cs, const float *d_pos,
                                 const void *d_attr, int count, const piko_raster_options *opt) {
-  if (!out) return PIKO_ERR_INVALID;
-  *out = nullptr;
-  int rc = validate_opts(cs, count, opt);
-  if (rc != PIKO_OK) return rc;
-  if (count > 0 && (!d_pos || !d_attr)) return PIKO_ERR_INVALID;
-  piko_raster_pipe *p = new (std::nothrow) piko_raster_pipe();
-  if (!p) return PIKO_ERR_OOM;
-  p->d_pos = d_pos;
-  p->d_attr = d_attr;
-  rc = allocate_common(out, cs, count, opt, p);
-  if (rc != PIKO_OK) {
-    *out = nullptr;
-    piko_raster_destroy(p);
-  }
-  return rc;
+  return plain_pipe(out, cs, d_pos, d_attr, count, opt, true);
 }
 
 int piko_raster_prepare(piko_raster_pipe *p, const piko_constant_state *cs) {
-  if (!p || !cs) return PIKO_ERR_INVALID;
+  if (!p || !cs || p->multi) return PIKO_ERR_INVALID;
   if (cs->screenSizeX != p->W || cs->screenSizeY != p->H) return PIKO_ERR_INVALID;
-  set_frame_args(p, cs);
+  p->fa = make_args(cs, p->S, p->T);
+  p->jobs.assign(1, p->fa);
   return PIKO_OK;
 }
 
@@ -836,15 +1203,10 @@ int piko_raster_profile_frames(piko_raster_pipe *p, int nframes, float *kernel_m
     HIPOK(hipEventElapsedTime(&ms, ev0, ev[ne - 1]));
     facc += ms;
   }
-  // map event slots to kernel names (the radix section repeats per pass)
-  int n = std::min(ne, cap);
+  const int n = std::min(ne, cap);
   for (int k = 0; k < n; ++k) {
     kernel_ms[k] = (float)(acc[k] / nframes);
-    if (names) {
-      if (k < 5) names[k] = kNames[k];
-      else if (k < 5 + 5 * p->passes) names[k] = kNames[5 + ((k - 5) % 5)];
-      else names[k] = k == ne - 1 ? "k_raster" : "k_bin_bounds";
-    }
+    if (names) names[k] = kernel_name(p, k);
   }
   *nk = n;
   if (frame_ms) *frame_ms = (float)(facc / nframes);
@@ -856,49 +1218,54 @@ int piko_raster_profile_frames(piko_raster_pipe *p, int nframes, float *kernel_m
 }
 
 int piko_raster_get_data(piko_raster_pipe *p, uint32_t *color) {
-  if (!p || !color) return PIKO_ERR_INVALID;
-  HIPOK(hipMemcpyAsync(color, p->d_color, (size_t)p->W * p->H * 4, hipMemcpyDeviceToHost, p->stream));
+  if (!p || !color || p->multi) return PIKO_ERR_INVALID;
+  HIPOK(hipMemcpyAsync(color, p->d_color, (size_t)p->npix * 4, hipMemcpyDeviceToHost, p->stream));
   HIPOK(hipStreamSynchronize(p->stream));
   return PIKO_OK;
 }
 
 int piko_raster_get_depth(piko_raster_pipe *p, float *depth) {
-  if (!p || !depth) return PIKO_ERR_INVALID;
-  HIPOK(hipMemcpyAsync(depth, p->d_depth, (size_t)p->W * p->H * 4, hipMemcpyDeviceToHost, p->stream));
+  if (!p || !depth || p->multi) return PIKO_ERR_INVALID;
+  HIPOK(hipMemcpyAsync(depth, p->d_depth, (size_t)p->npix * 4, hipMemcpyDeviceToHost, p->stream));
   HIPOK(hipStreamSynchronize(p->stream));
   return PIKO_OK;
 }
 
 long long piko_raster_get_bins(piko_raster_pipe *p, int64_t *bin_start, int32_t *bin_tris, long long cap) {
   if (!p || !bin_start) return PIKO_ERR_INVALID;
-  uint32_t total = 0;
-  HIPOK(hipMemcpyAsync(&total, p->d_off + p->count, 4, hipMemcpyDeviceToHost, p->stream));
-  uint32_t *st = (uint32_t *)std::malloc(((size_t)p->nbins + 1) * 4);
-  if (!st) return PIKO_ERR_OOM;
-  if (hipMemcpyAsync(st, p->d_start, ((size_t)p->nbins + 1) * 4, hipMemcpyDeviceToHost, p->stream) !=
-          hipSuccess ||
-      hipStreamSynchronize(p->stream) != hipSuccess) {
-    std::free(st);
+  FrameState fs;
+  int rc = read_frame_state(p, &fs);
+  if (rc != PIKO_OK) return rc;
+  const long long P = std::min<long long>(fs.total, p->cap);
+  uint32_t *keys = (uint32_t *)std::malloc((size_t)std::max<long long>(P, 1) * 4);
+  if (!keys) return PIKO_ERR_OOM;
+  if (P > 0 && (hipMemcpyAsync(keys, p->d_keys[p->final_buf], (size_t)P * 4, hipMemcpyDeviceToHost, p->stream) !=
+                    hipSuccess ||
+                hipStreamSynchronize(p->stream) != hipSuccess)) {
+    std::free(keys);
     return PIKO_ERR_HIP;
   }
-  for (int b = 0; b <= p->nbins; ++b) bin_start[b] = st[b];
-  std::free(st);
-  const long long n = std::min<long long>(std::min<long long>(total, p->cap), cap);
+  for (int b = 0; b <= p->nbins; ++b) bin_start[b] = 0;
+  for (long long i = 0; i < P; ++i) bin_start[keys[i] + 1]++;
+  for (int b = 0; b < p->nbins; ++b) bin_start[b + 1] += bin_start[b];
+  std::free(keys);
+  const long long n = std::min<long long>(P, cap);
   if (bin_tris && n > 0) {
     HIPOK(hipMemcpyAsync(bin_tris, p->d_vals[p->final_buf], (size_t)n * 4, hipMemcpyDeviceToHost, p->stream));
     HIPOK(hipStreamSynchronize(p->stream));
   }
-  return (long long)total;
+  return (long long)fs.total;
 }
 
 int piko_raster_get_stats(piko_raster_pipe *p, piko_raster_stats *st) {
   if (!p || !st) return PIKO_ERR_INVALID;
-  uint32_t total = 0;
-  const int nkb = (int)std::max<long long>(cdiv(p->count, kThreads), 1);
+  FrameState fs;
+  int rc = read_frame_state(p, &fs);
+  if (rc != PIKO_OK) return rc;
+  const int nkb = (int)std::max<long long>(cdiv(p->ntris, kVsTile), 1);
   uint32_t *kb = (uint32_t *)std::malloc((size_t)nkb * 4);
   if (!kb) return PIKO_ERR_OOM;
-  if (hipMemcpyAsync(&total, p->d_off + p->count, 4, hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
-      hipMemcpyAsync(kb, p->d_keep, (size_t)nkb * 4, hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
+  if (hipMemcpyAsync(kb, p->d_keep, (size_t)nkb * 4, hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
       hipStreamSynchronize(p->stream) != hipSuccess) {
     std::free(kb);
     return PIKO_ERR_HIP;
@@ -906,13 +1273,160 @@ int piko_raster_get_stats(piko_raster_pipe *p, piko_raster_stats *st) {
   long long emitted = 0;
   for (int k = 0; k < nkb; ++k) emitted += kb[k];
   std::free(kb);
-  st->triangles = p->count;
-  st->emitted = p->count ? emitted : 0;
-  st->pairs = total;
+  st->triangles = p->ntris;
+  st->emitted = p->ntris ? emitted : 0;
+  st->pairs = fs.total;
   st->pair_capacity = p->cap;
-  st->bins_x = p->nBx;
-  st->bins_y = p->nBy;
+  st->bins_x = p->multi ? 0 : p->nBx;
+  st->bins_y = p->multi ? 0 : p->nBy;
   st->device_bytes = p->device_bytes;
+  return PIKO_OK;
+}
+
+// ---- batches --------------------------------------------------------------
+
+int piko_batch_create(piko_raster_pipe **out, const float *pos, const void *attr, long long input_tris,
+                      int on_device, const piko_batch_options *opt) {
+  if (!out || !opt || input_tris < 0) return PIKO_ERR_INVALID;
+  *out = nullptr;
+  if (opt->subpixel_bits < 1 || opt->subpixel_bits > 8) return PIKO_ERR_INVALID;
+  if (opt->input_kind != PIKO_INPUT_PRETRANSFORMED && opt->input_kind != PIKO_INPUT_WORLD) return PIKO_ERR_INVALID;
+  if (opt->max_jobs <= 0 || opt->max_tris < 0 || opt->max_pixels <= 0) return PIKO_ERR_INVALID;
+  if (input_tris > 0 && (!pos || !attr)) return PIKO_ERR_INVALID;
+  piko_raster_pipe *p = new (std::nothrow) piko_raster_pipe();
+  if (!p) return PIKO_ERR_OOM;
+  p->multi = true;
+  p->input_tris = input_tris;
+  p->cap_jobs = opt->max_jobs;
+  const int bins = (int)std::min<long long>(opt->max_pixels / 64 + 1024, 1ll << 30);
+  int rc = create_common(p, opt->input_kind, opt->subpixel_bits, opt->max_tris, opt->max_pixels, bins,
+                         opt->pair_capacity);
+  if (rc == PIKO_OK) rc = dalloc(p, &p->d_jobs, (long long)p->cap_jobs);
+  if (rc == PIKO_OK && hipHostMalloc((void **)&p->h_jobs, (size_t)p->cap_jobs * sizeof(FrameArgs)) != hipSuccess)
+    rc = PIKO_ERR_OOM;
+  if (rc == PIKO_OK) {
+    if (on_device) {
+      p->d_pos = pos;
+      p->d_attr = attr;
+    } else {
+      rc = upload_input(p, pos, attr, input_tris, opt->input_kind);
+    }
+  }
+  if (rc != PIKO_OK) {
+    piko_raster_destroy(p);
+    return rc;
+  }
+  *out = p;
+  return PIKO_OK;
+}
+
+int piko_batch_wait(piko_raster_pipe *p) {
+  if (!p || !p->multi) return PIKO_ERR_INVALID;
+  if (!p->in_flight) return PIKO_OK;
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    int rerun = 0;
+    int rc = check_capacity(p, &rerun);
+    if (rc != PIKO_OK) return rc;
+    if (!rerun) {
+      p->in_flight = false;
+      return PIKO_OK;
+    }
+    rc = enqueue_frame(p, nullptr, nullptr);
+    if (rc != PIKO_OK) return rc;
+  }
+  return PIKO_ERR_CAPACITY;
+}
+
+int piko_batch_query(piko_raster_pipe *p) {
+  if (!p || !p->multi) return PIKO_ERR_INVALID;
+  if (!p->in_flight) return 1;
+  return hipStreamQuery(p->stream) == hipSuccess ? 1 : 0;
+}
+
+int piko_batch_submit(piko_raster_pipe *p, const piko_raster_job *jobs, int njobs) {
+  if (!p || !p->multi || !jobs || njobs <= 0 || njobs > p->cap_jobs) return PIKO_ERR_INVALID;
+  int rc = piko_batch_wait(p);  // one batch in flight per pipe
+  if (rc != PIKO_OK) return rc;
+  long long tris = 0, pix = 0, bins = 0;
+  for (int j = 0; j < njobs; ++j) {
+    const piko_raster_job &jb = jobs[j];
+    if (jb.count < 0 || jb.src_base < 0 || jb.src_base + jb.count > p->input_tris) return PIKO_ERR_INVALID;
+    if (!valid_bin(jb.bin_size) || !valid_screen(jb.cs.screenSizeX, jb.cs.screenSizeY)) return PIKO_ERR_INVALID;
+    tris += jb.count;
+    pix += (long long)jb.cs.screenSizeX * jb.cs.screenSizeY;
+    bins += (long long)((jb.cs.screenSizeX + jb.bin_size - 1) / jb.bin_size) *
+            ((jb.cs.screenSizeY + jb.bin_size - 1) / jb.bin_size);
+  }
+  if (tris > 0x7FFFFFFFll || bins > 0x7FFFFFFFll) return PIKO_ERR_CAPACITY;
+  // grow per-triangle, per-pixel and per-bin storage when this batch needs it
+  HIPOK(hipStreamSynchronize(p->stream));
+  if (tris > p->cap_tris) {
+    free_tris(p);
+    if ((rc = alloc_tris(p, tris)) != PIKO_OK) return rc;
+  }
+  if (pix > p->cap_pix) {
+    free_pix(p);
+    if ((rc = alloc_pix(p, pix)) != PIKO_OK) return rc;
+  }
+  if (bins > p->cap_bins) {
+    free_bins(p);
+    if ((rc = alloc_bins(p, (int)bins)) != PIKO_OK) return rc;
+  }
+  // Jobs are stored grouped by bin size (stable in submission order), so that
+  // each raster launch covers one contiguous bin range and both the triangle
+  // and the bin numbering increase with the stored index (the kernels find a
+  // triangle's or a bin's job by binary search).
+  p->jobs.clear();
+  p->job_slot.assign(njobs, -1);
+  p->classes.clear();
+  uint32_t vbase = 0, bb = 0;
+  unsigned long long fbb = 0;
+  for (int T : {8, 16, 32, 64}) {
+    const uint32_t lo = bb;
+    for (int j = 0; j < njobs; ++j) {
+      if (jobs[j].bin_size != T) continue;
+      FrameArgs fa = make_args(&jobs[j].cs, p->S, T);
+      fa.vbase = vbase;
+      fa.src_base = (uint32_t)jobs[j].src_base;
+      fa.bin_base = bb;
+      fa.fb_base = fbb;
+      vbase += (uint32_t)jobs[j].count;
+      bb += fa.nbins;
+      fbb += (unsigned long long)fa.W * fa.H;
+      p->job_slot[j] = (int)p->jobs.size();
+      p->jobs.push_back(fa);
+    }
+    if (bb > lo) p->classes.push_back(RasterClass{T, (int)lo, (int)bb});
+  }
+  p->ntris = (int)tris;
+  p->nbins = (int)bb;
+  p->npix = (long long)fbb;
+  p->passes = radix_passes(p->nbins);
+  std::memcpy(p->h_jobs, p->jobs.data(), (size_t)njobs * sizeof(FrameArgs));
+  HIPOK(hipMemcpyAsync(p->d_jobs, p->h_jobs, (size_t)njobs * sizeof(FrameArgs), hipMemcpyHostToDevice, p->stream));
+  rc = enqueue_frame(p, nullptr, nullptr);
+  if (rc != PIKO_OK) return rc;
+  p->in_flight = true;
+  return PIKO_OK;
+}
+
+int piko_batch_get_color(piko_raster_pipe *p, int job, uint32_t *color) {
+  if (!p || !p->multi || !color || job < 0 || job >= (int)p->job_slot.size()) return PIKO_ERR_INVALID;
+  int rc = piko_batch_wait(p);
+  if (rc != PIKO_OK) return rc;
+  const FrameArgs &fa = p->jobs[p->job_slot[job]];
+  HIPOK(hipMemcpyAsync(color, p->d_color + fa.fb_base, (size_t)fa.W * fa.H * 4, hipMemcpyDeviceToHost, p->stream));
+  HIPOK(hipStreamSynchronize(p->stream));
+  return PIKO_OK;
+}
+
+int piko_batch_get_depth(piko_raster_pipe *p, int job, float *depth) {
+  if (!p || !p->multi || !depth || job < 0 || job >= (int)p->job_slot.size()) return PIKO_ERR_INVALID;
+  int rc = piko_batch_wait(p);
+  if (rc != PIKO_OK) return rc;
+  const FrameArgs &fa = p->jobs[p->job_slot[job]];
+  HIPOK(hipMemcpyAsync(depth, p->d_depth + fa.fb_base, (size_t)fa.W * fa.H * 4, hipMemcpyDeviceToHost, p->stream));
+  HIPOK(hipStreamSynchronize(p->stream));
   return PIKO_OK;
 }
 

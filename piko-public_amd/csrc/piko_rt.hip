@@ -79,6 +79,11 @@ int piko_rt_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
   return bytes ? st(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)) : PIKO_OK;
 }
 
+int piko_rt_memcpy_dtod(void *dst, const void *src, size_t bytes) {
+  if ((!dst || !src) && bytes) return PIKO_ERR_INVALID;
+  return bytes ? st(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice)) : PIKO_OK;
+}
+
 int piko_rt_memset_d32(void *dst, uint32_t value, size_t count) {
   if (!dst && count) return PIKO_ERR_INVALID;
   return count ? st(hipMemsetD32((hipDeviceptr_t)dst, (int)value, count)) : PIKO_OK;

@@ -20,11 +20,19 @@ constexpr unsigned long long kVisSentinel =
 // Per-frame uniforms passed by value (the reference keeps them in the
 // module-global __constant__ ConstantState, stage.h:22-24; passing them as
 // kernel arguments lets several pipes run concurrently).
+// In a batch every job (a scene, or a request on a resident scene) carries
+// its own copy, plus where its triangles and bins live:
+//   vbase    first triangle of the job in the batch's triangle numbering
+//   src_base first triangle of the job in the input buffer
+//   bin_base first bin of the job in the batch's bin numbering
+//   fb_base  first pixel of the job's framebuffer in the batch's buffers
 struct FrameArgs {
   int W, H, S, lgT, nBx, nBy;
   float halfW, halfH;
   float vp[16];
   float light[3], matcol[3], ambient[3];
+  uint32_t vbase, src_base, bin_base, nbins;
+  unsigned long long fb_base;
 };
 
 __device__ __forceinline__ int imin3(int a, int b, int c) { return min(min(a, b), c); }

@@ -71,6 +71,18 @@ class Options(ctypes.Structure):
                 ("input_kind", ctypes.c_int), ("pair_capacity", ctypes.c_longlong)]
 
 
+class RasterJob(ctypes.Structure):
+    """piko_raster_job: one screen of a batch."""
+    _fields_ = [("count", ctypes.c_int), ("src_base", ctypes.c_longlong), ("bin_size", ctypes.c_int),
+                ("cs", ConstantState)]
+
+
+class BatchOptions(ctypes.Structure):
+    _fields_ = [("input_kind", ctypes.c_int), ("subpixel_bits", ctypes.c_int), ("max_jobs", ctypes.c_int),
+                ("max_tris", ctypes.c_longlong), ("max_pixels", ctypes.c_longlong),
+                ("pair_capacity", ctypes.c_longlong)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("triangles", ctypes.c_longlong), ("emitted", ctypes.c_longlong),
                 ("pairs", ctypes.c_longlong), ("pair_capacity", ctypes.c_longlong),
@@ -107,6 +119,12 @@ def lib():
         L.piko_raster_get_bins.restype = LL
         L.piko_raster_get_stats.argtypes = [P, ctypes.POINTER(Stats)]
         L.piko_raster_destroy.argtypes = [P]
+        L.piko_batch_create.argtypes = [pp, P, P, LL, I, ctypes.POINTER(BatchOptions)]
+        L.piko_batch_submit.argtypes = [P, ctypes.POINTER(RasterJob), I]
+        L.piko_batch_query.argtypes = [P]
+        L.piko_batch_wait.argtypes = [P]
+        L.piko_batch_get_color.argtypes = [P, I, P]
+        L.piko_batch_get_depth.argtypes = [P, I, P]
         _lib = L
     return _lib
 
@@ -211,6 +229,90 @@ class RasterPipe:
 
     def __exit__(self, *a):
         self.destroy()
+
+
+class BatchPipe:
+    """Many screens (jobs) over one resident triangle buffer, each frame of the
+    whole batch in one kernel sequence (piko_batch_* in include/piko_hip.h)."""
+
+    def __init__(self, pos, attr, kind, subpixel_bits=8, max_jobs=1024, max_tris=0, max_pixels=1 << 20,
+                 pair_capacity=0, on_device=False):
+        self.kind = PIKO_INPUT_PRETRANSFORMED if kind == "pretransformed" else PIKO_INPUT_WORLD
+        self.S = subpixel_bits
+        if not on_device:
+            pos = np.ascontiguousarray(pos, np.float32)
+            attr = np.ascontiguousarray(attr)
+            self._keep = (pos, attr)
+            pp, ap, n = _p(pos), _p(attr), pos.shape[0]
+        else:
+            pp, ap, n = pos, attr, max_tris
+        self.input_tris = int(n)
+        opt = BatchOptions(self.kind, subpixel_bits, max_jobs, max(max_tris, 1), max_pixels, pair_capacity)
+        h = ctypes.c_void_p()
+        _check(lib().piko_batch_create(ctypes.byref(h), pp, ap, int(n), 1 if on_device else 0, ctypes.byref(opt)),
+               "batch_create")
+        self._h = h
+        self.jobs = []
+
+    def submit(self, jobs):
+        """jobs: list of (count, src_base, bin_size, ConstantState)."""
+        arr = (RasterJob * len(jobs))()
+        for k, (count, src, T, cs) in enumerate(jobs):
+            arr[k].count, arr[k].src_base, arr[k].bin_size, arr[k].cs = int(count), int(src), int(T), cs
+        self.jobs = list(jobs)
+        _check(lib().piko_batch_submit(self._h, arr, len(jobs)), "batch_submit")
+
+    def query(self):
+        return lib().piko_batch_query(self._h) == 1
+
+    def wait(self):
+        _check(lib().piko_batch_wait(self._h), "batch_wait")
+
+    def run_frames(self, n):
+        """Re-runs the last submitted batch n times back to back."""
+        _check(lib().piko_raster_run_frames(self._h, int(n)), "run_frames")
+
+    def profile_frames(self, n, cap=64):
+        return RasterPipe.profile_frames(self, n, cap)
+
+    def color(self, j):
+        cs = self.jobs[j][3]
+        out = np.empty(cs.screenSizeX * cs.screenSizeY, np.uint32)
+        _check(lib().piko_batch_get_color(self._h, int(j), _p(out)), "batch_get_color")
+        return out.reshape(cs.screenSizeY, cs.screenSizeX)
+
+    def depth(self, j):
+        cs = self.jobs[j][3]
+        out = np.empty(cs.screenSizeX * cs.screenSizeY, np.float32)
+        _check(lib().piko_batch_get_depth(self._h, int(j), _p(out)), "batch_get_depth")
+        return out.reshape(cs.screenSizeY, cs.screenSizeX)
+
+    def stats(self):
+        return RasterPipe.stats(self)
+
+    def destroy(self):
+        if self._h is not None:
+            lib().piko_raster_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.destroy()
+
+
+def concat_scenes(scene_list):
+    """Concatenates scenes of one kind into one input buffer; returns
+    (pos, attr, [(count, src_base, scene)])."""
+    kind = scene_list[0].kind
+    pos = np.concatenate([s.pos for s in scene_list]) if scene_list else np.zeros((0, 9), np.float32)
+    attr = np.concatenate([(s.col if kind == "pretransformed" else s.nrm) for s in scene_list])
+    out, base = [], 0
+    for s in scene_list:
+        out.append((s.n, base, s))
+        base += s.n
+    return pos, attr, out
 
 
 def render(scene, bin_size=16, subpixel_bits=8):
