@@ -7,7 +7,11 @@ and one directional Lambert light, 8 subpixel bits, FP32 depth, bin size
 swept over 8/16/32/64.  One step = one frame (vertex stage, binning, raster,
 depth, Gouraud, framebuffer) with the geometry resident in HBM.
 
-Prints one JSON line.  `value` is Mtri/s at the fastest bin size.
+Prints one JSON line.  `value` is Mtri/s at the fastest bin size.  The same
+line carries the batched form (256 uneven scenes in one launch sequence vs
+sequential plain launches, autotuned per scene class) under "batch" and the
+served form (resident scenes, Poisson request stream, continuous batching,
+p50/p99 latency) under "serve".
 """
 import argparse
 import json
@@ -81,6 +85,78 @@ def copy_calibration(mb):
     print(json.dumps({"copy_bytes": n, "copies": 5}), flush=True)
 
 
+def bench_batch(log, reps=5):
+    """256 scenes from seeds 1000-1255 (uniform soups, log-uniform sizes and
+    resolutions), autotuned per class; one batch vs sequential plain pipes."""
+    import numpy as np
+
+    import piko_amd
+    from piko_amd import autotune, scenes
+
+    spec = scenes.batch_spec()
+    scs = [scenes.tris_uniform(seed, n, W, H) for seed, n, W, H in spec]
+    ntris = sum(s.n for s in scs)
+    npix = sum(s.W * s.H for s in scs)
+    log("batch: %d scenes, %d triangles, %d pixels; autotuning" % (len(scs), ntris, npix))
+    per, table = autotune.autotune(scs, log=None)
+    pipes = []
+    for s, cfg in zip(scs, per):
+        pipes.append(piko_amd.RasterPipe().allocate(s, cfg["bin_size"], 8, waves_per_cu=cfg["waves_per_cu"],
+                                                    pair_capacity=int(cfg["queue_depth"] * s.n) + 64))
+    for p in pipes:
+        p.run_frames(1)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for p in pipes:
+            p.run_single()
+    t_seq = (time.perf_counter() - t0) / reps
+    pos, attr, parts = piko_amd.concat_scenes(scs)
+    bp = piko_amd.BatchPipe(pos, attr, "pretransformed", max_jobs=len(scs), max_tris=ntris, max_pixels=npix)
+    bp.submit([(n, base, cfg["bin_size"], piko_amd.ConstantState.for_scene(s), cfg["waves_per_cu"],
+                cfg["queue_depth"]) for (n, base, s), cfg in zip(parts, per)])
+    bp.wait()
+    bp.run_frames(1)
+    t0 = time.perf_counter()
+    bp.run_frames(reps)
+    t_batch = (time.perf_counter() - t0) / reps
+    rng = np.random.default_rng(0)
+    checked = [int(k) for k in rng.choice(len(scs), 12, replace=False)]
+    exact = all(np.array_equal(bp.color(k), pipes[k].pikoScreen.getData()) for k in checked)
+    bp.destroy()
+    for p in pipes:
+        p.destroy()
+    out = {"workload": "256 uniform-soup scenes, seeds 1000-1255, N log-uniform [1K,500K], "
+                       "pixels log-uniform [256^2,2048^2], aspect {1,4:3,16:9}, autotuned per class",
+           "scenes": len(scs), "triangles": ntris, "pixels": npix, "classes": len(table),
+           "seq_ms_per_batch": round(t_seq * 1e3, 3), "batch_ms": round(t_batch * 1e3, 3),
+           "seq_frames_per_s": round(len(scs) / t_seq, 1), "batch_frames_per_s": round(len(scs) / t_batch, 1),
+           "speedup": round(t_seq / t_batch, 2), "batch_Mtri_per_s": round(ntris / t_batch / 1e6, 1),
+           "bit_exact_vs_sequential": exact, "scenes_compared": len(checked)}
+    log("batch: %s" % out)
+    return out
+
+
+def bench_serve(log, tris_per_scene, nreq, rate_hz=2000.0):
+    """64 resident world-space scenes (soups and height fields), Poisson
+    requests with random cameras and 540p/720p/1080p screens."""
+    from piko_amd import serve
+    pos, nrm, base, count = serve.resident_scenes(64, tris_per_scene)
+    gb = (pos.nbytes + nrm.nbytes) / 1e9
+    log("serve: 64 scenes, %.2f GB resident, %d requests at %.0f/s" % (gb, nreq, rate_hz))
+    reqs, _ = serve.poisson_trace(nreq, rate_hz, 64, seed=11, bin_size=8)
+    warm, _ = serve.poisson_trace(64, 1e6, 64, seed=12, bin_size=8)
+    with serve.Server(pos, nrm, base, count, pipes=4, max_jobs=64, budget_bytes=4 << 30) as srv:
+        srv.replay(warm)
+        _, st, _ = srv.replay(reqs)
+    out = {"workload": "64 resident world-space scenes (triangle soups + height fields), Poisson %.0f req/s, "
+                       "camera radius U[2,5], FOV U[40,90], {540p,720p,1080p}" % rate_hz,
+           "resident_gb": round(gb, 2), "triangles_per_scene": int(count.mean()), "requests": nreq,
+           "frames_per_s": round(st["frames_per_s"], 1), "p50_ms": round(st["p50_ms"], 3),
+           "p99_ms": round(st["p99_ms"], 3), "batches": st["batches"], "mean_batch": round(st["mean_batch"], 2)}
+    log("serve: %s" % out)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +164,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--bins", default="8,16,32,64")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="all", choices=["raster", "batch", "serve", "all"])
+    ap.add_argument("--serve-tris", type=int, default=300_000, help="triangles per resident scene when serving")
+    ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -158,6 +237,10 @@ def main():
     if not args.no_cpu:
         log("timing the CPU oracle on a bounded sample")
         out["cpu_baseline"] = cpu_baseline(scene, T, S)
+    if args.workload in ("batch", "all"):
+        out["batch"] = bench_batch(log)
+    if args.workload in ("serve", "all"):
+        out["serve"] = bench_serve(log, args.serve_tris, args.serve_requests)
     print(json.dumps(out), flush=True)
 
 

@@ -65,6 +65,7 @@ typedef struct {
   int input_kind;         /* PIKO_INPUT_* */
   long long pair_capacity;/* (triangle, bin) queue capacity; 0 = automatic,
                              grows on demand between frames */
+  int waves_per_cu;       /* persistent raster waves per CU; 0 = default (20) */
 } piko_raster_options;
 
 typedef struct piko_raster_pipe piko_raster_pipe;
@@ -141,6 +142,8 @@ typedef struct {
   int count;
   long long src_base;
   int bin_size;
+  int waves_per_cu;        /* raster waves per CU for this job's bin-size class; 0 = default */
+  float queue_depth;       /* expected (triangle, bin) queue entries per triangle; 0 = 2 */
   piko_constant_state cs;
 } piko_raster_job;
 
@@ -168,6 +171,44 @@ int piko_batch_wait(piko_raster_pipe *pipe);
 /* Framebuffers of job `job` of the last batch (W*H words / floats). */
 int piko_batch_get_color(piko_raster_pipe *pipe, int job, uint32_t *color);
 int piko_batch_get_depth(piko_raster_pipe *pipe, int job, float *depth);
+
+/* ---- serving ------------------------------------------------------------ */
+
+/* A render request on a resident scene: camera and screen in `cs`. */
+typedef struct {
+  int scene;
+  int bin_size;
+  piko_constant_state cs;
+  double arrival_s;        /* seconds after the start of the trace */
+} piko_request;
+
+typedef struct {
+  int input_kind;
+  int subpixel_bits;
+  int pipes;                    /* batch pipes (HIP streams) in flight, <= 16 */
+  int max_jobs;                 /* requests per batch at most */
+  long long batch_budget_bytes; /* device bytes one batch may use (admission) */
+} piko_server_options;
+
+typedef struct {
+  long long requests, batches;
+  double wall_s, mean_batch;
+} piko_server_stats;
+
+typedef struct piko_server piko_server;
+
+/* Uploads (on_device == 0) or adopts the scenes' geometry once; scene k is
+ * triangles [scene_base[k], scene_base[k] + scene_count[k]). */
+int piko_server_create(piko_server **server, const float *pos, const void *attr, long long tris, int on_device,
+                       const long long *scene_base, const int *scene_count, int nscenes,
+                       const piko_server_options *opt);
+/* Replays `reqs` (sorted by arrival) in real time with continuous batching;
+ * done_s[i] receives request i's completion time (same clock as arrival_s).
+ * keep_color (optional, per request, may hold NULLs) receives framebuffers.
+ * PIKO_ERR_BUSY if a request can never fit the batch budget. */
+int piko_server_replay(piko_server *server, const piko_request *reqs, int n, double *done_s,
+                       uint32_t **keep_color, piko_server_stats *st);
+int piko_server_destroy(piko_server *server);
 
 /* ---- runtime shim (generated-code layer) ------------------------------- */
 

@@ -31,6 +31,13 @@ long long piko_scene_heightfield_count(int grid);
 int piko_scene_heightfield(uint64_t seed, int grid, float amplitude, int octaves, float *pos,
                            float *nrm);
 
+/* World-space triangle soup (the uniform-soup workload placed in a cube so a
+ * camera can view it): centroids uniform in [-extent, extent]^3, uniform
+ * random facing, equilateral with circumradius size_median*exp(sigma*N(0,1)),
+ * counter-clockwise about its unit normal (stored per vertex). */
+int piko_scene_soup_world(uint64_t seed, int n, float extent, float size_median, float sigma, float *pos,
+                          float *nrm);
+
 /* Right-handed look-at times a perspective projection that maps eye-space
  * depth [znear, zfar] to NDC z [0,1]; column-major 16 floats as
  * matmultfloat4 (math.h:365-373) indexes them. */

@@ -718,7 +718,7 @@ inline long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
 // batch run through one kernel sequence (one raster launch per bin size).
 
 struct RasterClass {
-  int T, bin_lo, bin_hi;
+  int T, bin_lo, bin_hi, waves;  // waves: persistent raster grid for this class
 };
 
 struct piko_raster_pipe {
@@ -759,7 +759,8 @@ struct piko_raster_pipe {
   uint32_t *d_color = nullptr;
   float *d_depth = nullptr;
   int final_buf = 0;
-  int raster_waves = 0;       // persistent raster grid
+  int raster_waves = 0;       // persistent raster grid (default)
+  int cus = 256;              // compute units of the device
   int raster_persistent = 1;  // 1: persistent pipelined waves; 0: one block per bin
   long long device_bytes = 0;
   long long pair_bytes = 0, tri_bytes = 0, pix_bytes = 0, bin_bytes = 0;
@@ -885,10 +886,12 @@ const char *kernel_name(const piko_raster_pipe *p, int k) {
 }
 
 template <bool MULTI>
-void launch_raster(piko_raster_pipe *p, int T, int bin_lo, int bin_hi, const uint32_t *bin_tris) {
+void launch_raster(piko_raster_pipe *p, int T, int bin_lo, int bin_hi, int waves_override,
+                   const uint32_t *bin_tris) {
   const int nb = bin_hi - bin_lo;
   if (nb <= 0) return;
-  const int rgrid = p->raster_persistent ? std::max(1, std::min(p->raster_waves, nb)) : 8 * ((nb + 7) / 8);
+  const int waves = waves_override > 0 ? waves_override : p->raster_waves;
+  const int rgrid = p->raster_persistent ? std::max(1, std::min(waves, nb)) : 8 * ((nb + 7) / 8);
 #define PIKO_RASTER_LAUNCH(TT)                                                                                   \
   k_raster<TT, MULTI><<<rgrid, kRT, 0, p->stream>>>(p->d_geo, p->d_col, p->d_bstart, p->d_bend, bin_tris, p->fa, \
                                                     p->d_jobs, (int)p->jobs.size(), bin_lo, bin_hi, p->d_color,  \
@@ -953,8 +956,8 @@ int enqueue_frame(piko_raster_pipe *p, hipEvent_t *ev, int *nev) {
   k_bounds<<<gstride, kThreads, 0, p->stream>>>(p->d_keys[cur], p->d_fs, p->cap, p->d_bstart, p->d_bend);
   mark();
   for (const RasterClass &c : p->classes) {
-    if (p->multi) launch_raster<true>(p, c.T, c.bin_lo, c.bin_hi, p->d_vals[cur]);
-    else launch_raster<false>(p, c.T, c.bin_lo, c.bin_hi, p->d_vals[cur]);
+    if (p->multi) launch_raster<true>(p, c.T, c.bin_lo, c.bin_hi, c.waves, p->d_vals[cur]);
+    else launch_raster<false>(p, c.T, c.bin_lo, c.bin_hi, c.waves, p->d_vals[cur]);
     if (c.bin_hi > c.bin_lo) mark();
   }
   if (nev) *nev = e;
@@ -1028,6 +1031,7 @@ int create_common(piko_raster_pipe *p, int kind, int S, long long tris, long lon
     int dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    p->cus = cus;
     p->raster_waves = cus * 20;  // ~20 resident waves per CU
     const char *m = std::getenv("PIKO_RASTER_PERSISTENT");
     if (m) p->raster_persistent = m[0] == '1';
@@ -1085,8 +1089,8 @@ int plain_pipe(piko_raster_pipe **out, const piko_constant_state *cs, const floa
   p->npix = (long long)p->W * p->H;
   p->passes = radix_passes(p->nbins);
   p->jobs.assign(1, p->fa);
-  p->classes.assign(1, RasterClass{p->T, 0, p->nbins});
   rc = create_common(p, opt->input_kind, opt->subpixel_bits, count, p->npix, p->nbins, opt->pair_capacity);
+  p->classes.assign(1, RasterClass{p->T, 0, p->nbins, opt->waves_per_cu > 0 ? opt->waves_per_cu * p->cus : 0});
   if (rc == PIKO_OK) {
     if (on_device) {
       p->d_pos = pos;
@@ -1381,10 +1385,15 @@ int piko_batch_submit(piko_raster_pipe *p, const piko_raster_job *jobs, int njob
   p->classes.clear();
   uint32_t vbase = 0, bb = 0;
   unsigned long long fbb = 0;
+  long long want_pairs = 0;
   for (int T : {8, 16, 32, 64}) {
     const uint32_t lo = bb;
+    int wpc = 0;
     for (int j = 0; j < njobs; ++j) {
       if (jobs[j].bin_size != T) continue;
+      wpc = std::max(wpc, jobs[j].waves_per_cu);
+      const double qd = jobs[j].queue_depth > 0.0f ? (double)jobs[j].queue_depth : 2.0;
+      want_pairs += (long long)(qd * jobs[j].count) + 64;
       FrameArgs fa = make_args(&jobs[j].cs, p->S, T);
       fa.vbase = vbase;
       fa.src_base = (uint32_t)jobs[j].src_base;
@@ -1396,7 +1405,11 @@ int piko_batch_submit(piko_raster_pipe *p, const piko_raster_job *jobs, int njob
       p->job_slot[j] = (int)p->jobs.size();
       p->jobs.push_back(fa);
     }
-    if (bb > lo) p->classes.push_back(RasterClass{T, (int)lo, (int)bb});
+    if (bb > lo) p->classes.push_back(RasterClass{T, (int)lo, (int)bb, wpc > 0 ? wpc * p->cus : 0});
+  }
+  if (want_pairs > p->cap) {  // preset the queues from the jobs' expected depth
+    free_pairs(p);
+    if ((rc = alloc_pairs(p, want_pairs)) != PIKO_OK) return rc;
   }
   p->ntris = (int)tris;
   p->nbins = (int)bb;

@@ -9,6 +9,7 @@
 // any subset of a scene can be regenerated independently and the output does
 // not depend on thread count.  All geometry maths runs in double and is
 // rounded once to float32.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -179,6 +180,40 @@ int piko_camera_view_proj(const float *eye, const float *target, const float *up
       for (int k = 0; k < 4; ++k) acc += P[r][k] * V[k][c];
       m[c * 4 + r] = (float)acc;  // column-major, as matmultfloat4 indexes it
     }
+  return 0;
+}
+
+int piko_scene_soup_world(uint64_t seed, int n, float extent, float size_median, float sigma, float *pos,
+                          float *nrm) {
+  if (n < 0 || !pos || !nrm) return -1;
+  for (int i = 0; i < n; ++i) {
+    Stream r(seed ^ 0x5EEDull, (uint64_t)i);
+    double c[3], nv[3];
+    for (double &x : c) x = (2.0 * r.u01() - 1.0) * extent;
+    // uniform random unit normal
+    const double zc = 2.0 * r.u01() - 1.0, ph = r.u01() * 6.283185307179586;
+    const double rs = std::sqrt(std::max(0.0, 1.0 - zc * zc));
+    nv[0] = rs * std::cos(ph);
+    nv[1] = rs * std::sin(ph);
+    nv[2] = zc;
+    // tangent frame (t1, t2) with t1 x t2 = n, so vertices at increasing angle
+    // are counter-clockwise seen from the side the normal points to
+    double a[3] = {std::fabs(nv[0]) < 0.9 ? 1.0 : 0.0, std::fabs(nv[0]) < 0.9 ? 0.0 : 1.0, 0.0};
+    double t1[3] = {a[1] * nv[2] - a[2] * nv[1], a[2] * nv[0] - a[0] * nv[2], a[0] * nv[1] - a[1] * nv[0]};
+    const double l1 = std::sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
+    for (double &x : t1) x /= l1;
+    double t2[3] = {nv[1] * t1[2] - nv[2] * t1[1], nv[2] * t1[0] - nv[0] * t1[2], nv[0] * t1[1] - nv[1] * t1[0]};
+    const double rad = (double)size_median * std::exp((double)sigma * r.normal());
+    const double th = r.u01() * 6.283185307179586;
+    for (int v = 0; v < 3; ++v) {
+      const double ang = th + v * (6.283185307179586 / 3.0);
+      const double ca = std::cos(ang) * rad, sa = std::sin(ang) * rad;
+      for (int k = 0; k < 3; ++k) {
+        pos[9 * (size_t)i + 3 * v + k] = (float)(c[k] + ca * t1[k] + sa * t2[k]);
+        nrm[9 * (size_t)i + 3 * v + k] = (float)nv[k];
+      }
+    }
+  }
   return 0;
 }
 

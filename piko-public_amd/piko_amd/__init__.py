@@ -68,13 +68,14 @@ class ConstantState(ctypes.Structure):
 
 class Options(ctypes.Structure):
     _fields_ = [("bin_size", ctypes.c_int), ("subpixel_bits", ctypes.c_int),
-                ("input_kind", ctypes.c_int), ("pair_capacity", ctypes.c_longlong)]
+                ("input_kind", ctypes.c_int), ("pair_capacity", ctypes.c_longlong),
+                ("waves_per_cu", ctypes.c_int)]
 
 
 class RasterJob(ctypes.Structure):
     """piko_raster_job: one screen of a batch."""
     _fields_ = [("count", ctypes.c_int), ("src_base", ctypes.c_longlong), ("bin_size", ctypes.c_int),
-                ("cs", ConstantState)]
+                ("waves_per_cu", ctypes.c_int), ("queue_depth", ctypes.c_float), ("cs", ConstantState)]
 
 
 class BatchOptions(ctypes.Structure):
@@ -159,7 +160,7 @@ class RasterPipe:
         self._h = None
         self.pikoScreen = PikoScreen(self)
 
-    def allocate(self, scene, bin_size=16, subpixel_bits=8, pair_capacity=0):
+    def allocate(self, scene, bin_size=16, subpixel_bits=8, pair_capacity=0, waves_per_cu=0):
         """RasterPipe::allocate (PTXBackend.cpp:649-789): uploads the input and
         sizes every stage.  `scene` is a scenes.TriScene or scenes.WorldScene."""
         if self._h is not None:
@@ -167,7 +168,7 @@ class RasterPipe:
         self.W, self.H, self.T, self.S = scene.W, scene.H, bin_size, subpixel_bits
         self.kind = PIKO_INPUT_PRETRANSFORMED if scene.kind == "pretransformed" else PIKO_INPUT_WORLD
         self.cs = ConstantState.for_scene(scene)
-        opt = Options(bin_size, subpixel_bits, self.kind, pair_capacity)
+        opt = Options(bin_size, subpixel_bits, self.kind, pair_capacity, waves_per_cu)
         pos = np.ascontiguousarray(scene.pos, np.float32)
         attr = np.ascontiguousarray(scene.col if self.kind == 0 else scene.nrm)
         h = ctypes.c_void_p()
@@ -255,11 +256,15 @@ class BatchPipe:
         self.jobs = []
 
     def submit(self, jobs):
-        """jobs: list of (count, src_base, bin_size, ConstantState)."""
+        """jobs: list of (count, src_base, bin_size, ConstantState[, waves_per_cu, queue_depth])."""
         arr = (RasterJob * len(jobs))()
-        for k, (count, src, T, cs) in enumerate(jobs):
+        for k, jb in enumerate(jobs):
+            count, src, T, cs = jb[:4]
             arr[k].count, arr[k].src_base, arr[k].bin_size, arr[k].cs = int(count), int(src), int(T), cs
-        self.jobs = list(jobs)
+            if len(jb) > 4:
+                arr[k].waves_per_cu = int(jb[4])
+                arr[k].queue_depth = float(jb[5])
+        self.jobs = [tuple(jb[:4]) for jb in jobs]
         _check(lib().piko_batch_submit(self._h, arr, len(jobs)), "batch_submit")
 
     def query(self):

@@ -1,0 +1,76 @@
+"""Per-scene-class autotuner for the raster pipeline.
+
+A scene class is (log2 of the triangle count, log2 of the pixel-count edge).
+For one representative scene per class, every candidate (bin size T, raster
+waves per CU) is timed on the GPU with the plain pipe; the fastest wins.  The
+queue depth recorded for the class is the measured (triangle, bin) pairs per
+triangle at the chosen T, with 25% headroom, so batches can size their queues
+up front instead of growing them after an overflow.
+"""
+import json
+import math
+import time
+
+import numpy as np
+
+from . import RasterPipe
+
+BIN_SIZES = (8, 16, 32, 64)
+WAVES_PER_CU = (8, 16, 24)
+
+
+def scene_class(n, W, H):
+    return (int(round(math.log2(max(n, 1)))), int(round(math.log2(max(W * H, 1)) / 2)))
+
+
+def time_plain(scene, T, waves, S=8, frames=4):
+    with RasterPipe().allocate(scene, T, S, waves_per_cu=waves) as pipe:
+        pipe.prepare()
+        pipe.run_frames(1)
+        t0 = time.perf_counter()
+        pipe.run_frames(frames)
+        dt = (time.perf_counter() - t0) / frames
+        st = pipe.stats()
+    return dt, st["pairs"] / max(scene.n, 1)
+
+
+def tune(scene, bin_sizes=BIN_SIZES, waves=WAVES_PER_CU, S=8):
+    best = None
+    trials = []
+    for T in bin_sizes:
+        for w in waves:
+            dt, qd = time_plain(scene, T, w, S)
+            trials.append((T, w, dt))
+            if best is None or dt < best[2]:
+                best = (T, w, dt, qd)
+    T, w, dt, qd = best
+    return {"bin_size": T, "waves_per_cu": w, "queue_depth": round(1.25 * qd + 0.05, 3), "ms": dt * 1e3,
+            "trials": trials}
+
+
+def autotune(scene_list, S=8, log=None):
+    """Returns (per-scene settings list, class table).  The representative of
+    a class is its scene with the median triangle count."""
+    groups = {}
+    for k, sc in enumerate(scene_list):
+        groups.setdefault(scene_class(sc.n, sc.W, sc.H), []).append(k)
+    table = {}
+    for cls, members in sorted(groups.items()):
+        members = sorted(members, key=lambda k: scene_list[k].n)
+        rep = scene_list[members[len(members) // 2]]
+        table[cls] = tune(rep, S=S)
+        if log:
+            log("class %s (%d scenes): T=%d waves/CU=%d queue depth %.2f (%.3f ms)" % (
+                cls, len(members), table[cls]["bin_size"], table[cls]["waves_per_cu"], table[cls]["queue_depth"],
+                table[cls]["ms"]))
+    per_scene = [table[scene_class(sc.n, sc.W, sc.H)] for sc in scene_list]
+    return per_scene, table
+
+
+def save(table, path):
+    with open(path, "w") as f:
+        json.dump({"%d,%d" % k: {kk: vv for kk, vv in v.items() if kk != "trials"} for k, v in table.items()}, f,
+                  indent=1)
+
+
+__all__ = ["scene_class", "tune", "autotune", "save", "np"]

@@ -24,6 +24,8 @@ def lib():
         L.piko_scene_heightfield_count.restype = ctypes.c_longlong
         L.piko_camera_view_proj.argtypes = [P, P, P, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                             ctypes.c_float, P]
+        L.piko_scene_soup_world.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                            ctypes.c_float, P, P]
         L.piko_scene_batch_spec.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int, P, P, P]
         _lib = L
@@ -103,6 +105,15 @@ def heightfield(seed=2, grid=1000, W=1920, H=1080, amplitude=0.2, octaves=6,
     vp = view_proj(eye, target, HF_UP, fovy, W / H, znear, zfar)
     return WorldScene(pos, nrm, W, H, vp, np.asarray(DEFAULT_LIGHT, np.float32),
                       np.asarray(DEFAULT_MATCOL, np.float32), np.asarray(DEFAULT_AMBIENT, np.float32), seed)
+
+
+def soup_world(seed, n, extent=1.0, size_median=0.01, sigma=1.0):
+    """World-space soup: pos/nrm [n, 9]."""
+    pos = np.empty((n, 9), np.float32)
+    nrm = np.empty((n, 9), np.float32)
+    if lib().piko_scene_soup_world(seed, n, extent, size_median, sigma, _p(pos), _p(nrm)) != 0:
+        raise ValueError("bad soup arguments")
+    return pos, nrm
 
 
 def batch_spec(base_seed=1000, nscenes=256, n_lo=1000, n_hi=500000, px_lo_edge=256, px_hi_edge=2048):

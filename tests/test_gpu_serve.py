@@ -1,0 +1,54 @@
+"""GPU serving: requests replayed with continuous batching give, for every
+request, exactly the frame it gives alone and the oracle's frame."""
+import sys
+
+import numpy as np
+import pytest
+
+import piko_amd
+from piko_amd import scenes, serve
+
+pytestmark = pytest.mark.gpu
+
+SMALL_RES = ((160, 90), (200, 120), (96, 96))
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def view_scene(pos, nrm, base, count, sc, view):
+    b, n = int(base[sc]), int(count[sc])
+    return scenes.WorldScene(pos[b:b + n], nrm[b:b + n], view.W, view.H, view.view_proj, view.light, view.matcol,
+                             view.ambient)
+
+
+def test_same_alone_and_in_the_mix(oracle_mod):
+    pos, nrm, base, count = serve.resident_scenes(6, 3000, seed=3)
+    reqs, meta = serve.poisson_trace(30, 50000.0, 6, seed=5, resolutions=SMALL_RES)
+    with serve.Server(pos, nrm, base, count, pipes=2, max_jobs=8, budget_bytes=64 << 20) as s:
+        done, st, cols = s.replay(reqs, keep=True)
+        log("replay: %s" % st)
+        assert st["requests"] == 30 and st["batches"] < 30  # requests were batched together
+        assert np.all(done >= np.array([r.arrival_s for r in reqs]))
+        for i in range(len(reqs)):
+            one = (serve.Request * 1)()
+            one[0] = reqs[i]
+            one[0].arrival_s = 0.0
+            _, st1, c1 = s.replay(one, keep=True)
+            assert st1["batches"] == 1
+            assert np.array_equal(c1[0], cols[i]), i
+    for i in range(4):
+        sc, view = meta[i]
+        oc, od, _ = oracle_mod.frame(view_scene(pos, nrm, base, count, sc, view), 8, 8)
+        assert np.array_equal(cols[i], oc), i
+
+
+def test_admission_refuses_what_never_fits():
+    pos, nrm, base, count = serve.resident_scenes(2, 2000, seed=9)
+    reqs, _ = serve.poisson_trace(3, 1000.0, 2, seed=1, resolutions=((1920, 1080),))
+    with serve.Server(pos, nrm, base, count, pipes=1, max_jobs=4, budget_bytes=1 << 20) as s:
+        with pytest.raises(piko_amd.PikoError) as e:
+            s.replay(reqs)
+        assert e.value.status == -6
