@@ -210,6 +210,80 @@ int piko_server_replay(piko_server *server, const piko_request *reqs, int n, dou
                        uint32_t **keep_color, piko_server_stats *st);
 int piko_server_destroy(piko_server *server);
 
+/* ---- Reyes pipeline ------------------------------------------------------ */
+
+/* The pipe pikoc generates for ReyesPipe (reyesPipe/reyesPipe.h:12-40):
+ * Split (looping on itself) -> Dice -> Shade -> PikoScreen, here as a split
+ * kernel that walks each patch's split tree on the device (the reference
+ * loops on the host with a device-to-host read per level,
+ * PTXBackend.cpp:962-980), a dice + sample kernel and a resolve.
+ * The input is bicubic Bezier patches: 16 control points x (x,y,z,w) floats
+ * per patch (piko_patch.CP, reyesPipe/pikoTypes.h:65-73).  The camera is
+ * cs->viewProjMatrix and the screen cs->screenSizeX/Y; the shading constants
+ * are the ones ShadeStage hard-codes (shade.pikostage:64-85). */
+typedef struct {
+  float threshold;     /* split until both screen extents are below this many
+                          pixels (THRESHOLD_PIXELS, split.pikostage:12: 4) */
+  int max_splits;      /* split depth cap, <= 20 (split.pikostage:284) */
+  int dice;            /* micropolygon grid edge per leaf: 1, 2, 4, 8, 16 or
+                          32 quads covering the patch; 0 = the reference's
+                          (nu+1) x (nv+1) grid from the leaf's extents
+                          (split.pikostage:307-316, dice.pikostage:176-177) */
+  int spp;             /* samples per pixel: 1 (the pixel centre,
+                          shade.pikostage:166-167) or 4 (2x2 strata centres) */
+  float displacement;  /* displacement along the normal by
+                          displacement*(x^2 - y^2) of the surface point; 0 = none */
+  int roots_per_wave;  /* patches whose split trees one wave walks; 0 = 4 */
+  int keep_samples;    /* 1: keep per-sample depth for piko_reyes_get_samples */
+} piko_reyes_options;
+
+typedef struct piko_reyes_pipe piko_reyes_pipe;
+
+/* ReyesPipe::allocate (reyesPipe/main.cpp:225, PTXBackend.cpp:649-789):
+ * patches [count*64] floats, uploaded once. */
+int piko_reyes_allocate(piko_reyes_pipe **pipe, const piko_constant_state *cs, const float *patches, int count,
+                        const piko_reyes_options *opt);
+/* Same with the patches already in device memory (must outlive the pipe). */
+int piko_reyes_allocate_device(piko_reyes_pipe **pipe, const piko_constant_state *cs, const float *d_patches,
+                               int count, const piko_reyes_options *opt);
+/* ReyesPipe::prepare: camera for the next frames (screen size fixed). */
+int piko_reyes_prepare(piko_reyes_pipe *pipe, const piko_constant_state *cs);
+/* ReyesPipe::run_single: one frame, synchronous. */
+int piko_reyes_run_single(piko_reyes_pipe *pipe);
+/* nframes frames back to back, one synchronisation at the end.  Work queues
+ * that overflow grow and the frames re-run (PIKO_ERR_CAPACITY if they cannot). */
+int piko_reyes_run_frames(piko_reyes_pipe *pipe, int nframes);
+/* As piko_raster_profile_frames. */
+int piko_reyes_profile_frames(piko_reyes_pipe *pipe, int nframes, float *kernel_ms, const char **names, int cap,
+                              int *nk, float *frame_ms);
+/* Resolved frame: W*H*4 floats RGBA (box filter over the samples; uncovered
+ * samples hold the PikoScreen clear colour 0xff663313, stage.h:83-85), row 0
+ * at the bottom. */
+int piko_reyes_get_color(piko_reyes_pipe *pipe, float *rgba);
+/* W*H floats: the nearest sample depth of each pixel (1.0 where empty). */
+int piko_reyes_get_depth(piko_reyes_pipe *pipe, float *depth);
+/* W*H*spp float bits of every sample's depth (needs keep_samples). */
+int piko_reyes_get_samples(piko_reyes_pipe *pipe, int32_t *z);
+/* Split leaves of the last frame in the order the split stage emits them
+ * (level by level, in order within a level): cp [cap*64] floats and the
+ * reference's dice grid nu/nv [cap] (split.pikostage:307-316).  Returns the
+ * leaf count (>= 0) or a piko_status; writes at most cap leaves. */
+long long piko_reyes_get_leaves(piko_reyes_pipe *pipe, float *cp, int *nu, int *nv, long long cap);
+
+typedef struct {
+  long long patches;
+  long long leaves;          /* split leaves of the last frame */
+  long long micropolygons;   /* diced quads of the last frame */
+  int levels;                /* split levels used */
+  long long device_bytes;
+  int capacity_reruns;       /* frames re-run by the last run call because a
+                                work queue grew */
+} piko_reyes_stats;
+
+int piko_reyes_get_stats(piko_reyes_pipe *pipe, piko_reyes_stats *st);
+/* ReyesPipe::destroy. */
+int piko_reyes_destroy(piko_reyes_pipe *pipe);
+
 /* ---- runtime shim (generated-code layer) ------------------------------- */
 
 typedef struct piko_rt_module *piko_module_t;

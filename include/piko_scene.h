@@ -50,6 +50,16 @@ int piko_camera_view_proj(const float *eye, const float *target, const float *up
 int piko_scene_batch_spec(uint64_t base_seed, int nscenes, int n_lo, int n_hi, int px_lo_edge,
                           int px_hi_edge, int *n_tris, int *W, int *H);
 
+/* Reyes input: nlat x nlon bicubic Bezier patches tiling the unit sphere
+ * (lat-long), every control point of a shared (3*nlat+1) x 3*nlon lattice
+ * pushed radially by U[-noise, noise] (longitude periodic), so patches stay
+ * connected.  The pole rows lie on rings of angular radius 1e-4 rad rather
+ * than on one point, so no patch has a vanishing derivative.  cp:
+ * [nlat*nlon*64] floats, 16 control points x (x,y,z,w=1) per patch as
+ * piko_patch.CP (reyesPipe/pikoTypes.h:65-73); patch (a,b) at index
+ * a*nlon+b; CP[4*r+c] has r along longitude (v) and c along latitude (u). */
+int piko_scene_sphere_patches(uint64_t seed, int nlat, int nlon, float noise, float *cp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,13 @@ def lib():
         L.oracle_from_abgr.argtypes = [ctypes.c_uint32, P]
         L.oracle_fixpt.argtypes = [ctypes.c_float, I]
         L.oracle_fixpt.restype = ctypes.c_int32
+        L.oracle_reyes_split.argtypes = [P, I, I, I, P, ctypes.c_float, I, P, P, P, LL]
+        L.oracle_reyes_split.restype = LL
+        L.oracle_reyes_dice.argtypes = [P, P, P, LL, I, I, I, P, ctypes.c_float, P, LL]
+        L.oracle_reyes_dice.restype = LL
+        L.oracle_reyes_shade.argtypes = [P, LL, I, I, I, P, P]
+        L.oracle_reyes_frame.argtypes = [P, I, I, I, P, ctypes.c_float, I, I, ctypes.c_float, P, P]
+        L.oracle_reyes_frame.restype = LL
         _lib = L
     return _lib
 
@@ -152,3 +159,53 @@ def from_abgr(c):
 
 def fixpt(v, S):
     return int(lib().oracle_fixpt(v, S))
+
+
+# ---- Reyes ---------------------------------------------------------------
+
+UPOLY = np.dtype([("x", np.float32, 4), ("y", np.float32, 4), ("z", np.float32), ("rgb", np.float32, 3)])
+
+
+def reyes_split(cp, W, H, vp, threshold, max_splits=20):
+    """Leaves in (level, position) order: (leaves [L,64] f32, nu [L], nv [L])."""
+    cp = np.ascontiguousarray(cp, np.float32)
+    vp = np.ascontiguousarray(vp, np.float32)
+    L = lib()
+    n = L.oracle_reyes_split(_p(cp), cp.shape[0], W, H, _p(vp), threshold, max_splits, None, None, None, 0)
+    leaves = np.empty((max(n, 1), 64), np.float32)
+    nu = np.empty(max(n, 1), np.int32)
+    nv = np.empty(max(n, 1), np.int32)
+    L.oracle_reyes_split(_p(cp), cp.shape[0], W, H, _p(vp), threshold, max_splits, _p(leaves), _p(nu), _p(nv), n)
+    return leaves[:n], nu[:n], nv[:n]
+
+
+def reyes_dice(leaves, nu, nv, dice, W, H, vp, disp):
+    leaves = np.ascontiguousarray(leaves, np.float32)
+    nu = np.ascontiguousarray(nu, np.int32)
+    nv = np.ascontiguousarray(nv, np.int32)
+    vp = np.ascontiguousarray(vp, np.float32)
+    L = lib()
+    n = L.oracle_reyes_dice(_p(leaves), _p(nu), _p(nv), leaves.shape[0], dice, W, H, _p(vp), disp, None, 0)
+    out = np.empty(max(n, 1), UPOLY)
+    L.oracle_reyes_dice(_p(leaves), _p(nu), _p(nv), leaves.shape[0], dice, W, H, _p(vp), disp, _p(out), n)
+    return out[:n]
+
+
+def reyes_shade(upolys, W, H, spp):
+    """(rgba [H,W,4] f32, sample z bits [H,W,spp] i32)."""
+    upolys = np.ascontiguousarray(upolys, UPOLY)
+    rgba = np.empty((H, W, 4), np.float32)
+    z = np.empty((H, W, spp), np.int32)
+    lib().oracle_reyes_shade(_p(upolys), upolys.shape[0], W, H, spp, _p(rgba), _p(z))
+    return rgba, z
+
+
+def reyes_frame(scene):
+    """Whole Reyes frame of a PatchScene: (rgba [H,W,4], sample z bits, micropolygons)."""
+    cp = np.ascontiguousarray(scene.cp, np.float32)
+    vp = np.ascontiguousarray(scene.view_proj, np.float32)
+    rgba = np.empty((scene.H, scene.W, 4), np.float32)
+    z = np.empty((scene.H, scene.W, scene.spp), np.int32)
+    nup = lib().oracle_reyes_frame(_p(cp), cp.shape[0], scene.W, scene.H, _p(vp), scene.threshold, scene.dice,
+                                   scene.spp, scene.disp, _p(rgba), _p(z))
+    return rgba, z, nup

@@ -106,6 +106,42 @@ uint32_t oracle_to_abgr(float r, float g, float b);       /* math.h:286-289 */
 void oracle_from_abgr(uint32_t c, float *rgb);             /* math.h:323-330 */
 int32_t oracle_fixpt(float v, int S);                      /* vertexShader:46 */
 
+/* ---- Reyes (reyes_oracle.c): Split -> Dice -> Shade ------------------- */
+
+/* Micropolygon after dice + shade set-up: screen corners in the order
+ * p0 (u,v), p1 (u+du,v), p2 (u+du,v+dv), p3 (u,v+dv) (dice.pikostage:
+ * 206-295), flat depth = mean corner z (shade.pikostage:98) and flat colour
+ * = clamped mean of the four Phong colours (shade.pikostage:108-138). */
+typedef struct {
+  float x[4], y[4], z;
+  float rgb[3];
+} oracle_upoly;
+
+/* Split stage (split.pikostage:240-342) run level by level.  patches: [np*64]
+ * control points (16 x xyzw).  Leaves in (level, position) order are written
+ * to leaves [cap*64] with the reference's nu/nv when the pointers are non-null.
+ * Returns the leaf count (also when it exceeds cap: call with cap=0 to size). */
+long long oracle_reyes_split(const float *patches, int np, int W, int H, const float *vp, float threshold,
+                             int max_splits, float *leaves, int *leaf_nu, int *leaf_nv, long long cap);
+
+/* Dice stage (dice.pikostage:159-312).  dice=0: the reference's (nu+1)(nv+1)
+ * grid from leaf_nu/leaf_nv; dice=N>0: N x N quads covering the leaf.
+ * disp: displacement disp*(x^2-y^2) along the unit normal.  Returns the
+ * micropolygon count (writes at most cap). */
+long long oracle_reyes_dice(const float *leaves, const int *leaf_nu, const int *leaf_nv, long long nleaf, int dice,
+                            int W, int H, const float *vp, float disp, oracle_upoly *out, long long cap);
+
+/* Shade stage sampling (shade.pikostage:88-257) in micropolygon order, spp
+ * in {1, 4} (pixel centre, or the centres of the 2x2 strata), resolved by a
+ * box filter to rgba [W*H*4] FP32; sample_z [W*H*spp] (float bits) if
+ * non-null. */
+void oracle_reyes_shade(const oracle_upoly *up, long long nup, int W, int H, int spp, float *rgba,
+                        int32_t *sample_z);
+
+/* Split + dice + shade.  Returns the micropolygon count. */
+long long oracle_reyes_frame(const float *patches, int np, int W, int H, const float *vp, float threshold,
+                             int dice, int spp, float disp, float *rgba, int32_t *sample_z);
+
 #ifdef __cplusplus
 }
 #endif

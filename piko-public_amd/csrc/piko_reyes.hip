@@ -1,0 +1,940 @@
+// piko_reyes.hip -- the Reyes pipeline (Split -> Dice -> Shade -> PikoScreen)
+// on gfx950 (reyesPipe/reyesPipe.h:12-40, split/dice/shade.pikostage).
+//
+// Frame = four launches on the pipe's stream, no host round trip:
+//   k_reyes_split   one wave walks the split trees of a contiguous run of
+//                   input patches level by level (16 lanes per patch, one per
+//                   control point, 4 patches per pass).  Its frontier lives
+//                   in a per-wave region; leaves go to a per-wave region in
+//                   (level, position) order with per-level counts.
+//   k_reyes_scan    one block: exclusive scan of the per-(level, wave) leaf
+//                   and quad counts in level-major order.  Because each wave
+//                   owns a contiguous run of roots, a leaf's index in the
+//                   split stage's emission order (level by level, in order
+//                   within a level: the reference's split self-loop, CPU
+//                   backend one launch per level) is
+//                   offset[level][wave] + its position in the wave's list.
+//   k_reyes_dice    one wave per leaf (persistent grid): per-u Bernstein rows,
+//                   the (g+1)^2 grid points once each (position, normal,
+//                   displacement, projection, Phong colour) into LDS, then one
+//                   lane per micropolygon: flat depth and colour, and the
+//                   sample tests with a 64-bit atomicMin of (depth, reversed
+//                   emission order) per sample, so the nearest micropolygon
+//                   wins and on equal depth the later one -- the outcome of
+//                   the reference's atomicMin loop + (zBuffer == z) test run
+//                   in emission order (shade.pikostage:226-241).
+//   k_reyes_resolve one thread per pixel: box filter of the samples' colours
+//                   to FP32 RGBA, nearest depth, and the visibility buffer
+//                   reset to empty for the next frame.
+// All floating-point expressions keep the reference's operation order and
+// are compiled without contraction (-ffp-contract=off).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "piko_hip.h"
+
+namespace {
+
+constexpr int kLevels = 21;  // levels 0..20: a patch splits while numSplits < 20 (split.pikostage:284)
+constexpr int kSplitCapMax = 20;
+constexpr unsigned long long kVisEmpty = ((unsigned long long)(0x3F800000u ^ 0x80000000u) << 32) | 0xFFFFFFFFull;
+constexpr unsigned long long kMaxOrder = 0xFFFFFFFEull;  // quads per frame must stay below this
+
+struct ReyesState {
+  unsigned long long nleaf;
+  unsigned long long nquad;
+  unsigned int overflow;  // bit 0: a split frontier, bit 1: a leaf region, bit 2: quad colours
+  unsigned int levels;
+  unsigned int pad[2];
+};
+
+struct ReyesArgs {
+  int W, H;
+  float vp[16];
+  float threshold;
+  int max_splits, dice, spp;
+  float disp;
+  int roots_per_wave, nroots, G;  // G: split waves
+  int front_cap, leaf_cap;        // per split wave: nodes per level buffer, leaves
+  unsigned long long qcap;        // quad colour slots
+  int keep_samples;
+};
+
+// piko::fminf / fmaxf of the CPU backend are std::min / std::max
+// (math.h:24-51): the first argument unless the second compares smaller /
+// larger.
+__device__ __forceinline__ float fmn(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float fmx(float a, float b) { return (a < b) ? b : a; }
+
+__device__ __forceinline__ float4 f4add(float4 a, float4 b) {
+  return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+__device__ __forceinline__ float4 f4half(float4 a) {
+  return make_float4(a.x * 0.5f, a.y * 0.5f, a.z * 0.5f, a.w * 0.5f);
+}
+__device__ __forceinline__ float4 shfl4(float4 v, int src) {
+  return make_float4(__shfl(v.x, src, 16), __shfl(v.y, src, 16), __shfl(v.z, src, 16), __shfl(v.w, src, 16));
+}
+
+// (int)ceil(ext / threshold) of split.pikostage:307-308 made total: 64 and
+// above give the same clamped grid as any larger value, NaN gives 0.
+__device__ __forceinline__ int ext_steps(float ext, float thr) {
+  const double q = ceil((double)(ext / thr));
+  return q >= 64.0 ? 64 : (q >= 0.0 ? (int)q : 0);
+}
+
+__device__ __forceinline__ int pow2roundup(int x) {  // split.pikostage:34-45
+  if (x < 0) return 0;
+  --x;
+  x |= x >> 1;
+  x |= x >> 2;
+  x |= x >> 4;
+  x |= x >> 8;
+  x |= x >> 16;
+  return x + 1;
+}
+
+__device__ __forceinline__ int grid_steps(float ext, float thr) {  // split.pikostage:307-314
+  int n = pow2roundup(ext_steps(ext, thr) + 1);
+  return n < 2 ? 2 : (n > 32 ? 32 : n);
+}
+
+// ---------------------------------------------------------------------------
+// Split.  Lane l of the wave holds control point (l & 15) of node slot l >> 4.
+
+__global__ __launch_bounds__(64) void k_reyes_split(const float4 *__restrict__ roots, ReyesArgs a,
+                                                    float4 *__restrict__ front, float4 *__restrict__ lcp,
+                                                    uint32_t *__restrict__ lmeta, uint32_t *__restrict__ lqloc,
+                                                    uint32_t *__restrict__ cnt, uint32_t *__restrict__ qcnt,
+                                                    uint32_t *__restrict__ lvlpre, uint32_t *__restrict__ wstat) {
+  const int w = blockIdx.x, lane = threadIdx.x;
+  const int slot = lane >> 4, i = lane & 15, r = i >> 2, c = i & 3;
+  const int G = a.G;
+  const int r0 = w * a.roots_per_wave;
+  int n = min(r0 + a.roots_per_wave, a.nroots) - r0;
+  float4 *fb0 = front + (size_t)w * 2 * a.front_cap * 16;
+  float4 *fb1 = fb0 + (size_t)a.front_cap * 16;
+  const float4 *fcur = roots + (size_t)r0 * 16;
+  float4 *fnext = fb0;
+  float4 *leaf = lcp + (size_t)w * a.leaf_cap * 16;
+  uint32_t *meta = lmeta + (size_t)w * a.leaf_cap;
+  uint32_t *qloc = lqloc + (size_t)w * a.leaf_cap;
+  const float *m = a.vp;
+  uint32_t nleaves = 0, status = 0;
+  int L = 0;
+  for (; L < kLevels; ++L) {
+    const uint32_t lvl0 = nleaves;
+    uint32_t qrun = 0;
+    int nn = 0;
+    for (int base = 0; base < n && !status; base += 4) {
+      const int node = base + slot;
+      const bool valid = node < n;
+      const float4 p = valid ? fcur[(size_t)node * 16 + i] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+      // SplitStage::process projection (split.pikostage:245-268)
+      float cx = ((m[0] * p.x + m[4] * p.y) + m[8] * p.z) + m[12] * p.w;
+      float cy = ((m[1] * p.x + m[5] * p.y) + m[9] * p.z) + m[13] * p.w;
+      float cw = ((m[3] * p.x + m[7] * p.y) + m[11] * p.z) + m[15] * p.w;
+      if (cw == 0.0f) cw = 1.0f;
+      cx = cx / cw;
+      cy = cy / cw;
+      const float X = (float)(((double)cx + 1.0) * 0.5 * (double)a.W);
+      const float Y = (float)(((double)cy + 1.0) * 0.5 * (double)a.H);
+      // getUVPixelExtents (split.pikostage:47-94): longest row / column
+      // control polyline on screen.
+      float nx = __shfl(X, (i + 1) & 15, 16), ny = __shfl(Y, (i + 1) & 15, 16);
+      float dx = X - nx, dy = Y - ny;
+      const float du = sqrtf(dx * dx + dy * dy);
+      const float rows = (__shfl(du, 4 * r, 16) + __shfl(du, 4 * r + 1, 16)) + __shfl(du, 4 * r + 2, 16);
+      nx = __shfl(X, (i + 4) & 15, 16);
+      ny = __shfl(Y, (i + 4) & 15, 16);
+      dx = X - nx;
+      dy = Y - ny;
+      const float dv = sqrtf(dx * dx + dy * dy);
+      const float cols = (__shfl(dv, c, 16) + __shfl(dv, c + 4, 16)) + __shfl(dv, c + 8, 16);
+      float ue = 0.0f, ve = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) ue = fmx(ue, __shfl(rows, 4 * k, 16));
+#pragma unroll
+      for (int k = 0; k < 4; ++k) ve = fmx(ve, __shfl(cols, k, 16));
+      const bool dirU = ue >= a.threshold;
+      const bool split = valid && (dirU || ve >= a.threshold) && L < a.max_splits;
+      const bool isleaf = valid && !split;
+      const int gu = grid_steps(ue, a.threshold), gv = grid_steps(ve, a.threshold);
+      const uint32_t nq = !isleaf ? 0u : (a.dice > 0 ? (uint32_t)(a.dice * a.dice) : (uint32_t)((gu + 1) * (gv + 1)));
+      // positions: children and leaves of the four slots in slot order
+      int pch = 0, plf = 0, tch = 0, tlf = 0;
+      uint32_t pq = 0, tq = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int ck = __shfl(split ? 2 : 0, 16 * k), lk = __shfl(isleaf ? 1 : 0, 16 * k);
+        const uint32_t qk = __shfl(nq, 16 * k);
+        if (k < slot) {
+          pch += ck;
+          plf += lk;
+          pq += qk;
+        }
+        tch += ck;
+        tlf += lk;
+        tq += qk;
+      }
+      if (nn + tch > a.front_cap) status |= 1u;
+      if ((int)(nleaves + tlf) > a.leaf_cap) status |= 2u;
+      if (status) break;  // uniform across the wave
+      // splitU / splitV (split.pikostage:95-224): midpoint de Casteljau on
+      // the row (U) or column (V) through this lane's control point.
+      const int s0 = dirU ? 4 * r : c, st = dirU ? 1 : 4, pos = dirU ? c : r;
+      const float4 a0 = shfl4(p, s0), a1 = shfl4(p, s0 + st), a2 = shfl4(p, s0 + 2 * st), a3 = shfl4(p, s0 + 3 * st);
+      if (split) {
+        const float4 n11 = f4half(f4add(a0, a1)), n22 = f4half(f4add(a2, a3)), tmp = f4half(f4add(a1, a2));
+        const float4 n12 = f4half(f4add(n11, tmp)), n21 = f4half(f4add(n22, tmp));
+        const float4 n13 = f4half(f4add(n12, n21));
+        const float4 c1 = pos == 0 ? a0 : (pos == 1 ? n11 : (pos == 2 ? n12 : n13));
+        const float4 c2 = pos == 0 ? n13 : (pos == 1 ? n21 : (pos == 2 ? n22 : a3));
+        fnext[(size_t)(nn + pch) * 16 + i] = c1;
+        fnext[(size_t)(nn + pch + 1) * 16 + i] = c2;
+      }
+      if (isleaf) {
+        const uint32_t at = nleaves + plf;
+        leaf[(size_t)at * 16 + i] = p;
+        if (i == 0) {
+          meta[at] = (uint32_t)gu | ((uint32_t)gv << 8);
+          qloc[at] = qrun + pq;
+        }
+      }
+      nn += tch;
+      nleaves += tlf;
+      qrun += tq;
+    }
+    if (lane == 0) {
+      cnt[L * G + w] = nleaves - lvl0;
+      qcnt[L * G + w] = qrun;
+      lvlpre[L * G + w] = lvl0;
+    }
+    if (status || nn == 0) break;
+    __syncthreads();  // the next level list is read by other lanes
+    fcur = fnext;
+    fnext = (fnext == fb0) ? fb1 : fb0;
+    n = nn;
+  }
+  if (lane == 0) {
+    for (int l = L + 1; l < kLevels; ++l) {
+      cnt[l * G + w] = 0;
+      qcnt[l * G + w] = 0;
+      lvlpre[l * G + w] = nleaves;
+    }
+    wstat[w] = status | ((uint32_t)(L + 1) << 8);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Scan of the (level, wave) counts, level-major; frame totals and status.
+
+constexpr int kScanThreads = 1024;
+
+__global__ __launch_bounds__(kScanThreads) void k_reyes_scan(const uint32_t *__restrict__ cnt,
+                                                             const uint32_t *__restrict__ qcnt,
+                                                             const uint32_t *__restrict__ wstat, int G,
+                                                             unsigned long long qcap, uint32_t *__restrict__ off,
+                                                             unsigned long long *__restrict__ qoff,
+                                                             ReyesState *__restrict__ st) {
+  __shared__ unsigned long long s_a[kScanThreads], s_b[kScanThreads];
+  __shared__ uint32_t s_or, s_lv;
+  const int t = threadIdx.x;
+  const int NL = kLevels * G;
+  const int per = (NL + kScanThreads - 1) / kScanThreads;
+  const int lo = min(t * per, NL), hi = min(lo + per, NL);
+  unsigned long long sa = 0, sb = 0;
+  for (int k = lo; k < hi; ++k) {
+    sa += cnt[k];
+    sb += qcnt[k];
+  }
+  if (t == 0) {
+    s_or = 0;
+    s_lv = 0;
+  }
+  s_a[t] = sa;
+  s_b[t] = sb;
+  __syncthreads();
+  for (int d = 1; d < kScanThreads; d <<= 1) {  // inclusive Hillis-Steele scan
+    const unsigned long long xa = t >= d ? s_a[t - d] : 0, xb = t >= d ? s_b[t - d] : 0;
+    __syncthreads();
+    s_a[t] += xa;
+    s_b[t] += xb;
+    __syncthreads();
+  }
+  unsigned long long ra = s_a[t] - sa, rb = s_b[t] - sb;
+  for (int k = lo; k < hi; ++k) {
+    off[k] = (uint32_t)ra;
+    qoff[k] = rb;
+    ra += cnt[k];
+    rb += qcnt[k];
+  }
+  uint32_t ovf = 0, lv = 0;
+  for (int k = t; k < G; k += kScanThreads) {
+    ovf |= wstat[k] & 3u;
+    lv = max(lv, wstat[k] >> 8);
+  }
+  atomicOr(&s_or, ovf);
+  atomicMax(&s_lv, lv);
+  __syncthreads();
+  if (t == 0) {
+    const unsigned long long nleaf = s_a[kScanThreads - 1], nquad = s_b[kScanThreads - 1];
+    st->nleaf = nleaf;
+    st->nquad = nquad;
+    st->overflow = s_or | ((nquad > qcap || nquad >= kMaxOrder || nleaf >= 0xFFFFFFFFull) ? 4u : 0u);
+    st->levels = s_lv;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Dice + sample.
+
+// Bernstein basis and derivatives (dice.pikostage:85-93).
+__device__ __forceinline__ void bern(float u, float *b, float *d) {
+  b[0] = ((1.0f - u) * (1.0f - u) * (1.0f - u));
+  b[1] = ((3.0f * u) * (1.0f - u) * (1.0f - u));
+  b[2] = ((3.0f * u) * (u - u * u));
+  b[3] = ((u) * (u * u));
+  d[0] = (-3.0f * (1.0f - u) * (1.0f - u));
+  d[1] = (3.0f * (1.0f - u) * (1.0f - 3.0f * u));
+  d[2] = (3.0f * u * (2.0f - 3.0f * u));
+  d[3] = (3.0f * u * u);
+}
+
+// bez2D / dbez2D (dice.pikostage:96-109) with the basis values given.
+__device__ __forceinline__ float4 comb(const float *B, float4 v0, float4 v1, float4 v2, float4 v3) {
+  return make_float4(B[0] * v0.x + B[1] * v1.x + B[2] * v2.x + B[3] * v3.x,
+                     B[0] * v0.y + B[1] * v1.y + B[2] * v2.y + B[3] * v3.y,
+                     B[0] * v0.z + B[1] * v1.z + B[2] * v2.z + B[3] * v3.z,
+                     B[0] * v0.w + B[1] * v1.w + B[2] * v2.w + B[3] * v3.w);
+}
+
+// phongShadeReyes + computeLighting (shade.pikostage:22-30, 64-85): the call
+// passes (normal, matcol, lightvec) into the macro's (_mynor, _lightvec,
+// _matcol), so the material colour acts as the light vector and vice versa.
+__device__ __forceinline__ float sat(float p) { return p > 1.0f ? 1.0f : (p < 0.0f ? 0.0f : p); }
+
+// The sample position of sample s (pixel centre, or the centres of the 2x2
+// strata in x-fastest order).
+__device__ __forceinline__ float sample_off(int spp, int s, int axis) {
+  if (spp == 1) return 0.5f;
+  return ((s >> axis) & 1) ? 0.75f : 0.25f;
+}
+
+__device__ __forceinline__ int clampi(float f) {
+  if (!(f > -1073741824.0f)) return -1073741824;
+  if (f > 1073741824.0f) return 1073741824;
+  return (int)f;
+}
+
+template <int MAXP>
+__global__ __launch_bounds__(64) void k_reyes_dice(ReyesArgs a, const float4 *__restrict__ lcp,
+                                                   const uint32_t *__restrict__ lmeta,
+                                                   const uint32_t *__restrict__ lqloc,
+                                                   const uint32_t *__restrict__ off,
+                                                   const unsigned long long *__restrict__ qoff,
+                                                   const uint32_t *__restrict__ lvlpre,
+                                                   const ReyesState *__restrict__ st, float *__restrict__ qcol,
+                                                   unsigned long long *__restrict__ vis) {
+  __shared__ float4 s_cp[16];
+  __shared__ float4 s_row[MAXP][8];  // per grid u: bez2D rows 0..3, dbez2D rows 4..7
+  __shared__ float s_pt[MAXP * MAXP][6];
+  const int lane = threadIdx.x;
+  const unsigned long long nleaf = st->nleaf;
+  const int NL = kLevels * a.G;
+  const float *m = a.vp;
+  const int W = a.W, H = a.H, spp = a.spp;
+  const int nbx = (W + 63) / 64, nby = (H + 63) / 64;
+  for (unsigned long long g = blockIdx.x; g < nleaf; g += gridDim.x) {
+    int lo = 0, hi = NL;  // first (level, wave) group whose offset exceeds g
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if ((unsigned long long)off[mid] <= g) lo = mid + 1;
+      else hi = mid;
+    }
+    const int grp = lo - 1, w = grp % a.G;
+    const size_t at = (size_t)w * a.leaf_cap + lvlpre[grp] + (uint32_t)(g - off[grp]);
+    if (lane < 16) s_cp[lane] = lcp[at * 16 + lane];
+    const uint32_t mt = lmeta[at];
+    const unsigned long long qbase = qoff[grp] + lqloc[at];
+    int nu, nv, gu, gv;  // dice steps; micropolygons along u and v
+    if (a.dice > 0) {
+      nu = nv = gu = gv = a.dice;
+    } else {
+      nu = (int)(mt & 0xffu);
+      nv = (int)((mt >> 8) & 0xffu);
+      gu = nu + 1;  // k <= nu (dice.pikostage:176-177)
+      gv = nv + 1;
+    }
+    const float du = 1.0f / ((float)nu), dv = 1.0f / ((float)nv);
+    __syncthreads();
+    // grid u values: myU = k*du for k < gu, and the last quad's myU + du
+    // (dice.pikostage:180, 233); with power-of-two steps these coincide with
+    // the neighbouring quads' corners bit for bit.
+    for (int iu = lane; iu <= gu; iu += 64) {
+      const float u = iu < gu ? (float)iu * du : (float)(gu - 1) * du + du;
+      float B[4], D[4];
+      bern(u, B, D);
+#pragma unroll
+      for (int row = 0; row < 4; ++row) {
+        s_row[iu][row] = comb(B, s_cp[4 * row], s_cp[4 * row + 1], s_cp[4 * row + 2], s_cp[4 * row + 3]);
+        s_row[iu][4 + row] = comb(D, s_cp[4 * row], s_cp[4 * row + 1], s_cp[4 * row + 2], s_cp[4 * row + 3]);
+      }
+    }
+    __syncthreads();
+    const int np = (gu + 1) * (gv + 1);
+    for (int pt = lane; pt < np; pt += 64) {
+      const int iu = pt / (gv + 1), jv = pt - iu * (gv + 1);
+      const float v = jv < gv ? (float)jv * dv : (float)(gv - 1) * dv + dv;
+      float B[4], D[4];
+      bern(v, B, D);
+      // patchEval (dice.pikostage:111-121), normEval (123-146)
+      float4 P = comb(B, s_row[iu][0], s_row[iu][1], s_row[iu][2], s_row[iu][3]);
+      const float4 A = comb(B, s_row[iu][4], s_row[iu][5], s_row[iu][6], s_row[iu][7]);
+      const float4 Bv = comb(D, s_row[iu][0], s_row[iu][1], s_row[iu][2], s_row[iu][3]);
+      float nx = A.y * Bv.z - A.z * Bv.y, ny = A.z * Bv.x - A.x * Bv.z, nz = A.x * Bv.y - A.y * Bv.x;
+      const float mag = sqrtf(nx * nx + ny * ny + nz * nz);
+      nx /= mag;
+      ny /= mag;
+      nz /= mag;
+      if (a.disp != 0.0f) {
+        const float d = a.disp * (P.x * P.x - P.y * P.y);
+        P.x = P.x + d * nx;
+        P.y = P.y + d * ny;
+        P.z = P.z + d * nz;
+      }
+      // projection (dice.pikostage:209-216)
+      const float sx = ((m[0] * P.x + m[4] * P.y) + m[8] * P.z) + m[12] * P.w;
+      const float sy = ((m[1] * P.x + m[5] * P.y) + m[9] * P.z) + m[13] * P.w;
+      const float sz = ((m[2] * P.x + m[6] * P.y) + m[10] * P.z) + m[14] * P.w;
+      const float sw = ((m[3] * P.x + m[7] * P.y) + m[11] * P.z) + m[15] * P.w;
+      const float onebyw = 1.0f / sw;
+      float *o = s_pt[pt];
+      o[0] = ((sx * onebyw + 1.0f) * (float)W) * 0.5f;
+      o[1] = ((sy * onebyw + 1.0f) * (float)H) * 0.5f;
+      o[2] = (sz * onebyw);
+      const float ix = -nx, iy = -ny, iz = -nz;
+      float dif = ix * 0.10f + iy * 0.35f + iz * 0.85f;
+      dif = dif < 0.0f ? 0.0f : dif;
+      o[3] = sat(0.2f + dif * 0.1773f);
+      o[4] = sat(0.2f + dif * 0.5773f);
+      o[5] = sat(0.2f + dif * 0.973f);
+    }
+    __syncthreads();
+    const int nq = gu * gv;
+    for (int qd = lane; qd < nq; qd += 64) {
+      const int k = qd / gv, j = qd - k * gv;
+      const float *c0 = s_pt[k * (gv + 1) + j], *c1 = s_pt[(k + 1) * (gv + 1) + j];
+      const float *c2 = s_pt[(k + 1) * (gv + 1) + j + 1], *c3 = s_pt[k * (gv + 1) + j + 1];
+      const float X[4] = {c0[0], c1[0], c2[0], c3[0]}, Y[4] = {c0[1], c1[1], c2[1], c3[1]};
+      const float z = (((c0[2] + c1[2]) + c2[2]) + c3[2]) * 0.25f;  // shade.pikostage:98
+      float rgb[3];
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch)  // shade.pikostage:130-138
+        rgb[ch] = fmn(((c0[3 + ch] + c1[3 + ch]) + c2[3 + ch]) + c3[3 + ch], 4.0f) * 0.25f;
+      const unsigned long long order = qbase + (unsigned long long)qd;
+      if (order >= a.qcap) continue;  // frame is re-run with room (status bit 2)
+      qcol[order * 3 + 0] = rgb[0];
+      qcol[order * 3 + 1] = rgb[1];
+      qcol[order * 3 + 2] = rgb[2];
+      // pixels tested: ShadeStage bins by the box of the first three corners
+      // (computeBoundingBox, dice.pikostage:18-43) into 64-px bins and tests
+      // the four-corner box inside each bin (shade.pikostage:140-163).
+      const float lo3x = floorf(fmn(X[0], fmn(X[1], X[2]))), lo3y = floorf(fmn(Y[0], fmn(Y[1], Y[2])));
+      const float hi3x = ceilf(fmx(X[0], fmx(X[1], X[2]))), hi3y = ceilf(fmx(Y[0], fmx(Y[1], Y[2])));
+      int bx1 = clampi(floorf(lo3x / 64.0f)), bx2 = clampi(floorf(hi3x / 64.0f));
+      int by1 = clampi(floorf(lo3y / 64.0f)), by2 = clampi(floorf(hi3y / 64.0f));
+      bx1 = max(bx1, 0);
+      by1 = max(by1, 0);
+      bx2 = min(bx2, nbx - 1);
+      by2 = min(by2, nby - 1);
+      if (bx1 > bx2 || by1 > by2) continue;
+      int x0 = clampi(floorf(fmn(fmn(X[0], X[1]), fmn(X[2], X[3]))));
+      int y0 = clampi(floorf(fmn(fmn(Y[0], Y[1]), fmn(Y[2], Y[3]))));
+      int x1 = clampi(ceilf(fmx(fmx(X[0], X[1]), fmx(X[2], X[3]))));
+      int y1 = clampi(ceilf(fmx(fmx(Y[0], Y[1]), fmx(Y[2], Y[3]))));
+      x0 = max(x0, bx1 * 64);
+      y0 = max(y0, by1 * 64);
+      x1 = min(min(x1, (bx2 + 1) * 64), W);
+      y1 = min(min(y1, (by2 + 1) * 64), H);
+      const uint32_t zb = __float_as_uint(z) ^ 0x80000000u;
+      const unsigned long long key = ((unsigned long long)zb << 32) | (unsigned long long)(0xFFFFFFFEu - (uint32_t)order);
+      for (int y = y0; y < y1; ++y)
+        for (int x = x0; x < x1; ++x)
+          for (int s = 0; s < spp; ++s) {
+            const float px = (float)x + sample_off(spp, s, 0), py = (float)y + sample_off(spp, s, 1);
+            // two 4-edge fan tests (shade.pikostage:176-218)
+            bool r1 = true, r2 = true;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int ea = e, eb = (e + 1) & 3;
+              const float t = ((py - Y[ea]) * (X[eb] - X[ea]) - (px - X[ea]) * (Y[eb] - Y[ea]));
+              if (t <= 0.0f) r1 = false;
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int ea = (4 - e) & 3, eb = (3 - e) & 3;
+              const float t = ((py - Y[ea]) * (X[eb] - X[ea]) - (px - X[ea]) * (Y[eb] - Y[ea]));
+              if (t <= 0.0f) r2 = false;
+            }
+            if (!(r1 || r2)) continue;
+            unsigned long long *dst = vis + ((size_t)y * W + x) * spp + s;
+            if (key < *dst) atomicMin(dst, key);
+          }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Resolve: box filter, nearest depth, visibility reset.
+
+__global__ __launch_bounds__(256) void k_reyes_resolve(ReyesArgs a, unsigned long long *__restrict__ vis,
+                                                       const float *__restrict__ qcol, float4 *__restrict__ rgba,
+                                                       float *__restrict__ depth, int32_t *__restrict__ sz) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (long long)a.W * a.H) return;
+  const int spp = a.spp;
+  // PikoScreen clear colour 0xff663313 through fromABGR (math.h:323-330)
+  const float clr[3] = {(float)0x13 / 255.0f, (float)0x33 / 255.0f, (float)0x66 / 255.0f};
+  float acc[3] = {0.0f, 0.0f, 0.0f};
+  int32_t zmin = 0;
+  for (int s = 0; s < spp; ++s) {
+    const size_t si = (size_t)p * spp + s;
+    const unsigned long long key = vis[si];
+    const uint32_t low = (uint32_t)key;
+    float c[3];
+    int32_t zi;
+    if (low == 0xFFFFFFFFu) {
+      c[0] = clr[0];
+      c[1] = clr[1];
+      c[2] = clr[2];
+      zi = 0x3F800000;
+    } else {
+      const unsigned long long order = 0xFFFFFFFEull - low;
+      c[0] = qcol[order * 3 + 0];
+      c[1] = qcol[order * 3 + 1];
+      c[2] = qcol[order * 3 + 2];
+      zi = (int32_t)((uint32_t)(key >> 32) ^ 0x80000000u);
+    }
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) acc[ch] = s == 0 ? c[ch] : acc[ch] + c[ch];
+    zmin = (s == 0 || zi < zmin) ? zi : zmin;
+    if (a.keep_samples) sz[si] = zi;
+    vis[si] = kVisEmpty;
+  }
+  const float inv = 1.0f / (float)spp;
+  rgba[p] = make_float4(acc[0] * inv, acc[1] * inv, acc[2] * inv, 1.0f);
+  depth[p] = __int_as_float(zmin);
+}
+
+__global__ void k_fill_u64(unsigned long long *p, long long n, unsigned long long v) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+    p[k] = v;
+}
+
+inline long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+struct piko_reyes_pipe {
+  int W = 0, H = 0, P = 0;
+  piko_reyes_options opt{};
+  ReyesArgs ra{};
+  hipStream_t stream = nullptr;
+  const float4 *d_cp = nullptr;
+  bool owns_input = false;
+  float4 *d_front = nullptr, *d_lcp = nullptr;
+  uint32_t *d_lmeta = nullptr, *d_lqloc = nullptr;
+  uint32_t *d_cnt = nullptr, *d_qcnt = nullptr, *d_lvlpre = nullptr, *d_off = nullptr, *d_wstat = nullptr;
+  unsigned long long *d_qoff = nullptr;
+  ReyesState *d_st = nullptr;
+  float *d_qcol = nullptr;
+  unsigned long long *d_vis = nullptr;
+  float4 *d_rgba = nullptr;
+  float *d_depth = nullptr;
+  int32_t *d_sz = nullptr;
+  int cus = 256, dice_grid = 0;
+  int reruns = 0;  // capacity re-runs of the last run call
+  long long device_bytes = 0;
+  long long front_bytes = 0, leaf_bytes = 0, qcol_bytes = 0;
+};
+
+namespace {
+
+#define HIPOK(x)                                \
+  do {                                          \
+    if ((x) != hipSuccess) return PIKO_ERR_HIP; \
+  } while (0)
+
+template <typename T>
+int ralloc(piko_reyes_pipe *p, T **ptr, long long elems, long long *acct = nullptr) {
+  const size_t bytes = (size_t)std::max<long long>(elems, 1) * sizeof(T);
+  if (hipMalloc((void **)ptr, bytes) != hipSuccess) {
+    *ptr = nullptr;
+    return PIKO_ERR_OOM;
+  }
+  p->device_bytes += (long long)bytes;
+  if (acct) *acct += (long long)bytes;
+  return PIKO_OK;
+}
+
+template <typename T>
+void rfree(T *&ptr) {
+  if (ptr) (void)hipFree((void *)ptr);
+  ptr = nullptr;
+}
+
+void free_front(piko_reyes_pipe *p) {
+  rfree(p->d_front);
+  p->device_bytes -= p->front_bytes;
+  p->front_bytes = 0;
+}
+
+int alloc_front(piko_reyes_pipe *p) {
+  return ralloc(p, &p->d_front, (long long)p->ra.G * 2 * p->ra.front_cap * 16, &p->front_bytes);
+}
+
+void free_leaves(piko_reyes_pipe *p) {
+  rfree(p->d_lcp);
+  rfree(p->d_lmeta);
+  rfree(p->d_lqloc);
+  p->device_bytes -= p->leaf_bytes;
+  p->leaf_bytes = 0;
+}
+
+int alloc_leaves(piko_reyes_pipe *p) {
+  const long long n = (long long)p->ra.G * p->ra.leaf_cap;
+  int rc = ralloc(p, &p->d_lcp, n * 16, &p->leaf_bytes);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->d_lmeta, n, &p->leaf_bytes);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->d_lqloc, n, &p->leaf_bytes);
+  return rc;
+}
+
+void free_qcol(piko_reyes_pipe *p) {
+  rfree(p->d_qcol);
+  p->device_bytes -= p->qcol_bytes;
+  p->qcol_bytes = 0;
+}
+
+int alloc_qcol(piko_reyes_pipe *p) { return ralloc(p, &p->d_qcol, (long long)p->ra.qcap * 3, &p->qcol_bytes); }
+
+const char *kReyesKernels[] = {"k_reyes_split", "k_reyes_scan", "k_reyes_dice", "k_reyes_resolve"};
+
+int enqueue_frame(piko_reyes_pipe *p, hipEvent_t *ev) {
+  const ReyesArgs &a = p->ra;
+  const int NL = kLevels * a.G;
+  (void)NL;
+  k_reyes_split<<<a.G, 64, 0, p->stream>>>(p->d_cp, a, p->d_front, p->d_lcp, p->d_lmeta, p->d_lqloc, p->d_cnt,
+                                           p->d_qcnt, p->d_lvlpre, p->d_wstat);
+  if (ev) (void)hipEventRecord(ev[0], p->stream);
+  k_reyes_scan<<<1, kScanThreads, 0, p->stream>>>(p->d_cnt, p->d_qcnt, p->d_wstat, a.G, a.qcap, p->d_off,
+                                                  p->d_qoff, p->d_st);
+  if (ev) (void)hipEventRecord(ev[1], p->stream);
+  if (a.dice > 0 && a.dice <= 8)
+    k_reyes_dice<9><<<p->dice_grid, 64, 0, p->stream>>>(a, p->d_lcp, p->d_lmeta, p->d_lqloc, p->d_off, p->d_qoff,
+                                                        p->d_lvlpre, p->d_st, p->d_qcol, p->d_vis);
+  else
+    k_reyes_dice<34><<<p->dice_grid, 64, 0, p->stream>>>(a, p->d_lcp, p->d_lmeta, p->d_lqloc, p->d_off, p->d_qoff,
+                                                         p->d_lvlpre, p->d_st, p->d_qcol, p->d_vis);
+  if (ev) (void)hipEventRecord(ev[2], p->stream);
+  k_reyes_resolve<<<(unsigned)cdiv((long long)a.W * a.H, 256), 256, 0, p->stream>>>(a, p->d_vis, p->d_qcol,
+                                                                                    p->d_rgba, p->d_depth, p->d_sz);
+  if (ev) (void)hipEventRecord(ev[3], p->stream);
+  return hipGetLastError() == hipSuccess ? PIKO_OK : PIKO_ERR_HIP;
+}
+
+int read_state(piko_reyes_pipe *p, ReyesState *s) {
+  HIPOK(hipMemcpyAsync(s, p->d_st, sizeof(ReyesState), hipMemcpyDeviceToHost, p->stream));
+  HIPOK(hipStreamSynchronize(p->stream));
+  return PIKO_OK;
+}
+
+// Grows whatever overflowed in the last frame; *rerun = 1 if it did.
+int check_capacity(piko_reyes_pipe *p, int *rerun) {
+  ReyesState s;
+  int rc = read_state(p, &s);
+  if (rc != PIKO_OK) return rc;
+  *rerun = 0;
+  if (!s.overflow) return PIKO_OK;
+  if (s.overflow & 1u) {
+    free_front(p);
+    p->ra.front_cap *= 2;
+    if ((rc = alloc_front(p)) != PIKO_OK) return rc;
+  }
+  if (s.overflow & 2u) {
+    free_leaves(p);
+    p->ra.leaf_cap *= 2;
+    if ((rc = alloc_leaves(p)) != PIKO_OK) return rc;
+  }
+  if ((s.overflow & 4u) && !(s.overflow & 3u)) {
+    if (s.nquad >= kMaxOrder) return PIKO_ERR_CAPACITY;
+    free_qcol(p);
+    p->ra.qcap = std::min<unsigned long long>(s.nquad + s.nquad / 8 + 4096, kMaxOrder);
+    if ((rc = alloc_qcol(p)) != PIKO_OK) return rc;
+  }
+  *rerun = 1;
+  return PIKO_OK;
+}
+
+bool valid_dice(int d) { return d == 0 || d == 1 || d == 2 || d == 4 || d == 8 || d == 16 || d == 32; }
+
+void set_camera(ReyesArgs *a, const piko_constant_state *cs) {
+  std::memcpy(a->vp, cs->viewProjMatrix, sizeof(a->vp));
+}
+
+int make_pipe(piko_reyes_pipe **out, const piko_constant_state *cs, const float *patches, int count,
+              const piko_reyes_options *opt, bool on_device) {
+  if (!out) return PIKO_ERR_INVALID;
+  *out = nullptr;
+  if (!cs || !opt || count < 0 || (count > 0 && !patches)) return PIKO_ERR_INVALID;
+  const int W = cs->screenSizeX, H = cs->screenSizeY;
+  if (W <= 0 || H <= 0 || W > 8192 || H > 8192) return PIKO_ERR_INVALID;
+  if (!(opt->threshold > 0.0f) || opt->max_splits < 0 || opt->max_splits > kSplitCapMax) return PIKO_ERR_INVALID;
+  if (!valid_dice(opt->dice) || (opt->spp != 1 && opt->spp != 4)) return PIKO_ERR_INVALID;
+  if (!(opt->displacement == opt->displacement) || opt->roots_per_wave < 0 || opt->roots_per_wave > 256)
+    return PIKO_ERR_INVALID;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PIKO_ERR_NO_DEVICE;
+  piko_reyes_pipe *p = new (std::nothrow) piko_reyes_pipe();
+  if (!p) return PIKO_ERR_OOM;
+  p->W = W;
+  p->H = H;
+  p->P = count;
+  p->opt = *opt;
+  ReyesArgs &a = p->ra;
+  a.W = W;
+  a.H = H;
+  set_camera(&a, cs);
+  a.threshold = opt->threshold;
+  a.max_splits = opt->max_splits;
+  a.dice = opt->dice;
+  a.spp = opt->spp;
+  a.disp = opt->displacement;
+  a.roots_per_wave = opt->roots_per_wave > 0 ? opt->roots_per_wave : 4;
+  a.nroots = count;
+  a.G = (int)std::max<long long>(cdiv(count, a.roots_per_wave), 1);
+  a.front_cap = 64 * a.roots_per_wave;
+  a.leaf_cap = 64 * a.roots_per_wave;
+  const unsigned long long per_leaf = a.dice > 0 ? (unsigned long long)a.dice * a.dice : 33ull * 33ull;
+  a.qcap = std::min<unsigned long long>((unsigned long long)a.G * a.leaf_cap * per_leaf,
+                                        std::max<unsigned long long>(1ull << 24, (unsigned long long)count * 64));
+  a.keep_samples = opt->keep_samples ? 1 : 0;
+  int rc = PIKO_OK;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) == hipSuccess && prop.multiProcessorCount > 0) p->cus = prop.multiProcessorCount;
+  p->dice_grid = p->cus * 16;
+  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) rc = PIKO_ERR_HIP;
+  if (rc == PIKO_OK) {
+    if (on_device) {
+      p->d_cp = (const float4 *)patches;
+    } else {
+      float4 *d = nullptr;
+      rc = ralloc(p, &d, (long long)count * 16);
+      if (rc == PIKO_OK && count > 0 &&
+          hipMemcpy(d, patches, (size_t)count * 256, hipMemcpyHostToDevice) != hipSuccess)
+        rc = PIKO_ERR_HIP;
+      p->d_cp = d;
+      p->owns_input = true;
+    }
+  }
+  const long long NL = (long long)kLevels * a.G;
+  const long long npix = (long long)W * H, ns = npix * a.spp;
+  if (rc == PIKO_OK) rc = alloc_front(p);
+  if (rc == PIKO_OK) rc = alloc_leaves(p);
+  if (rc == PIKO_OK) rc = alloc_qcol(p);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->d_cnt, NL);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->d_qcnt, NL);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->d_lvlpre, NL);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->d_off, NL);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->d_qoff, NL);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->d_wstat, a.G);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->d_st, 1);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->d_vis, ns);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->d_rgba, npix);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->d_depth, npix);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->d_sz, a.keep_samples ? ns : 1);
+  if (rc == PIKO_OK) {
+    k_fill_u64<<<(unsigned)std::min<long long>(cdiv(ns, 256), 4096), 256, 0, p->stream>>>(p->d_vis, ns, kVisEmpty);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(p->stream) != hipSuccess) rc = PIKO_ERR_HIP;
+  }
+  if (rc != PIKO_OK) {
+    piko_reyes_destroy(p);
+    return rc;
+  }
+  *out = p;
+  return PIKO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int piko_reyes_allocate(piko_reyes_pipe **out, const piko_constant_state *cs, const float *patches, int count,
+                        const piko_reyes_options *opt) {
+  return make_pipe(out, cs, patches, count, opt, false);
+}
+
+int piko_reyes_allocate_device(piko_reyes_pipe **out, const piko_constant_state *cs, const float *d_patches,
+                               int count, const piko_reyes_options *opt) {
+  return make_pipe(out, cs, d_patches, count, opt, true);
+}
+
+int piko_reyes_prepare(piko_reyes_pipe *p, const piko_constant_state *cs) {
+  if (!p || !cs) return PIKO_ERR_INVALID;
+  if (cs->screenSizeX != p->W || cs->screenSizeY != p->H) return PIKO_ERR_INVALID;
+  set_camera(&p->ra, cs);
+  return PIKO_OK;
+}
+
+int piko_reyes_run_frames(piko_reyes_pipe *p, int nframes) {
+  if (!p || nframes < 0) return PIKO_ERR_INVALID;
+  p->reruns = 0;
+  for (int attempt = 0; attempt < 12; ++attempt) {
+    for (int f = 0; f < nframes; ++f) {
+      const int rc = enqueue_frame(p, nullptr);
+      if (rc != PIKO_OK) return rc;
+    }
+    int rerun = 0;
+    const int rc = check_capacity(p, &rerun);
+    if (rc != PIKO_OK) return rc;
+    if (!rerun) return PIKO_OK;
+    if (nframes == 0) return PIKO_OK;
+    ++p->reruns;
+  }
+  return PIKO_ERR_CAPACITY;
+}
+
+int piko_reyes_run_single(piko_reyes_pipe *p) { return piko_reyes_run_frames(p, 1); }
+
+int piko_reyes_profile_frames(piko_reyes_pipe *p, int nframes, float *kernel_ms, const char **names, int cap,
+                              int *nk, float *frame_ms) {
+  if (!p || nframes <= 0 || !kernel_ms || !nk) return PIKO_ERR_INVALID;
+  int rc = piko_reyes_run_frames(p, 1);  // settles the capacities
+  if (rc != PIKO_OK) return rc;
+  hipEvent_t ev0, ev[4];
+  HIPOK(hipEventCreate(&ev0));
+  for (int k = 0; k < 4; ++k) HIPOK(hipEventCreate(&ev[k]));
+  double acc[4] = {0, 0, 0, 0}, facc = 0;
+  for (int f = 0; f < nframes; ++f) {
+    HIPOK(hipEventRecord(ev0, p->stream));
+    if ((rc = enqueue_frame(p, ev)) != PIKO_OK) return rc;
+    HIPOK(hipEventSynchronize(ev[3]));
+    float ms;
+    HIPOK(hipEventElapsedTime(&ms, ev0, ev[0]));
+    acc[0] += ms;
+    for (int k = 1; k < 4; ++k) {
+      HIPOK(hipEventElapsedTime(&ms, ev[k - 1], ev[k]));
+      acc[k] += ms;
+    }
+    HIPOK(hipEventElapsedTime(&ms, ev0, ev[3]));
+    facc += ms;
+  }
+  const int n = std::min(4, cap);
+  for (int k = 0; k < n; ++k) {
+    kernel_ms[k] = (float)(acc[k] / nframes);
+    if (names) names[k] = kReyesKernels[k];
+  }
+  *nk = n;
+  if (frame_ms) *frame_ms = (float)(facc / nframes);
+  (void)hipEventDestroy(ev0);
+  for (int k = 0; k < 4; ++k) (void)hipEventDestroy(ev[k]);
+  int rerun = 0;
+  rc = check_capacity(p, &rerun);
+  return rc != PIKO_OK ? rc : (rerun ? PIKO_ERR_CAPACITY : PIKO_OK);
+}
+
+int piko_reyes_get_color(piko_reyes_pipe *p, float *rgba) {
+  if (!p || !rgba) return PIKO_ERR_INVALID;
+  HIPOK(hipMemcpyAsync(rgba, p->d_rgba, (size_t)p->W * p->H * 16, hipMemcpyDeviceToHost, p->stream));
+  HIPOK(hipStreamSynchronize(p->stream));
+  return PIKO_OK;
+}
+
+int piko_reyes_get_depth(piko_reyes_pipe *p, float *depth) {
+  if (!p || !depth) return PIKO_ERR_INVALID;
+  HIPOK(hipMemcpyAsync(depth, p->d_depth, (size_t)p->W * p->H * 4, hipMemcpyDeviceToHost, p->stream));
+  HIPOK(hipStreamSynchronize(p->stream));
+  return PIKO_OK;
+}
+
+int piko_reyes_get_samples(piko_reyes_pipe *p, int32_t *z) {
+  if (!p || !z || !p->ra.keep_samples) return PIKO_ERR_INVALID;
+  HIPOK(hipMemcpyAsync(z, p->d_sz, (size_t)p->W * p->H * p->ra.spp * 4, hipMemcpyDeviceToHost, p->stream));
+  HIPOK(hipStreamSynchronize(p->stream));
+  return PIKO_OK;
+}
+
+long long piko_reyes_get_leaves(piko_reyes_pipe *p, float *cp, int *nu, int *nv, long long cap) {
+  if (!p || cap < 0 || (cap > 0 && !cp)) return PIKO_ERR_INVALID;
+  ReyesState s;
+  int rc = read_state(p, &s);
+  if (rc != PIKO_OK) return rc;
+  const ReyesArgs &a = p->ra;
+  const long long NL = (long long)kLevels * a.G, region = (long long)a.G * a.leaf_cap;
+  std::vector<uint32_t> cnt(NL), off(NL), pre(NL), meta(region);
+  std::vector<float4> leaves(cap > 0 ? region * 16 : 0);
+  HIPOK(hipMemcpy(cnt.data(), p->d_cnt, NL * 4, hipMemcpyDeviceToHost));
+  HIPOK(hipMemcpy(off.data(), p->d_off, NL * 4, hipMemcpyDeviceToHost));
+  HIPOK(hipMemcpy(pre.data(), p->d_lvlpre, NL * 4, hipMemcpyDeviceToHost));
+  HIPOK(hipMemcpy(meta.data(), p->d_lmeta, region * 4, hipMemcpyDeviceToHost));
+  if (cap > 0) HIPOK(hipMemcpy(leaves.data(), p->d_lcp, region * 256, hipMemcpyDeviceToHost));
+  for (long long grp = 0; grp < NL; ++grp) {
+    const long long w = grp % a.G;
+    for (uint32_t k = 0; k < cnt[grp]; ++k) {
+      const long long rank = (long long)off[grp] + k;
+      if (rank >= cap) continue;
+      const long long at = w * a.leaf_cap + pre[grp] + k;
+      std::memcpy(cp + rank * 64, &leaves[at * 16], 256);
+      if (nu) nu[rank] = (int)(meta[at] & 0xffu);
+      if (nv) nv[rank] = (int)((meta[at] >> 8) & 0xffu);
+    }
+  }
+  return (long long)s.nleaf;
+}
+
+int piko_reyes_get_stats(piko_reyes_pipe *p, piko_reyes_stats *st) {
+  if (!p || !st) return PIKO_ERR_INVALID;
+  ReyesState s;
+  const int rc = read_state(p, &s);
+  if (rc != PIKO_OK) return rc;
+  st->patches = p->P;
+  st->leaves = (long long)s.nleaf;
+  st->micropolygons = (long long)s.nquad;
+  st->levels = (int)s.levels;
+  st->device_bytes = p->device_bytes;
+  st->capacity_reruns = p->reruns;
+  return PIKO_OK;
+}
+
+int piko_reyes_destroy(piko_reyes_pipe *p) {
+  if (!p) return PIKO_ERR_INVALID;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  free_front(p);
+  free_leaves(p);
+  free_qcol(p);
+  rfree(p->d_cnt);
+  rfree(p->d_qcnt);
+  rfree(p->d_lvlpre);
+  rfree(p->d_off);
+  rfree(p->d_qoff);
+  rfree(p->d_wstat);
+  rfree(p->d_st);
+  rfree(p->d_vis);
+  rfree(p->d_rgba);
+  rfree(p->d_depth);
+  rfree(p->d_sz);
+  if (p->owns_input) {
+    float4 *d = const_cast<float4 *>(p->d_cp);
+    rfree(d);
+  }
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+  return PIKO_OK;
+}
+
+}  // extern "C"

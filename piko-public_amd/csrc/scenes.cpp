@@ -1,6 +1,7 @@
 // scenes.cpp -- deterministic synthetic scene generators for the Piko raster
 // pipeline workloads (pretransformed triangle soups, a world-space height
-// field with per-vertex normals, batched multi-scene specs, camera matrices).
+// field with per-vertex normals, batched multi-scene specs, camera matrices,
+// Bezier patches tiling a sphere for the Reyes pipeline).
 //
 // The reference draws its scenes from .scene/.obj files through assimp and
 // builds its camera with gluPerspective/gluLookAt (rasterPipelineFixPt/
@@ -233,6 +234,44 @@ int piko_scene_batch_spec(uint64_t base_seed, int nscenes, int n_lo, int n_hi, i
     W[s] = w < 16 ? 16 : w;
     H[s] = h < 16 ? 16 : h;
   }
+  return 0;
+}
+
+// Bicubic Bezier patches tiling the unit sphere on a lat-long grid: a global
+// control-point lattice of (3*nlat+1) rows (theta = pi*i/(3*nlat), pole to
+// pole) by 3*nlon columns (phi = 2*pi*j/(3*nlon), periodic), each lattice
+// point pushed radially by U[-noise, noise] (one draw per lattice point), so
+// neighbouring patches share their boundary rows of control points.  The two
+// pole rows sit on rings of angular radius kPoleRing instead of collapsing to
+// a point: a collapsed row makes dP/du vanish there and the dice stage's
+// normal (cross product, then normalise) would divide 0 by 0.  Patch (a, b)
+// takes lattice rows 3a..3a+3 and columns 3b..3b+3; its control point
+// CP[4*r + c] (row r along v = phi, column c along u = theta) is x,y,z,w
+// with w = 1, the piko_patch CP layout (reyesPipe/pikoTypes.h:65-73).  With
+// u southward and v eastward, dP/du x dP/dv points into the sphere, which the
+// shade stage's normal inversion (shade.pikostage:70-73) turns outward.
+// Patches are stored row-major in (a, b).
+int piko_scene_sphere_patches(uint64_t seed, int nlat, int nlon, float noise, float *cp) {
+  if (nlat <= 0 || nlon <= 0 || !cp || !(noise >= 0.0f)) return -1;
+  const int rows = 3 * nlat + 1, cols = 3 * nlon;
+  const double pi = 3.141592653589793, kPoleRing = 1e-4;
+  for (int a = 0; a < nlat; ++a)
+    for (int b = 0; b < nlon; ++b) {
+      float *P = cp + ((size_t)a * nlon + b) * 64;
+      for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) {
+          const int i = 3 * a + c, j = (3 * b + r) % cols;
+          Stream st(seed, (uint64_t)i * cols + j);
+          const double rad = 1.0 + (double)noise * (2.0 * st.u01() - 1.0);
+          const double th = std::min(std::max(pi * i / (rows - 1), kPoleRing), pi - kPoleRing);
+          const double ph = 2.0 * pi * j / cols;
+          float *q = P + 4 * (4 * r + c);
+          q[0] = (float)(rad * std::sin(th) * std::cos(ph));
+          q[1] = (float)(rad * std::cos(th));
+          q[2] = (float)(rad * std::sin(th) * std::sin(ph));
+          q[3] = 1.0f;
+        }
+    }
   return 0;
 }
 

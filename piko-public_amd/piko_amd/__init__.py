@@ -1,4 +1,5 @@
-"""Python host mirror of Piko's RasterPipe over the MI355X C-ABI (libpiko_hip.so).
+"""Python host mirror of Piko's RasterPipe and ReyesPipe over the MI355X C-ABI
+(libpiko_hip.so).
 
 The reference's user surface is the generated C++ pipe
 (pikoc/samples/rasterPipelineFixPt/rasterPipe.h:16-45):
@@ -91,6 +92,19 @@ class Stats(ctypes.Structure):
                 ("device_bytes", ctypes.c_longlong)]
 
 
+class ReyesOptions(ctypes.Structure):
+    """piko_reyes_options (include/piko_hip.h)."""
+    _fields_ = [("threshold", ctypes.c_float), ("max_splits", ctypes.c_int), ("dice", ctypes.c_int),
+                ("spp", ctypes.c_int), ("displacement", ctypes.c_float), ("roots_per_wave", ctypes.c_int),
+                ("keep_samples", ctypes.c_int)]
+
+
+class ReyesStats(ctypes.Structure):
+    _fields_ = [("patches", ctypes.c_longlong), ("leaves", ctypes.c_longlong),
+                ("micropolygons", ctypes.c_longlong), ("levels", ctypes.c_int),
+                ("device_bytes", ctypes.c_longlong), ("capacity_reruns", ctypes.c_int)]
+
+
 def lib_path():
     return os.path.join(_HERE, "libpiko_hip.so")
 
@@ -126,6 +140,20 @@ def lib():
         L.piko_batch_wait.argtypes = [P]
         L.piko_batch_get_color.argtypes = [P, I, P]
         L.piko_batch_get_depth.argtypes = [P, I, P]
+        L.piko_reyes_allocate.argtypes = [pp, ctypes.POINTER(ConstantState), P, I, ctypes.POINTER(ReyesOptions)]
+        L.piko_reyes_allocate_device.argtypes = [pp, ctypes.POINTER(ConstantState), P, I,
+                                                 ctypes.POINTER(ReyesOptions)]
+        L.piko_reyes_prepare.argtypes = [P, ctypes.POINTER(ConstantState)]
+        L.piko_reyes_run_single.argtypes = [P]
+        L.piko_reyes_run_frames.argtypes = [P, I]
+        L.piko_reyes_profile_frames.argtypes = [P, I, P, P, I, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_float)]
+        L.piko_reyes_get_color.argtypes = [P, P]
+        L.piko_reyes_get_depth.argtypes = [P, P]
+        L.piko_reyes_get_samples.argtypes = [P, P]
+        L.piko_reyes_get_leaves.argtypes = [P, P, P, P, LL]
+        L.piko_reyes_get_leaves.restype = LL
+        L.piko_reyes_get_stats.argtypes = [P, ctypes.POINTER(ReyesStats)]
+        L.piko_reyes_destroy.argtypes = [P]
         _lib = L
     return _lib
 
@@ -305,6 +333,104 @@ class BatchPipe:
 
     def __exit__(self, *a):
         self.destroy()
+
+
+class ReyesPipe:
+    """Split -> Dice -> Shade -> PikoScreen on the GPU: the pipe pikoc generates
+    for ReyesPipe (reyesPipe/reyesPipe.h:12-40), over the piko_reyes_* C-ABI.
+    `scene` is a scenes.PatchScene (patches, camera and Reyes settings)."""
+
+    def __init__(self):
+        self._h = None
+
+    def allocate(self, scene, roots_per_wave=0, keep_samples=False, max_splits=20):
+        if self._h is not None:
+            raise RuntimeError("pipe already allocated")
+        self.W, self.H, self.spp = scene.W, scene.H, scene.spp
+        self.cs = ConstantState.for_scene(scene)
+        opt = ReyesOptions(float(scene.threshold), int(max_splits), int(scene.dice), int(scene.spp),
+                           float(scene.disp), int(roots_per_wave), 1 if keep_samples else 0)
+        cp = np.ascontiguousarray(scene.cp, np.float32)
+        h = ctypes.c_void_p()
+        _check(lib().piko_reyes_allocate(ctypes.byref(h), ctypes.byref(self.cs), _p(cp), int(cp.shape[0]),
+                                         ctypes.byref(opt)), "reyes_allocate")
+        self._h = h
+        return self
+
+    def prepare(self, scene=None):
+        if scene is not None:
+            self.cs = ConstantState.for_scene(scene)
+        _check(lib().piko_reyes_prepare(self._h, ctypes.byref(self.cs)), "reyes_prepare")
+
+    def run_single(self):
+        _check(lib().piko_reyes_run_single(self._h), "reyes_run_single")
+
+    def run_frames(self, n):
+        _check(lib().piko_reyes_run_frames(self._h, int(n)), "reyes_run_frames")
+
+    def profile_frames(self, n, cap=8):
+        ms = (ctypes.c_float * cap)()
+        names = (ctypes.c_char_p * cap)()
+        nk = ctypes.c_int()
+        fm = ctypes.c_float()
+        _check(lib().piko_reyes_profile_frames(self._h, int(n), ctypes.cast(ms, ctypes.c_void_p),
+                                               ctypes.cast(names, ctypes.c_void_p), cap, ctypes.byref(nk),
+                                               ctypes.byref(fm)), "reyes_profile_frames")
+        return [(names[k].decode(), float(ms[k])) for k in range(nk.value)], float(fm.value)
+
+    def color(self):
+        """FP32 RGBA [H, W, 4], row 0 at the bottom."""
+        out = np.empty((self.H, self.W, 4), np.float32)
+        _check(lib().piko_reyes_get_color(self._h, _p(out)), "reyes_get_color")
+        return out
+
+    def depth(self):
+        out = np.empty((self.H, self.W), np.float32)
+        _check(lib().piko_reyes_get_depth(self._h, _p(out)), "reyes_get_depth")
+        return out
+
+    def samples(self):
+        """Per-sample depth bits [H, W, spp] (allocate with keep_samples=True)."""
+        out = np.empty((self.H, self.W, self.spp), np.int32)
+        _check(lib().piko_reyes_get_samples(self._h, _p(out)), "reyes_get_samples")
+        return out
+
+    def leaves(self):
+        """Split leaves in emission order: (cp [L, 64] f32, nu [L], nv [L])."""
+        n = lib().piko_reyes_get_leaves(self._h, None, None, None, 0)
+        if n < 0:
+            raise PikoError(int(n), "reyes_get_leaves")
+        cp = np.empty((max(n, 1), 64), np.float32)
+        nu = np.empty(max(n, 1), np.int32)
+        nv = np.empty(max(n, 1), np.int32)
+        n = lib().piko_reyes_get_leaves(self._h, _p(cp), _p(nu), _p(nv), int(n))
+        if n < 0:
+            raise PikoError(int(n), "reyes_get_leaves")
+        return cp[:n], nu[:n], nv[:n]
+
+    def stats(self):
+        st = ReyesStats()
+        _check(lib().piko_reyes_get_stats(self._h, ctypes.byref(st)), "reyes_stats")
+        return {k: getattr(st, k) for k, _ in ReyesStats._fields_}
+
+    def destroy(self):
+        if self._h is not None:
+            lib().piko_reyes_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.destroy()
+
+
+def render_reyes(scene, keep_samples=False):
+    """One Reyes frame; returns (rgba [H,W,4] f32, depth [H,W] f32, stats)."""
+    with ReyesPipe().allocate(scene, keep_samples=keep_samples) as pipe:
+        pipe.prepare()
+        pipe.run_single()
+        return pipe.color(), pipe.depth(), pipe.stats()
 
 
 def concat_scenes(scene_list):

@@ -28,6 +28,7 @@ def lib():
                                             ctypes.c_float, P, P]
         L.piko_scene_batch_spec.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int, P, P, P]
+        L.piko_scene_sphere_patches.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_float, P]
         _lib = L
     return _lib
 
@@ -126,12 +127,48 @@ def batch_spec(base_seed=1000, nscenes=256, n_lo=1000, n_hi=500000, px_lo_edge=2
     return [(base_seed + s, int(n[s]), int(w[s]), int(h[s])) for s in range(nscenes)]
 
 
+class PatchScene:
+    """Reyes input: cp [P, 64] f32 (16 control points x xyzw per patch, the
+    piko_patch CP layout) with a camera and the Reyes settings: split
+    threshold in pixels, dice grid edge (0 = the reference's nu/nv), samples
+    per pixel (1 or 4) and displacement amplitude."""
+
+    kind = "patches"
+
+    def __init__(self, cp, W, H, view_proj, threshold=16.0, dice=8, spp=4, disp=0.05, seed=None):
+        self.cp, self.W, self.H, self.view_proj = cp, W, H, view_proj
+        self.threshold, self.dice, self.spp, self.disp, self.seed = threshold, dice, spp, disp, seed
+
+    @property
+    def n(self):
+        return self.cp.shape[0]
+
+
+# Reyes sphere camera: the unit sphere seen from distance 3 (about 700 px
+# across at 720p), depth range [1, 5] so every displaced point is in front.
+SPHERE_EYE = (0.4, 0.9, 2.8)
+SPHERE_TARGET = (0.0, 0.0, 0.0)
+
+
+def sphere_patches(seed=3, nlat=64, nlon=64, noise=0.05, W=1280, H=720, eye=SPHERE_EYE, fovy=40.0,
+                   threshold=16.0, dice=8, spp=4, disp=0.05):
+    cp = np.empty((nlat * nlon, 64), np.float32)
+    if lib().piko_scene_sphere_patches(seed, nlat, nlon, noise, _p(cp)) != 0:
+        raise ValueError("bad patch arguments")
+    vp = view_proj(eye, SPHERE_TARGET, (0.0, 1.0, 0.0), fovy, W / H, 1.0, 5.0)
+    return PatchScene(cp, W, H, vp, threshold, dice, spp, disp, seed)
+
+
 def algorithmic_bytes(scene):
-    """Compulsory HBM bytes of one frame: input read once + colour and depth
-    written once (4 B each per pixel)."""
+    """Compulsory HBM bytes of one frame: input read once + the framebuffer
+    written once (raster: colour + depth, 4 B each per pixel; Reyes: FP32
+    RGBA + FP32 depth per pixel)."""
+    if scene.kind == "patches":
+        return 256 * scene.n + 20 * scene.W * scene.H
     b_in = 48 if scene.kind == "pretransformed" else 72
     return b_in * scene.n + 8 * scene.W * scene.H
 
 
-__all__ = ["TriScene", "WorldScene", "tris_uniform", "heightfield", "view_proj", "batch_spec",
+__all__ = ["TriScene", "WorldScene", "PatchScene", "tris_uniform", "heightfield", "sphere_patches", "view_proj",
+           "batch_spec",
            "algorithmic_bytes", "DEFAULT_LIGHT", "DEFAULT_MATCOL", "DEFAULT_AMBIENT", "math"]
