@@ -11,7 +11,8 @@ Prints one JSON line.  `value` is Mtri/s at the fastest bin size.  The same
 line carries the batched form (256 uneven scenes in one launch sequence vs
 sequential plain launches, autotuned per scene class) under "batch" and the
 served form (resident scenes, Poisson request stream, continuous batching,
-p50/p99 latency) under "serve".
+p50/p99 latency) under "serve", and the Reyes pipeline (split, dice,
+sample, resolve of 4,096 Bezier patches at 720p, 4 spp) under "reyes".
 """
 import argparse
 import json
@@ -157,6 +158,50 @@ def bench_serve(log, tris_per_scene, nreq, rate_hz=2000.0):
     return out
 
 
+def bench_reyes(log, steps, warmup, cpu=True):
+    """The Reyes configuration: 4,096 bicubic patches tiling a noisy sphere,
+    split below 16 px, 8x8 dice, displacement, 4 samples, 1280x720 FP32."""
+    import piko_amd
+    from piko_amd import scenes
+    sc = scenes.sphere_patches()
+    alg = scenes.algorithmic_bytes(sc)
+    with piko_amd.ReyesPipe().allocate(sc) as pipe:
+        pipe.prepare()
+        pipe.run_frames(max(warmup, 1))
+        t0 = time.perf_counter()
+        pipe.run_frames(steps)
+        dt = (time.perf_counter() - t0) / steps
+        kernels, frame_ms = pipe.profile_frames(max(5, min(steps, 20)))
+        st = pipe.stats()
+    dom = max(kernels, key=lambda k: k[1])
+    out = {"workload": "reyes-sphere-4096-720p: 64x64 bicubic patches on a unit sphere, radial noise +-0.05 "
+                       "(seed 3), split < 16 px, dice 8x8, displacement, 4 spp, 1280x720 FP32 RGBA",
+           "frames_per_s": round(1.0 / dt, 2), "ms_per_frame": round(dt * 1e3, 4),
+           "event_frame_ms": round(frame_ms, 4), "patches": st["patches"], "leaves": st["leaves"],
+           "micropolygons": st["micropolygons"], "split_levels": st["levels"],
+           "Mmicropolygons_per_s": round(st["micropolygons"] / dt / 1e6, 1),
+           "kernels_ms": [[n, round(m, 4)] for n, m in kernels],
+           "algorithmic_bytes_per_frame": alg,
+           "frame_GBs": round(alg / (frame_ms / 1e3) / 1e9, 2),
+           "dominant_kernel": dom[0], "dominant_ms": round(dom[1], 4)}
+    if cpu:
+        from oracle import oracle
+        oracle.build()
+        times = []
+        t_all = time.perf_counter()
+        while len(times) < 5 and time.perf_counter() - t_all < 12.0:
+            t0 = time.perf_counter()
+            oracle.reyes_frame(sc)
+            times.append(time.perf_counter() - t0)
+        times.sort()
+        med = times[len(times) // 2]
+        out["cpu_baseline"] = {"value": round(1.0 / med, 4), "unit": "frames/s", "cores": 1, "kind": "port",
+                               "sample": "%d full frames, median %.3f s/frame, single-threaded oracle, host %s" % (
+                                   len(times), med, cpu_model())}
+    log("reyes: %s" % out)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,7 +209,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--bins", default="8,16,32,64")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="all", choices=["raster", "batch", "serve", "all"])
+    ap.add_argument("--workload", default="all", choices=["raster", "batch", "serve", "reyes", "all"])
     ap.add_argument("--serve-tris", type=int, default=300_000, help="triangles per resident scene when serving")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -241,6 +286,8 @@ def main():
         out["batch"] = bench_batch(log)
     if args.workload in ("serve", "all"):
         out["serve"] = bench_serve(log, args.serve_tris, args.serve_requests)
+    if args.workload in ("reyes", "all"):
+        out["reyes"] = bench_reyes(log, args.steps, args.warmup, cpu=not args.no_cpu)
     print(json.dumps(out), flush=True)
 
 

@@ -233,7 +233,7 @@ typedef struct {
                           shade.pikostage:166-167) or 4 (2x2 strata centres) */
   float displacement;  /* displacement along the normal by
                           displacement*(x^2 - y^2) of the surface point; 0 = none */
-  int roots_per_wave;  /* patches whose split trees one wave walks; 0 = 4 */
+  int roots_per_wave;  /* patches whose split trees one wave walks; 0 = 1 */
   int keep_samples;    /* 1: keep per-sample depth for piko_reyes_get_samples */
 } piko_reyes_options;
 

@@ -1,25 +1,29 @@
 // piko_reyes.hip -- the Reyes pipeline (Split -> Dice -> Shade -> PikoScreen)
 // on gfx950 (reyesPipe/reyesPipe.h:12-40, split/dice/shade.pikostage).
 //
-// Frame = four launches on the pipe's stream, no host round trip:
+// Frame = five launches on the pipe's stream, no host round trip:
 //   k_reyes_split   one wave walks the split trees of a contiguous run of
 //                   input patches level by level (16 lanes per patch, one per
 //                   control point, 4 patches per pass).  Its frontier lives
 //                   in a per-wave region; leaves go to a per-wave region in
 //                   (level, position) order with per-level counts.
-//   k_reyes_scan    one block: exclusive scan of the per-(level, wave) leaf
-//                   and quad counts in level-major order.  Because each wave
-//                   owns a contiguous run of roots, a leaf's index in the
-//                   split stage's emission order (level by level, in order
+//   k_reyes_levels, k_reyes_rank
+//                   one block per level: level totals, then the exclusive scan
+//                   of the per-(level, wave) leaf and quad counts in
+//                   level-major order.  Because
+//                   each wave owns a contiguous run of roots, a leaf's index in
+//                   the split stage's emission order (level by level, in order
 //                   within a level: the reference's split self-loop, CPU
 //                   backend one launch per level) is
-//                   offset[level][wave] + its position in the wave's list.
+//                   offset[level][wave] + its position in the wave's list; the
+//                   kernel writes that index densely (rank -> region slot).
 //   k_reyes_dice    one wave per leaf (persistent grid): per-u Bernstein rows,
 //                   the (g+1)^2 grid points once each (position, normal,
 //                   displacement, projection, Phong colour) into LDS, then one
-//                   lane per micropolygon: flat depth and colour, and the
-//                   sample tests with a 64-bit atomicMin of (depth, reversed
-//                   emission order) per sample, so the nearest micropolygon
+//                   lane per micropolygon for flat depth, colour and pixel
+//                   box, and the (micropolygon, pixel) pairs spread evenly over
+//                   the lanes for the sample tests, each hit a 64-bit atomicMin
+//                   of (depth, reversed emission order), so the nearest micropolygon
 //                   wins and on equal depth the later one -- the outcome of
 //                   the reference's atomicMin loop + (zBuffer == z) test run
 //                   in emission order (shade.pikostage:226-241).
@@ -78,6 +82,16 @@ __device__ __forceinline__ float4 f4half(float4 a) {
 }
 __device__ __forceinline__ float4 shfl4(float4 v, int src) {
   return make_float4(__shfl(v.x, src, 16), __shfl(v.y, src, 16), __shfl(v.z, src, 16), __shfl(v.w, src, 16));
+}
+
+// Synchronises the lanes of a one-wave workgroup around LDS traffic.  The
+// fences are wavefront-scoped, so outstanding global stores and atomics are
+// not waited for (a __syncthreads would wait for them).  LDS accesses of one
+// wave execute in order, so this orders them.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // (int)ceil(ext / threshold) of split.pikostage:307-308 made total: 64 and
@@ -201,7 +215,7 @@ __global__ __launch_bounds__(64) void k_reyes_split(const float4 *__restrict__ r
         const uint32_t at = nleaves + plf;
         leaf[(size_t)at * 16 + i] = p;
         if (i == 0) {
-          meta[at] = (uint32_t)gu | ((uint32_t)gv << 8);
+          meta[at] = (uint32_t)gu | ((uint32_t)gv << 8) | ((uint32_t)L << 16);
           qloc[at] = qrun + pq;
         }
       }
@@ -231,61 +245,133 @@ __global__ __launch_bounds__(64) void k_reyes_split(const float4 *__restrict__ r
 }
 
 // ---------------------------------------------------------------------------
-// Scan of the (level, wave) counts, level-major; frame totals and status.
+// Ranks.  k_reyes_levels: one block per level sums the level's leaf and quad
+// counts over the waves.  k_reyes_rank: one block per level adds the levels
+// before it and scans the level's counts over the waves (each thread a
+// contiguous run of waves), writing the dense emission-order index
+// leaf_at[rank] = region slot and qoff[level][wave], the order index of the
+// group's first quad.  The last level's block writes the frame totals and
+// status.
 
-constexpr int kScanThreads = 1024;
+constexpr int kRankThreads = 1024;
 
-__global__ __launch_bounds__(kScanThreads) void k_reyes_scan(const uint32_t *__restrict__ cnt,
-                                                             const uint32_t *__restrict__ qcnt,
-                                                             const uint32_t *__restrict__ wstat, int G,
-                                                             unsigned long long qcap, uint32_t *__restrict__ off,
-                                                             unsigned long long *__restrict__ qoff,
-                                                             ReyesState *__restrict__ st) {
-  __shared__ unsigned long long s_a[kScanThreads], s_b[kScanThreads];
-  __shared__ uint32_t s_or, s_lv;
-  const int t = threadIdx.x;
-  const int NL = kLevels * G;
-  const int per = (NL + kScanThreads - 1) / kScanThreads;
-  const int lo = min(t * per, NL), hi = min(lo + per, NL);
-  unsigned long long sa = 0, sb = 0;
-  for (int k = lo; k < hi; ++k) {
-    sa += cnt[k];
-    sb += qcnt[k];
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const T y = __shfl_up(v, d);
+    if (lane >= d) v += y;
   }
+  return v;
+}
+
+// Block-wide sums of two values (all threads receive them).
+__device__ __forceinline__ void block_sum2(unsigned long long &a, unsigned long long &b,
+                                           unsigned long long (*sh)[kRankThreads / 64]) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  a = wave_sum_u64(a);
+  b = wave_sum_u64(b);
+  if (lane == 0) {
+    sh[0][wv] = a;
+    sh[1][wv] = b;
+  }
+  __syncthreads();
+  a = b = 0;
+#pragma unroll
+  for (int k = 0; k < kRankThreads / 64; ++k) {
+    a += sh[0][k];
+    b += sh[1][k];
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kRankThreads) void k_reyes_levels(const uint32_t *__restrict__ cnt,
+                                                               const uint32_t *__restrict__ qcnt, int G,
+                                                               unsigned long long *__restrict__ lvltot) {
+  __shared__ unsigned long long sh[2][kRankThreads / 64];
+  const int L = blockIdx.x;
+  unsigned long long a = 0, b = 0;
+  for (int w = threadIdx.x; w < G; w += kRankThreads) {
+    a += cnt[L * G + w];
+    b += qcnt[L * G + w];
+  }
+  block_sum2(a, b, sh);
+  if (threadIdx.x == 0) {
+    lvltot[2 * L] = a;
+    lvltot[2 * L + 1] = b;
+  }
+}
+
+__global__ __launch_bounds__(kRankThreads) void k_reyes_rank(
+    const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ qcnt, const uint32_t *__restrict__ lvlpre,
+    const uint32_t *__restrict__ wstat, const unsigned long long *__restrict__ lvltot, int G, int leaf_cap,
+    unsigned long long qcap, uint32_t *__restrict__ off, uint32_t *__restrict__ qoff, uint32_t *__restrict__ leaf_at,
+    ReyesState *__restrict__ st) {
+  __shared__ unsigned long long sh[2][kRankThreads / 64];
+  __shared__ uint32_t s_or, s_lv;
+  const int L = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (lvltot[2 * L] == 0 && L != kLevels - 1) return;  // no leaves at this level
+  unsigned long long base_l = 0, base_q = 0;
+  for (int l = 0; l < L; ++l) {
+    base_l += lvltot[2 * l];
+    base_q += lvltot[2 * l + 1];
+  }
+  const int run = (G + kRankThreads - 1) / kRankThreads;
+  const int w0 = min(t * run, G), w1 = min(w0 + run, G);
+  unsigned long long sl = 0, sq = 0;
+  for (int w = w0; w < w1; ++w) {
+    sl += cnt[L * G + w];
+    sq += qcnt[L * G + w];
+  }
+  // exclusive block scan of the per-thread run totals
+  const unsigned long long il = wave_incl_scan<unsigned long long>(sl, lane);
+  const unsigned long long iq = wave_incl_scan<unsigned long long>(sq, lane);
+  if (lane == 63) {
+    sh[0][wv] = il;
+    sh[1][wv] = iq;
+  }
+  __syncthreads();
+  unsigned long long o = base_l + il - sl, oq = base_q + iq - sq;
+#pragma unroll
+  for (int k = 0; k < kRankThreads / 64; ++k)
+    if (k < wv) {
+      o += sh[0][k];
+      oq += sh[1][k];
+    }
+  for (int w = w0; w < w1; ++w) {
+    const uint32_t c = cnt[L * G + w];
+    off[L * G + w] = (uint32_t)o;
+    qoff[L * G + w] = (uint32_t)oq;  // exact whenever the frame's quads fit (status bit 2 clear)
+    const uint32_t slot0 = (uint32_t)w * (uint32_t)leaf_cap + lvlpre[L * G + w];
+    for (uint32_t k = 0; k < c; ++k) leaf_at[o + k] = slot0 + k;
+    o += c;
+    oq += qcnt[L * G + w];
+  }
+  if (L != kLevels - 1) return;
   if (t == 0) {
     s_or = 0;
     s_lv = 0;
   }
-  s_a[t] = sa;
-  s_b[t] = sb;
   __syncthreads();
-  for (int d = 1; d < kScanThreads; d <<= 1) {  // inclusive Hillis-Steele scan
-    const unsigned long long xa = t >= d ? s_a[t - d] : 0, xb = t >= d ? s_b[t - d] : 0;
-    __syncthreads();
-    s_a[t] += xa;
-    s_b[t] += xb;
-    __syncthreads();
-  }
-  unsigned long long ra = s_a[t] - sa, rb = s_b[t] - sb;
-  for (int k = lo; k < hi; ++k) {
-    off[k] = (uint32_t)ra;
-    qoff[k] = rb;
-    ra += cnt[k];
-    rb += qcnt[k];
-  }
   uint32_t ovf = 0, lv = 0;
-  for (int k = t; k < G; k += kScanThreads) {
-    ovf |= wstat[k] & 3u;
-    lv = max(lv, wstat[k] >> 8);
+  for (int w = t; w < G; w += kRankThreads) {
+    ovf |= wstat[w] & 3u;
+    lv = max(lv, wstat[w] >> 8);
   }
   atomicOr(&s_or, ovf);
   atomicMax(&s_lv, lv);
   __syncthreads();
   if (t == 0) {
-    const unsigned long long nleaf = s_a[kScanThreads - 1], nquad = s_b[kScanThreads - 1];
+    const unsigned long long nleaf = base_l + lvltot[2 * L], nquad = base_q + lvltot[2 * L + 1];
     st->nleaf = nleaf;
     st->nquad = nquad;
-    st->overflow = s_or | ((nquad > qcap || nquad >= kMaxOrder || nleaf >= 0xFFFFFFFFull) ? 4u : 0u);
+    st->overflow = s_or | ((nquad > qcap || nquad >= kMaxOrder) ? 4u : 0u);
     st->levels = s_lv;
   }
 }
@@ -331,36 +417,68 @@ __device__ __forceinline__ int clampi(float f) {
   return (int)f;
 }
 
-template <int MAXP>
+// Pixels a micropolygon is tested on: ShadeStage bins it by the box of its
+// first three corners (computeBoundingBox, dice.pikostage:18-43) into 64-px
+// bins and tests the four-corner box inside each bin (shade.pikostage:
+// 140-163).  Bins per axis are ceil(W/64), pixels clipped to the screen.
+__device__ __forceinline__ void quad_pixels(const float *X, const float *Y, int W, int H, int *x0, int *y0,
+                                            int *x1, int *y1) {
+  const int nbx = (W + 63) / 64, nby = (H + 63) / 64;
+  const float lo3x = floorf(fmn(X[0], fmn(X[1], X[2]))), lo3y = floorf(fmn(Y[0], fmn(Y[1], Y[2])));
+  const float hi3x = ceilf(fmx(X[0], fmx(X[1], X[2]))), hi3y = ceilf(fmx(Y[0], fmx(Y[1], Y[2])));
+  const int bx1 = max(clampi(floorf(lo3x / 64.0f)), 0), bx2 = min(clampi(floorf(hi3x / 64.0f)), nbx - 1);
+  const int by1 = max(clampi(floorf(lo3y / 64.0f)), 0), by2 = min(clampi(floorf(hi3y / 64.0f)), nby - 1);
+  *x0 = max(clampi(floorf(fmn(fmn(X[0], X[1]), fmn(X[2], X[3])))), bx1 * 64);
+  *y0 = max(clampi(floorf(fmn(fmn(Y[0], Y[1]), fmn(Y[2], Y[3])))), by1 * 64);
+  *x1 = min(min(clampi(ceilf(fmx(fmx(X[0], X[1]), fmx(X[2], X[3])))), (bx2 + 1) * 64), W);
+  *y1 = min(min(clampi(ceilf(fmx(fmx(Y[0], Y[1]), fmx(Y[2], Y[3])))), (by2 + 1) * 64), H);
+  if (bx1 > bx2 || by1 > by2 || *x0 >= *x1 || *y0 >= *y1) {
+    *x0 = *y0 = 0;
+    *x1 = *y1 = 0;
+  }
+}
+
+// Two 4-edge fan tests at a sample (shade.pikostage:176-218): hit if either
+// orientation has all edge functions > 0.
+__device__ __forceinline__ bool quad_hit(const float *X, const float *Y, float px, float py) {
+  bool r1 = true, r2 = true;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int ea = e, eb = (e + 1) & 3;
+    if (((py - Y[ea]) * (X[eb] - X[ea]) - (px - X[ea]) * (Y[eb] - Y[ea])) <= 0.0f) r1 = false;
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int ea = (4 - e) & 3, eb = (3 - e) & 3;
+    if (((py - Y[ea]) * (X[eb] - X[ea]) - (px - X[ea]) * (Y[eb] - Y[ea])) <= 0.0f) r2 = false;
+  }
+  return r1 || r2;
+}
+
+template <int MAXP, int SPP>
 __global__ __launch_bounds__(64) void k_reyes_dice(ReyesArgs a, const float4 *__restrict__ lcp,
                                                    const uint32_t *__restrict__ lmeta,
                                                    const uint32_t *__restrict__ lqloc,
-                                                   const uint32_t *__restrict__ off,
-                                                   const unsigned long long *__restrict__ qoff,
-                                                   const uint32_t *__restrict__ lvlpre,
+                                                   const uint32_t *__restrict__ leaf_at,
+                                                   const uint32_t *__restrict__ qoff,
                                                    const ReyesState *__restrict__ st, float *__restrict__ qcol,
                                                    unsigned long long *__restrict__ vis) {
   __shared__ float4 s_cp[16];
   __shared__ float4 s_row[MAXP][8];  // per grid u: bez2D rows 0..3, dbez2D rows 4..7
   __shared__ float s_pt[MAXP * MAXP][6];
+  __shared__ float s_qx[64][4], s_qy[64][4];
+  __shared__ unsigned long long s_key[64];
+  __shared__ int s_x0[64], s_y0[64], s_wid[64], s_pre[64];
   const int lane = threadIdx.x;
+  if (st->overflow & 4u) return;  // quad colours would not fit: the frame is re-run with room
   const unsigned long long nleaf = st->nleaf;
-  const int NL = kLevels * a.G;
   const float *m = a.vp;
-  const int W = a.W, H = a.H, spp = a.spp;
-  const int nbx = (W + 63) / 64, nby = (H + 63) / 64;
+  const int W = a.W, H = a.H;
   for (unsigned long long g = blockIdx.x; g < nleaf; g += gridDim.x) {
-    int lo = 0, hi = NL;  // first (level, wave) group whose offset exceeds g
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if ((unsigned long long)off[mid] <= g) lo = mid + 1;
-      else hi = mid;
-    }
-    const int grp = lo - 1, w = grp % a.G;
-    const size_t at = (size_t)w * a.leaf_cap + lvlpre[grp] + (uint32_t)(g - off[grp]);
+    const size_t at = leaf_at[g];
     if (lane < 16) s_cp[lane] = lcp[at * 16 + lane];
     const uint32_t mt = lmeta[at];
-    const unsigned long long qbase = qoff[grp] + lqloc[at];
+    const uint32_t qbase = qoff[(mt >> 16) * (uint32_t)a.G + (uint32_t)(at / (size_t)a.leaf_cap)] + lqloc[at];
     int nu, nv, gu, gv;  // dice steps; micropolygons along u and v
     if (a.dice > 0) {
       nu = nv = gu = gv = a.dice;
@@ -371,21 +489,18 @@ __global__ __launch_bounds__(64) void k_reyes_dice(ReyesArgs a, const float4 *__
       gv = nv + 1;
     }
     const float du = 1.0f / ((float)nu), dv = 1.0f / ((float)nv);
-    __syncthreads();
+    wave_sync();
     // grid u values: myU = k*du for k < gu, and the last quad's myU + du
     // (dice.pikostage:180, 233); with power-of-two steps these coincide with
     // the neighbouring quads' corners bit for bit.
-    for (int iu = lane; iu <= gu; iu += 64) {
+    for (int e = lane; e < 8 * (gu + 1); e += 64) {  // (grid u, row) pairs
+      const int iu = e >> 3, row = e & 7, r4 = 4 * (row & 3);
       const float u = iu < gu ? (float)iu * du : (float)(gu - 1) * du + du;
       float B[4], D[4];
       bern(u, B, D);
-#pragma unroll
-      for (int row = 0; row < 4; ++row) {
-        s_row[iu][row] = comb(B, s_cp[4 * row], s_cp[4 * row + 1], s_cp[4 * row + 2], s_cp[4 * row + 3]);
-        s_row[iu][4 + row] = comb(D, s_cp[4 * row], s_cp[4 * row + 1], s_cp[4 * row + 2], s_cp[4 * row + 3]);
-      }
+      s_row[iu][row] = comb(row < 4 ? B : D, s_cp[r4], s_cp[r4 + 1], s_cp[r4 + 2], s_cp[r4 + 3]);
     }
-    __syncthreads();
+    wave_sync();
     const int np = (gu + 1) * (gv + 1);
     for (int pt = lane; pt < np; pt += 64) {
       const int iu = pt / (gv + 1), jv = pt - iu * (gv + 1);
@@ -424,69 +539,61 @@ __global__ __launch_bounds__(64) void k_reyes_dice(ReyesArgs a, const float4 *__
       o[4] = sat(0.2f + dif * 0.5773f);
       o[5] = sat(0.2f + dif * 0.973f);
     }
-    __syncthreads();
+    wave_sync();
     const int nq = gu * gv;
-    for (int qd = lane; qd < nq; qd += 64) {
-      const int k = qd / gv, j = qd - k * gv;
-      const float *c0 = s_pt[k * (gv + 1) + j], *c1 = s_pt[(k + 1) * (gv + 1) + j];
-      const float *c2 = s_pt[(k + 1) * (gv + 1) + j + 1], *c3 = s_pt[k * (gv + 1) + j + 1];
-      const float X[4] = {c0[0], c1[0], c2[0], c3[0]}, Y[4] = {c0[1], c1[1], c2[1], c3[1]};
-      const float z = (((c0[2] + c1[2]) + c2[2]) + c3[2]) * 0.25f;  // shade.pikostage:98
-      float rgb[3];
+    for (int q0 = 0; q0 < nq; q0 += 64) {
+      // one lane per micropolygon: flat depth and colour, pixel box
+      const int qd = q0 + lane;
+      int x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+      if (qd < nq) {
+        const int k = qd / gv, j = qd - k * gv;
+        const float *c0 = s_pt[k * (gv + 1) + j], *c1 = s_pt[(k + 1) * (gv + 1) + j];
+        const float *c2 = s_pt[(k + 1) * (gv + 1) + j + 1], *c3 = s_pt[k * (gv + 1) + j + 1];
+        const float X[4] = {c0[0], c1[0], c2[0], c3[0]}, Y[4] = {c0[1], c1[1], c2[1], c3[1]};
+        const float z = (((c0[2] + c1[2]) + c2[2]) + c3[2]) * 0.25f;  // shade.pikostage:98
+        const uint32_t order = qbase + (uint32_t)qd;
 #pragma unroll
-      for (int ch = 0; ch < 3; ++ch)  // shade.pikostage:130-138
-        rgb[ch] = fmn(((c0[3 + ch] + c1[3 + ch]) + c2[3 + ch]) + c3[3 + ch], 4.0f) * 0.25f;
-      const unsigned long long order = qbase + (unsigned long long)qd;
-      if (order >= a.qcap) continue;  // frame is re-run with room (status bit 2)
-      qcol[order * 3 + 0] = rgb[0];
-      qcol[order * 3 + 1] = rgb[1];
-      qcol[order * 3 + 2] = rgb[2];
-      // pixels tested: ShadeStage bins by the box of the first three corners
-      // (computeBoundingBox, dice.pikostage:18-43) into 64-px bins and tests
-      // the four-corner box inside each bin (shade.pikostage:140-163).
-      const float lo3x = floorf(fmn(X[0], fmn(X[1], X[2]))), lo3y = floorf(fmn(Y[0], fmn(Y[1], Y[2])));
-      const float hi3x = ceilf(fmx(X[0], fmx(X[1], X[2]))), hi3y = ceilf(fmx(Y[0], fmx(Y[1], Y[2])));
-      int bx1 = clampi(floorf(lo3x / 64.0f)), bx2 = clampi(floorf(hi3x / 64.0f));
-      int by1 = clampi(floorf(lo3y / 64.0f)), by2 = clampi(floorf(hi3y / 64.0f));
-      bx1 = max(bx1, 0);
-      by1 = max(by1, 0);
-      bx2 = min(bx2, nbx - 1);
-      by2 = min(by2, nby - 1);
-      if (bx1 > bx2 || by1 > by2) continue;
-      int x0 = clampi(floorf(fmn(fmn(X[0], X[1]), fmn(X[2], X[3]))));
-      int y0 = clampi(floorf(fmn(fmn(Y[0], Y[1]), fmn(Y[2], Y[3]))));
-      int x1 = clampi(ceilf(fmx(fmx(X[0], X[1]), fmx(X[2], X[3]))));
-      int y1 = clampi(ceilf(fmx(fmx(Y[0], Y[1]), fmx(Y[2], Y[3]))));
-      x0 = max(x0, bx1 * 64);
-      y0 = max(y0, by1 * 64);
-      x1 = min(min(x1, (bx2 + 1) * 64), W);
-      y1 = min(min(y1, (by2 + 1) * 64), H);
-      const uint32_t zb = __float_as_uint(z) ^ 0x80000000u;
-      const unsigned long long key = ((unsigned long long)zb << 32) | (unsigned long long)(0xFFFFFFFEu - (uint32_t)order);
-      for (int y = y0; y < y1; ++y)
-        for (int x = x0; x < x1; ++x)
-          for (int s = 0; s < spp; ++s) {
-            const float px = (float)x + sample_off(spp, s, 0), py = (float)y + sample_off(spp, s, 1);
-            // two 4-edge fan tests (shade.pikostage:176-218)
-            bool r1 = true, r2 = true;
+        for (int ch = 0; ch < 3; ++ch)  // shade.pikostage:130-138
+          qcol[(size_t)order * 3 + ch] = fmn(((c0[3 + ch] + c1[3 + ch]) + c2[3 + ch]) + c3[3 + ch], 4.0f) * 0.25f;
+        quad_pixels(X, Y, W, H, &x0, &y0, &x1, &y1);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const int ea = e, eb = (e + 1) & 3;
-              const float t = ((py - Y[ea]) * (X[eb] - X[ea]) - (px - X[ea]) * (Y[eb] - Y[ea]));
-              if (t <= 0.0f) r1 = false;
-            }
+        for (int e = 0; e < 4; ++e) {
+          s_qx[lane][e] = X[e];
+          s_qy[lane][e] = Y[e];
+        }
+        s_key[lane] = ((unsigned long long)(__float_as_uint(z) ^ 0x80000000u) << 32) |
+                      (unsigned long long)(0xFFFFFFFEu - order);
+      }
+      // spread the (micropolygon, pixel) pairs of these 64 quads evenly
+      // over the lanes
+      const int wid = x1 - x0, npx = wid * (y1 - y0);
+      const int inc = wave_incl_scan<int>(npx, lane);
+      const int total = __shfl(inc, 63);
+      s_x0[lane] = x0;
+      s_y0[lane] = y0;
+      s_wid[lane] = wid;
+      s_pre[lane] = inc - npx;
+      wave_sync();
+      const int nqc = min(64, nq - q0);
+      for (int t = lane; t < total; t += 64) {
+        int q = 0;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const int ea = (4 - e) & 3, eb = (3 - e) & 3;
-              const float t = ((py - Y[ea]) * (X[eb] - X[ea]) - (px - X[ea]) * (Y[eb] - Y[ea]));
-              if (t <= 0.0f) r2 = false;
-            }
-            if (!(r1 || r2)) continue;
-            unsigned long long *dst = vis + ((size_t)y * W + x) * spp + s;
-            if (key < *dst) atomicMin(dst, key);
-          }
+        for (int step = 32; step > 0; step >>= 1)
+          if (q + step < nqc && s_pre[q + step] <= t) q += step;
+        const int local = t - s_pre[q], w = s_wid[q];
+        const int dy = local / w, dx = local - dy * w;
+        const int x = s_x0[q] + dx, y = s_y0[q] + dy;
+        const float X[4] = {s_qx[q][0], s_qx[q][1], s_qx[q][2], s_qx[q][3]};
+        const float Y[4] = {s_qy[q][0], s_qy[q][1], s_qy[q][2], s_qy[q][3]};
+        const unsigned long long key = s_key[q];
+        unsigned long long *dst = vis + ((size_t)y * W + x) * SPP;
+#pragma unroll
+        for (int smp = 0; smp < SPP; ++smp)
+          if (quad_hit(X, Y, (float)x + sample_off(SPP, smp, 0), (float)y + sample_off(SPP, smp, 1)))
+            atomicMin(dst + smp, key);
+      }
+      wave_sync();
     }
-    __syncthreads();
   }
 }
 
@@ -525,7 +632,7 @@ __global__ __launch_bounds__(256) void k_reyes_resolve(ReyesArgs a, unsigned lon
     for (int ch = 0; ch < 3; ++ch) acc[ch] = s == 0 ? c[ch] : acc[ch] + c[ch];
     zmin = (s == 0 || zi < zmin) ? zi : zmin;
     if (a.keep_samples) sz[si] = zi;
-    vis[si] = kVisEmpty;
+    if (low != 0xFFFFFFFFu) vis[si] = kVisEmpty;
   }
   const float inv = 1.0f / (float)spp;
   rgba[p] = make_float4(acc[0] * inv, acc[1] * inv, acc[2] * inv, 1.0f);
@@ -551,7 +658,9 @@ struct piko_reyes_pipe {
   float4 *d_front = nullptr, *d_lcp = nullptr;
   uint32_t *d_lmeta = nullptr, *d_lqloc = nullptr;
   uint32_t *d_cnt = nullptr, *d_qcnt = nullptr, *d_lvlpre = nullptr, *d_off = nullptr, *d_wstat = nullptr;
-  unsigned long long *d_qoff = nullptr;
+  uint32_t *d_leaf_at = nullptr;  // emission rank -> region slot
+  uint32_t *d_qoff = nullptr;     // [level][wave] order index of the group's first quad
+  unsigned long long *d_lvltot = nullptr;  // [level][leaves, quads]
   ReyesState *d_st = nullptr;
   float *d_qcol = nullptr;
   unsigned long long *d_vis = nullptr;
@@ -603,6 +712,7 @@ void free_leaves(piko_reyes_pipe *p) {
   rfree(p->d_lcp);
   rfree(p->d_lmeta);
   rfree(p->d_lqloc);
+  rfree(p->d_leaf_at);
   p->device_bytes -= p->leaf_bytes;
   p->leaf_bytes = 0;
 }
@@ -612,6 +722,7 @@ int alloc_leaves(piko_reyes_pipe *p) {
   int rc = ralloc(p, &p->d_lcp, n * 16, &p->leaf_bytes);
   if (rc == PIKO_OK) rc = ralloc(p, &p->d_lmeta, n, &p->leaf_bytes);
   if (rc == PIKO_OK) rc = ralloc(p, &p->d_lqloc, n, &p->leaf_bytes);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->d_leaf_at, n, &p->leaf_bytes);
   return rc;
 }
 
@@ -623,7 +734,7 @@ void free_qcol(piko_reyes_pipe *p) {
 
 int alloc_qcol(piko_reyes_pipe *p) { return ralloc(p, &p->d_qcol, (long long)p->ra.qcap * 3, &p->qcol_bytes); }
 
-const char *kReyesKernels[] = {"k_reyes_split", "k_reyes_scan", "k_reyes_dice", "k_reyes_resolve"};
+const char *kReyesKernels[] = {"k_reyes_split", "k_reyes_levels+k_reyes_rank", "k_reyes_dice", "k_reyes_resolve"};
 
 int enqueue_frame(piko_reyes_pipe *p, hipEvent_t *ev) {
   const ReyesArgs &a = p->ra;
@@ -632,15 +743,23 @@ int enqueue_frame(piko_reyes_pipe *p, hipEvent_t *ev) {
   k_reyes_split<<<a.G, 64, 0, p->stream>>>(p->d_cp, a, p->d_front, p->d_lcp, p->d_lmeta, p->d_lqloc, p->d_cnt,
                                            p->d_qcnt, p->d_lvlpre, p->d_wstat);
   if (ev) (void)hipEventRecord(ev[0], p->stream);
-  k_reyes_scan<<<1, kScanThreads, 0, p->stream>>>(p->d_cnt, p->d_qcnt, p->d_wstat, a.G, a.qcap, p->d_off,
-                                                  p->d_qoff, p->d_st);
+  k_reyes_levels<<<kLevels, kRankThreads, 0, p->stream>>>(p->d_cnt, p->d_qcnt, a.G, p->d_lvltot);
+  k_reyes_rank<<<kLevels, kRankThreads, 0, p->stream>>>(p->d_cnt, p->d_qcnt, p->d_lvlpre, p->d_wstat, p->d_lvltot,
+                                                        a.G, a.leaf_cap, a.qcap, p->d_off, p->d_qoff, p->d_leaf_at,
+                                                        p->d_st);
   if (ev) (void)hipEventRecord(ev[1], p->stream);
-  if (a.dice > 0 && a.dice <= 8)
-    k_reyes_dice<9><<<p->dice_grid, 64, 0, p->stream>>>(a, p->d_lcp, p->d_lmeta, p->d_lqloc, p->d_off, p->d_qoff,
-                                                        p->d_lvlpre, p->d_st, p->d_qcol, p->d_vis);
-  else
-    k_reyes_dice<34><<<p->dice_grid, 64, 0, p->stream>>>(a, p->d_lcp, p->d_lmeta, p->d_lqloc, p->d_off, p->d_qoff,
-                                                         p->d_lvlpre, p->d_st, p->d_qcol, p->d_vis);
+#define PIKO_DICE(MAXP, SPP)                                                                                  \
+  k_reyes_dice<MAXP, SPP><<<p->dice_grid, 64, 0, p->stream>>>(a, p->d_lcp, p->d_lmeta, p->d_lqloc, p->d_leaf_at, \
+                                                              p->d_qoff, p->d_st, p->d_qcol, p->d_vis)
+  const bool small = a.dice > 0 && a.dice <= 8;
+  if (a.spp == 4) {
+    if (small) PIKO_DICE(9, 4);
+    else PIKO_DICE(34, 4);
+  } else {
+    if (small) PIKO_DICE(9, 1);
+    else PIKO_DICE(34, 1);
+  }
+#undef PIKO_DICE
   if (ev) (void)hipEventRecord(ev[2], p->stream);
   k_reyes_resolve<<<(unsigned)cdiv((long long)a.W * a.H, 256), 256, 0, p->stream>>>(a, p->d_vis, p->d_qcol,
                                                                                     p->d_rgba, p->d_depth, p->d_sz);
@@ -715,7 +834,7 @@ int make_pipe(piko_reyes_pipe **out, const piko_constant_state *cs, const float 
   a.dice = opt->dice;
   a.spp = opt->spp;
   a.disp = opt->displacement;
-  a.roots_per_wave = opt->roots_per_wave > 0 ? opt->roots_per_wave : 4;
+  a.roots_per_wave = opt->roots_per_wave > 0 ? opt->roots_per_wave : 1;
   a.nroots = count;
   a.G = (int)std::max<long long>(cdiv(count, a.roots_per_wave), 1);
   a.front_cap = 64 * a.roots_per_wave;
@@ -752,6 +871,7 @@ int make_pipe(piko_reyes_pipe **out, const piko_constant_state *cs, const float 
   if (rc == PIKO_OK) rc = ralloc(p, &p->d_lvlpre, NL);
   if (rc == PIKO_OK) rc = ralloc(p, &p->d_off, NL);
   if (rc == PIKO_OK) rc = ralloc(p, &p->d_qoff, NL);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->d_lvltot, 2 * kLevels);
   if (rc == PIKO_OK) rc = ralloc(p, &p->d_wstat, a.G);
   if (rc == PIKO_OK) rc = ralloc(p, &p->d_st, 1);
   if (rc == PIKO_OK) rc = ralloc(p, &p->d_vis, ns);
@@ -922,6 +1042,7 @@ int piko_reyes_destroy(piko_reyes_pipe *p) {
   rfree(p->d_lvlpre);
   rfree(p->d_off);
   rfree(p->d_qoff);
+  rfree(p->d_lvltot);
   rfree(p->d_wstat);
   rfree(p->d_st);
   rfree(p->d_vis);
