@@ -63,6 +63,19 @@ def cpu_baseline(scene, T, S, budget_s=12.0):
                       "single-threaded oracle, host %s" % (len(times), T, med, cpu_model())}
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE measurements (profiles/traffic.json,
+    each counter scaled by its reading on a copy of known size), or None."""
+    path = os.path.join(REPO, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            entry = json.load(f)[workload][kernel]
+    except (OSError, KeyError, ValueError):
+        return None, None
+    return int(entry["bytes_per_launch"]), "profiles/traffic.json: %s" % entry["summary"]
+
+
 def copy_calibration(mb):
     """A plain device-to-device copy of known size through the runtime shim:
     rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this run give the counters'
@@ -249,11 +262,21 @@ def main():
         pipe.destroy()
 
     T, dt, kernels, frame_ms, st = best
-    # dominant kernel of the frame by measured time
-    agg = {}
-    for n, m in kernels:
-        agg[n] = agg.get(n, 0.0) + m
-    dom = max(agg, key=agg.get)
+    # dominant kernel: the longest single launch of the frame (HIP events on
+    # the pipe's stream); the raster kernel at every bin size measured so far
+    dom, dom_ms = max(kernels, key=lambda k: k[1])
+    dom_key = "%s<%d>" % (dom, T) if dom == "k_raster" else dom
+    if dom == "k_raster":
+        # framebuffer written once (colour + depth), each emitted triangle's
+        # setup (3 fixed-point xy pairs, 3 depths, 3 packed colours) read once,
+        # each (triangle, bin) queue entry read once
+        units = {"pixels": scene.W * scene.H, "emitted_triangles": st["emitted"], "queue_entries": st["pairs"]}
+        alg_k = 8 * units["pixels"] + 48 * units["emitted_triangles"] + 4 * units["queue_entries"]
+    else:
+        units = {"frame": 1}
+        alg_k = alg_bytes
+    achieved_k = alg_k / (dom_ms / 1e3) / 1e9
+    traffic, traffic_src = measured_traffic(dom_key, "heightfield-2M-1080p")
     frame_s = frame_ms / 1e3
     achieved = alg_bytes / frame_s / 1e9
     out = {
@@ -274,10 +297,14 @@ def main():
                    "triangles": scene.n, "W": scene.W, "H": scene.H, "subpixel_bits": S, "bin_size": T,
                    "emitted": st["emitted"], "pairs": st["pairs"],
                    "sweep": sweep},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "whole frame (%d launches); dominant kernel %s %.4f ms" % (len(kernels), dom, agg[dom]),
-                     "algorithmic_bytes_per_frame": alg_bytes},
+        "roofline": {"bound": "hbm", "achieved": round(achieved_k, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved_k / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": dom_key, "kernel_ms": round(dom_ms, 4),
+                     "algorithmic_bytes_per_launch": alg_k, "units": units,
+                     "traffic_source": traffic_src},
+        "frame_roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(achieved / HBM_PEAK_GBS, 4), "event_frame_ms": round(frame_ms, 4),
+                           "launches": len(kernels), "algorithmic_bytes_per_frame": alg_bytes},
     }
     if not args.no_cpu:
         log("timing the CPU oracle on a bounded sample")

@@ -1,0 +1,22 @@
+/*
+ * piko/piko.h -- the C++ pipeline surface of the MI355X Piko backend.
+ *
+ * The user-facing API of Piko (api/include/piko/{pipe,stage,builtinTypes}.h):
+ * Stage<binX, binY, threadsPerTile, In, Out> classes with assignBin /
+ * schedule / process phases and their directives, PikoPipe, pikoConnect,
+ * PikoScreen and PikoArray, ConstantState and MutableState.  A pipe class
+ * written against it is compiled by pikoc_hip (--target=HIP), which emits
+ * __pikoDefines.h and __pikoCompiledPipe.h: the pipe's allocate / prepare /
+ * run_single / run / destroy methods over the C-ABI of libpiko_hip.so
+ * (include/piko_hip.h).  The stage bodies are not compiled to device code:
+ * pikoc_hip maps the stage graph onto the library's hand-written gfx950
+ * kernels.
+ */
+#ifndef PIKO_PIKO_H
+#define PIKO_PIKO_H
+
+#include "piko/builtinTypes.h"
+#include "piko/pipe.h"
+#include "piko/stage.h"
+
+#endif

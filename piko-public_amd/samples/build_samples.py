@@ -1,0 +1,46 @@
+"""Builds the C++ samples: pikoc_hip --target=HIP on each pipe header, then
+g++ on the host program against libpiko_hip.so and libpiko_scene.so.
+
+  python piko-public_amd/samples/build_samples.py
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+REPO = os.path.dirname(PKG)
+sys.path.insert(0, PKG)
+
+# (sample dir, pipe header, binary, defines)
+SAMPLES = [
+    ("raster", "rasterPipe.h", "raster_sample", {}),
+    ("raster", "rasterPipe.h", "raster_sample_world", {"RASTER_WORLD_INPUT": "1"}),
+    ("reyes", "reyesPipe.h", "reyes_sample", {}),
+]
+
+
+def build_one(sub, header, binary, defines, log=None):
+    from pikoc_hip import compile_pipe
+    src = os.path.join(HERE, sub)
+    gen = os.path.join(src, "gen_" + binary)
+    inc = os.path.join(PKG, "include")
+    compile_pipe(os.path.join(src, header), gen, [inc], defines, log=log)
+    out = os.path.join(src, binary)
+    libdir = os.path.join(PKG, "piko_amd")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(REPO, "include"), "-I", inc, "-I", gen, "-I", src]
+    cmd += ["-D%s=%s" % kv for kv in defines.items()]
+    cmd += [os.path.join(src, "main.cpp"), "-L", libdir, "-lpiko_hip", "-lpiko_scene",
+            "-Wl,-rpath,$ORIGIN/../../piko_amd", "-Wl,-rpath-link,/opt/rocm/lib", "-o", out + ".tmp"]
+    subprocess.run(cmd, check=True)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_all(log=None):
+    return [build_one(*s, log=log) for s in SAMPLES]
+
+
+if __name__ == "__main__":
+    for p in build_all(log=print):
+        print(p)

@@ -1,0 +1,180 @@
+"""pikoc_hip, the --target=HIP code generator: the pipe summary its frontend
+reads from the sample pipes, the kernel plan with pikoc's fusion rules
+(PikoSummary.cpp:686-755), the emitted host code, the error paths, and that
+the samples compile against the C-ABI."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(REPO, "piko-public_amd")
+INC = os.path.join(PKG, "include")
+SAMPLES = os.path.join(PKG, "samples")
+
+from pikoc_hip import BackendError, FrontendError, compile_pipe, parse_pipe  # noqa: E402
+from pikoc_hip.frontend import eval_number  # noqa: E402
+from pikoc_hip.summary import PipeSummary, StageSummary  # noqa: E402
+
+
+def raster_summary(defines=None):
+    return parse_pipe(os.path.join(SAMPLES, "raster", "rasterPipe.h"), [INC], defines)
+
+
+def test_raster_summary():
+    ps = raster_summary()
+    assert ps.name == "RasterPipe"
+    assert [s.name for s in ps.order] == ["vertexShader", "raster"]
+    vs, ra = ps.order
+    assert vs.binsize == (32, 32) and vs.threads_per_tile == 512
+    assert ra.binsize == (8, 8) and ra.threads_per_tile == 64
+    assert (vs.prim_in, vs.prim_out, ra.prim_in, ra.prim_out) == ("raster_wtri", "raster_stri", "raster_stri", "Pixel")
+    assert vs.sched_policy == ra.sched_policy == "LoadBalance" and vs.sched_trivial
+    assert vs.assign_policy == "Custom"
+    assert vs.next_by_port == {0: [ra]} and ra.prev == [vs]
+    assert [s.name for s in ps.drain] == ["raster"]
+    assert (ps.const_state_type, ps.mutable_state_type, ps.input_type) == ("ConstantState", "MutableState",
+                                                                           "raster_wtri")
+    assert "VTX_PRETRANSFORM" in ps.macros
+    assert [f[1] for f in ps.input_fields] == ["id", "vertPos", "vertColor"]
+    # the reference's plan for this pipe: kernel0 = VS.assignBin over the input,
+    # kernel1 = VS.process emitting into Raster.assignBin, kernel2 = Raster.process
+    plan = ps.kernel_plan()
+    assert [k for k, _ in plan] == ["kernel0", "kernel1", "kernel2"]
+    assert plan[1][1] == ["vertexShader.process", "emit[0] -> raster.assignBin"]
+
+
+def test_world_input_define():
+    ps = raster_summary({"RASTER_WORLD_INPUT": "1"})
+    assert "VTX_PRETRANSFORM" not in ps.macros
+    assert [f[1] for f in ps.input_fields] == ["id", "vertPos", "vertNormal"]
+
+
+def test_reyes_summary_and_loop():
+    ps = parse_pipe(os.path.join(SAMPLES, "reyes", "reyesPipe.h"), [INC])
+    assert [s.name for s in ps.order] == ["split", "dice", "shade"]
+    split, dice, shade = ps.order
+    assert split.loop and ps.has_loop and not dice.loop
+    assert split.next_by_port == {0: [dice], 1: [split]}
+    assert split.binsize == (0, 0) and dice.binsize == shade.binsize == (64, 64)
+    assert [s.name for s in ps.drain] == ["shade"]
+    assert [k for k, _ in ps.kernel_plan()] == ["kernel0", "kernel1", "kernel2", "kernel3"]
+
+
+def stage(name, T=8, sched="LoadBalance", assign="InBin", tile=0, wait="None"):
+    s = StageSummary(name, name)
+    s.binsize, s.sched_policy, s.assign_policy, s.tile_split_size, s.wait_policy = (T, T), sched, assign, tile, wait
+    s.sched_trivial = True
+    return s
+
+
+def test_can_fuse_rules():
+    f = PipeSummary.can_fuse
+    assert f(stage("a"), stage("b"), [])                                   # LoadBalance + InBin, same bins
+    assert not f(stage("a"), stage("b", T=16), [])                         # different bins
+    assert not f(stage("a"), stage("b", assign="Custom"), [])              # LoadBalance needs InBin
+    assert not f(stage("a"), stage("a"), [])                               # same stage type
+    assert f(stage("a", sched="Serialize", assign="Custom"), stage("b", sched="Serialize", assign="Custom"), [])
+    assert f(stage("a", sched="DirectMap"), stage("b", sched="DirectMap"), [])
+    assert not f(stage("a", sched="DirectMap"), stage("b", sched="LoadBalance"), [])
+    assert not f(stage("a", tile=2), stage("b", tile=4), [])
+    assert f(stage("a", sched="All"), stage("b", sched="All"), [])
+    assert not f(stage("a", sched="All"), stage("b", sched="All", wait="EndStage"), [])
+
+
+def test_fused_plan(tmp_path):
+    # two InBin stages under LOAD_BALANCE with equal bins fuse into one kernel
+    (tmp_path / "p.h").write_text("""
+#include "piko/pipe.h"
+struct In : public Primitive { cvec3f vertPos[3]; };
+class A : public Stage<16, 16, 64, In, In> {
+  void assignBin(In p) { this->assignToBin(p, PREVIOUS_BINS); }
+  void schedule(int b) { specifySchedule(LOAD_BALANCE); }
+  void process(In p) {}
+};
+class B : public Stage<16, 16, 64, In, Pixel> {
+  void assignBin(In p) { this->assignToBin(p, PREVIOUS_BINS); }
+  void schedule(int b) { specifySchedule(LOAD_BALANCE); specifyWait(END_BIN); }
+  void process(In p) {}
+};
+class P : public PikoPipe {
+  A a; B b;
+ public:
+  P() { pikoConnect(a, b, 0, 0); }
+  void run(ConstantState &c, MutableState &m, In *in, int n);
+  void allocate(ConstantState &c, MutableState &m, In *in, int n);
+  void prepare(); void run_single(); void destroy();
+};
+""")
+    ps = parse_pipe(str(tmp_path / "p.h"), [INC])
+    a, b = ps.order
+    assert a.assign_policy == "InBin" and a.assign_trivial and b.wait_policy == "EndBin" and b.sched_trivial
+    plan = ps.kernel_plan()
+    assert len(plan) == 2 and plan[1][1] == ["a.process", "emit[0] -> b.assignBin", "b.process"]
+    # not a graph the kernel library implements (bins 16 is fine, but the
+    # raster stage must drain to the screen from a different input type)
+    with pytest.raises(BackendError):
+        compile_pipe(str(tmp_path / "p.h"), str(tmp_path / "out"), [INC], log=None)
+
+
+def test_source_errors(tmp_path):
+    (tmp_path / "bad.h").write_text("class P : public PikoPipe { P() {} };\n")
+    with pytest.raises(FrontendError):
+        parse_pipe(str(tmp_path / "bad.h"), [INC])
+    (tmp_path / "bad2.h").write_text("#ifdef X\nclass P : public PikoPipe {};\n")
+    with pytest.raises(FrontendError):
+        parse_pipe(str(tmp_path / "bad2.h"), [INC])
+
+
+def test_constant_expressions():
+    m = {"LG": "3", "T": "(1 << LG)", "F": "16.0f"}
+    assert eval_number("T * 2 + 1", m) == 17
+    assert eval_number("F / 4", m) == 4.0
+    with pytest.raises(FrontendError):
+        eval_number("__import__('os')", m)
+
+
+def test_emitted_raster_host_code(tmp_path):
+    be = compile_pipe(os.path.join(SAMPLES, "raster", "rasterPipe.h"), str(tmp_path), [INC], log=None)
+    assert be.kind == "raster" and be.T == 8 and be.S == 8 and be.pretransformed
+    h = (tmp_path / "__pikoCompiledPipe.h").read_text()
+    for needle in ("inline void RasterPipe::allocate(ConstantState &constState", "opt.bin_size = 8;",
+                   "opt.subpixel_bits = 8;", "PIKO_INPUT_PRETRANSFORMED", "piko_raster_prepare(piko_hip_pipe_",
+                   "piko_raster_run_single(piko_hip_pipe_)", "piko_raster_destroy(piko_hip_pipe_)",
+                   "input[i].vertColor[v]"):
+        assert needle in h, needle
+    d = (tmp_path / "__pikoDefines.h").read_text()
+    assert "#define __PIKOC_HIP__" in d and "piko_raster_pipe *piko_hip_pipe_" in d
+    k = (tmp_path / "__pikoCompiledPipe.kernels").read_text()
+    assert "k_raster<8>" in k and "k_vs_count" in k
+
+
+def test_emitted_reyes_host_code(tmp_path):
+    be = compile_pipe(os.path.join(SAMPLES, "reyes", "reyesPipe.h"), str(tmp_path), [INC], log=None)
+    assert be.kind == "reyes" and be.threshold == 16.0 and be.dice == 8 and be.spp == 4
+    h = (tmp_path / "__pikoCompiledPipe.h").read_text()
+    assert "piko_reyes_allocate(&piko_hip_pipe_" in h and "opt.dice = 8;" in h and "input[i].CP[k].w" in h
+
+
+def test_cli_exit_codes(tmp_path):
+    env = dict(os.environ, PYTHONPATH=PKG)
+    r = subprocess.run([sys.executable, "-m", "pikoc_hip", "--target=HIP", "-I", INC, "-o", str(tmp_path),
+                        os.path.join(SAMPLES, "raster", "rasterPipe.h")], env=env, capture_output=True, text=True)
+    assert r.returncode == 0 and "Kernel plan" in r.stdout
+    r = subprocess.run([sys.executable, "-m", "pikoc_hip", "--target=PTX", "x.h"], env=env, capture_output=True)
+    assert r.returncode == 2
+
+
+@pytest.mark.parametrize("binary,args", [("raster/raster_sample", []),
+                                         ("raster/raster_sample_world", ["--scene", "heightfield"]),
+                                         ("reyes/reyes_sample", [])])
+def test_samples_built_and_fail_loudly_without_gpu(binary, args):
+    path = os.path.join(SAMPLES, binary)
+    if not os.path.exists(path):
+        pytest.skip("samples not built (run __graft_entry__.build())")
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present: the GPU tests run the samples")
+    r = subprocess.run([path, "--runs", "1"] + args, capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "no HIP device" in r.stderr
