@@ -1,14 +1,19 @@
 /*
  * piko_hip.h -- C-ABI of the MI355X (gfx950) Piko backend, libpiko_hip.so.
  *
- * Two layers, both plain C (pointers and sizes, no C++ or torch types):
+ * All plain C (pointers and sizes, no C++ or torch types):
  *
  * 1. The raster pipeline (piko_raster_*): the pipe methods pikoc generates
  *    for RasterPipe (rasterPipelineFixPt/rasterPipe.h:16-45) --
  *    allocate/prepare/run_single/destroy plus PikoScreen::getData -- backed
- *    by hand-written HIP kernels instead of generated PTX.
+ *    by hand-written HIP kernels instead of generated PTX; its batched form
+ *    (piko_batch_*: many screens in one kernel sequence) and a serving loop
+ *    over resident scenes (piko_server_*).
  *
- * 2. The runtime shim (piko_rt_*): a 1:1 functional cover of the CUDA driver
+ * 2. The Reyes pipeline (piko_reyes_*): the same methods for ReyesPipe
+ *    (reyesPipe/reyesPipe.h:12-40).
+ *
+ * 3. The runtime shim (piko_rt_*): a 1:1 functional cover of the CUDA driver
  *    calls the generated host code of the PTX backend makes
  *    (pikoc/src/Backend/PTXBackend.cpp), over hipModule*, for code emitted
  *    against a HIP target.
