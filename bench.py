@@ -220,7 +220,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--bins", default="8,16,32,64")
+    ap.add_argument("--bins", default="8,16,32,64", help="bin sizes of the binned raster mode")
+    ap.add_argument("--modes", default="binned,direct", help="raster modes to sweep")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="all", choices=["raster", "batch", "serve", "reyes", "all"])
     ap.add_argument("--serve-tris", type=int, default=300_000, help="triangles per resident scene when serving")
@@ -245,8 +246,15 @@ def main():
     alg_bytes = scenes.algorithmic_bytes(scene)
     sweep = {}
     best = None
-    for T in [int(t) for t in args.bins.split(",")]:
-        pipe = piko_amd.RasterPipe().allocate(scene, T, S)
+    runs = []
+    modes = args.modes.split(",")
+    if "binned" in modes:
+        runs += [("binned", int(t)) for t in args.bins.split(",")]
+    if "direct" in modes:
+        runs.append(("direct", 16))  # the bin size only shapes the on-demand queues
+    for mode, T in runs:
+        rm = piko_amd.RASTER_BINNED if mode == "binned" else piko_amd.RASTER_DIRECT
+        pipe = piko_amd.RasterPipe().allocate(scene, T, S, raster_mode=rm)
         pipe.prepare()
         pipe.run_frames(max(args.warmup, 1))
         t0 = time.perf_counter()
@@ -254,14 +262,15 @@ def main():
         dt = (time.perf_counter() - t0) / args.steps
         kernels, frame_ms = pipe.profile_frames(max(5, min(args.steps, 20)))
         st = pipe.stats()
-        sweep[T] = {"ms_per_frame": round(dt * 1e3, 4), "event_frame_ms": round(frame_ms, 4),
-                    "pairs": st["pairs"], "kernels_ms": [[n, round(m, 4)] for n, m in kernels]}
-        log("bin %d: %.3f ms/frame (events %.3f ms), pairs %d" % (T, dt * 1e3, frame_ms, st["pairs"]))
+        key = "binned-%d" % T if mode == "binned" else "direct"
+        sweep[key] = {"ms_per_frame": round(dt * 1e3, 4), "event_frame_ms": round(frame_ms, 4),
+                      "pairs": st["pairs"], "kernels_ms": [[n, round(m, 4)] for n, m in kernels]}
+        log("%s: %.3f ms/frame (events %.3f ms), kernels %s" % (key, dt * 1e3, frame_ms, sweep[key]["kernels_ms"]))
         if best is None or dt < best[1]:
-            best = (T, dt, kernels, frame_ms, st)
+            best = (T, dt, kernels, frame_ms, st, mode)
         pipe.destroy()
 
-    T, dt, kernels, frame_ms, st = best
+    T, dt, kernels, frame_ms, st, mode = best
     # dominant kernel: the longest single launch of the frame (HIP events on
     # the pipe's stream); the raster kernel at every bin size measured so far
     dom, dom_ms = max(kernels, key=lambda k: k[1])
@@ -294,7 +303,8 @@ def main():
         "dtype": "int64 edge functions + f32 shading",
         "data": "synthetic",
         "config": {"workload": "heightfield-2M-1080p raster (VS->bin->raster->depth<=->Gouraud)",
-                   "triangles": scene.n, "W": scene.W, "H": scene.H, "subpixel_bits": S, "bin_size": T,
+                   "triangles": scene.n, "W": scene.W, "H": scene.H, "subpixel_bits": S, "raster_mode": mode,
+                   "bin_size": T if mode == "binned" else None,
                    "emitted": st["emitted"], "pairs": st["pairs"],
                    "sweep": sweep},
         "roofline": {"bound": "hbm", "achieved": round(achieved_k, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -64,6 +64,17 @@ typedef struct {
 
 enum { PIKO_INPUT_PRETRANSFORMED = 0, PIKO_INPUT_WORLD = 1 };
 
+/* How a frame is rasterised.  Both give the reference's image bit for bit.
+ *   BINNED: vertex stage -> (triangle, bin) queues sorted by bin -> one raster
+ *           wave per bin (the reference's schedule, rasterPipe.h:16-45).
+ *   DIRECT: the vertex stage rasterises small triangles itself into a per-pixel
+ *           (depth, submission index) visibility word; large ones are drawn
+ *           in 8x8 tiles; a resolve shades the winner.  No queues; the bin
+ *           queues are built on demand when asked for (piko_raster_get_bins).
+ *   AUTO:   DIRECT.  The environment variable PIKO_RASTER_MODE=binned|direct
+ *           overrides AUTO. */
+enum { PIKO_RASTER_AUTO = 0, PIKO_RASTER_BINNED = 1, PIKO_RASTER_DIRECT = 2 };
+
 typedef struct {
   int bin_size;           /* raster bin edge T in pixels: 8, 16, 32 or 64 */
   int subpixel_bits;      /* S in [1, 8] (reference: 4) */
@@ -71,6 +82,7 @@ typedef struct {
   long long pair_capacity;/* (triangle, bin) queue capacity; 0 = automatic,
                              grows on demand between frames */
   int waves_per_cu;       /* persistent raster waves per CU; 0 = default (20) */
+  int raster_mode;        /* PIKO_RASTER_*; 0 = AUTO */
 } piko_raster_options;
 
 typedef struct piko_raster_pipe piko_raster_pipe;
@@ -119,14 +131,16 @@ int piko_raster_get_depth(piko_raster_pipe *pipe, float *depth);
 
 /* Raster-stage bin queues of the last frame (Stage::getData, stage.h:
  * 331-333): CSR, bin_start[nbins+1], bin_tris[pairs] in submission order.
- * Returns the pair count (>=0) or a piko_status; writes at most cap entries. */
+ * Returns the pair count (>=0) or a piko_status; writes at most cap entries.
+ * After a DIRECT frame the queues are built here, on the pipe's stream. */
 long long piko_raster_get_bins(piko_raster_pipe *pipe, int64_t *bin_start, int32_t *bin_tris,
                                long long cap);
 
 typedef struct {
   long long triangles;       /* input count */
   long long emitted;         /* triangles the vertex stage emitted */
-  long long pairs;           /* (triangle, bin) queue entries */
+  long long pairs;           /* (triangle, bin) queue entries (built on demand
+                                after a DIRECT frame) */
   long long pair_capacity;
   int bins_x, bins_y;
   long long device_bytes;    /* device memory held by the pipe */
@@ -159,6 +173,7 @@ typedef struct {
   long long max_tris;      /* initial per-batch triangle capacity (grows) */
   long long max_pixels;    /* initial per-batch pixel capacity (grows) */
   long long pair_capacity; /* 0 = automatic */
+  int raster_mode;         /* PIKO_RASTER_* for every batch; 0 = AUTO */
 } piko_batch_options;
 
 /* Creates a batch pipe over `input_tris` resident triangles (pos/attr as for

@@ -79,9 +79,12 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh, ui
 // ---- frame state ----------------------------------------------------------
 
 struct FrameState {
-  uint32_t total;     // (triangle, bin) pairs of this frame
-  uint32_t overflow;  // sticky: pairs exceeded the queue capacity
-  uint32_t pad[2];
+  uint32_t total;        // (triangle, bin) pairs of this frame (binned path)
+  uint32_t overflow;     // sticky: bit 0 pairs exceeded the queues, bit 1 the large-triangle list,
+                         // bit 2 the large triangles' 8x8 tiles exceeded 2^32 (cannot grow)
+  uint32_t large;        // large triangles appended this frame (direct path)
+  uint32_t large_tiles;  // their 8x8 tiles
+  uint32_t large_peak;   // most large triangles of one frame since the host last looked
 };
 
 // One 1024-thread block scans n values in place (exclusive) and returns the
@@ -213,7 +216,7 @@ __global__ void __launch_bounds__(1024) k_scan_tiles(uint32_t *__restrict__ blk_
   const uint32_t tot = block_scan_inplace(blk_pairs, ntiles, [](int i) { return (size_t)i; }, sh);
   if (threadIdx.x == 0) {
     fs->total = tot;
-    if ((long long)tot > cap) fs->overflow = 1u;
+    if ((long long)tot > cap) fs->overflow |= 1u;
   }
 }
 
@@ -699,6 +702,286 @@ __global__ void __launch_bounds__(kRT) k_raster(const int4 *__restrict__ geo, co
   }
 }
 
+// ---- direct path --------------------------------------------------------
+//
+// For triangles whose clipped rounded bbox (rasterMacros.h:48-49, 58-63) has
+// at most kDirectMaxPx pixels -- nearly all of them in dense meshes -- the
+// vertex stage rasterises the triangle itself: the (triangle, pixel)
+// candidates of a wave's 64 triangles are spread evenly over its lanes, and
+// every covered pixel takes a 64-bit atomicMin of (z bits, 0xFFFFFFFE -
+// triangle index) in a per-pixel visibility buffer.  Larger triangles are
+// listed and rasterised in 8x8 tiles by every wave of k_large_raster.  A
+// per-pixel resolve shades the winner (the same arithmetic k_raster uses)
+// and resets the buffer.  With the submission index in the key the result
+// is the sequential one whatever the order of the atomics, so no bin queues
+// are needed; the binned path remains for the queues themselves.
+
+constexpr int kDirThreads = 256;     // one triangle per lane
+constexpr int kDirectMaxPx = 1024;   // larger bboxes take the tile path
+constexpr int kLargeTile = 8;
+
+struct DirStage {  // one wave's triangles
+  uint32_t pre[64];
+  uint32_t x0y0[64];  // x0 | y0 << 16
+  uint32_t w[64];
+  int xy[6][64];
+  float zr[4][64];  // z2, z0mz2, z1mz2, 1/denom
+  uint32_t small[64];
+  uint32_t tri[64];
+  uint32_t fbw[64];             // framebuffer width of the triangle's job
+  unsigned long long fb[64];    // framebuffer base of the triangle's job
+};
+
+__device__ __forceinline__ void wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Rounded pixel bbox (rasterMacros.h:48-49, 58-63) clipped to the screen.
+__device__ __forceinline__ void clip_rect(const int *p, int S, int W, int H, int &x0, int &y0, int &x1, int &y1) {
+  x0 = max((imin3(p[0], p[2], p[4]) & (int)0xffffff00) >> S, 0);
+  y0 = max((imin3(p[1], p[3], p[5]) & (int)0xffffff00) >> S, 0);
+  x1 = min(((imax3(p[0], p[2], p[4]) + 0xff) & (int)0xffffff00) >> S, W);
+  y1 = min(((imax3(p[1], p[3], p[5]) + 0xff) & (int)0xffffff00) >> S, H);
+}
+
+__device__ __forceinline__ int job_by_pixel(const FrameArgs *__restrict__ jobs, int njobs, unsigned long long g) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].fb_base <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int KIND, bool MULTI>
+__global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restrict__ pos, const void *__restrict__ attr,
+                                                            int n, int S, FrameArgs fa, const FrameArgs *__restrict__ jobs,
+                                                            int njobs, int4 *__restrict__ geo,
+                                                            uint4 *__restrict__ colr,
+                                                            unsigned long long *__restrict__ vis,
+                                                            uint32_t *__restrict__ large, FrameState *__restrict__ fs,
+                                                            uint32_t large_cap, uint32_t *__restrict__ blk_keep) {
+  __shared__ DirStage st[kDirThreads / 64];
+  __shared__ uint32_t s_keep[kDirThreads / 64];
+  const int t = threadIdx.x, lane = t & 63;
+  DirStage &s = st[t >> 6];
+  const int i = blockIdx.x * kDirThreads + t;
+  uint32_t npx = 0;
+  bool big = false, kept = false;
+  if (i < n) {
+    const FrameArgs &ja = MULTI ? jobs[job_search(jobs, njobs, (uint32_t)i, false)] : fa;
+    const size_t src = MULTI ? (size_t)ja.src_base + (size_t)(i - (int)ja.vbase) : (size_t)i;
+    int p[6];
+    float z[3];
+    uint32_t c[3];
+    const float *v = pos + 9 * src;
+    if (KIND == PIKO_INPUT_PRETRANSFORMED) {
+      const uint32_t *cc = (const uint32_t *)attr + 3 * src;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        p[2 * q] = fixpt(v[3 * q], ja.S);
+        p[2 * q + 1] = fixpt(v[3 * q + 1], ja.S);
+        z[q] = v[3 * q + 2];
+        c[q] = cc[q];
+      }
+    } else {
+      const float *nn = (const float *)attr + 9 * src;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        transform_vertex(v + 3 * q, ja, p[2 * q], p[2 * q + 1], z[q]);
+        c[q] = light_vertex(nn[3 * q], nn[3 * q + 1], nn[3 * q + 2], ja);
+      }
+    }
+    if (vs_keep(p, ja.W, ja.H, ja.S)) {
+      kept = true;
+      Edges e;
+      e.set(p);
+      const float rcp = 1.0f / (float)e.denom();  // raster.pikostage:174
+      const float z0mz2 = z[0] - z[2], z1mz2 = z[1] - z[2];
+      geo[3 * (size_t)i + 0] = make_int4(p[0], p[1], p[2], p[3]);
+      geo[3 * (size_t)i + 1] = make_int4(p[4], p[5], __float_as_int(z[2]), __float_as_int(z0mz2));
+      geo[3 * (size_t)i + 2] = make_int4(__float_as_int(z1mz2), __float_as_int(rcp), 0, 0);
+      colr[i] = make_uint4(c[0], c[1], c[2], 0u);
+      int x0, y0, x1, y1;
+      clip_rect(p, ja.S, ja.W, ja.H, x0, y0, x1, y1);
+      const int w = max(0, x1 - x0), h = max(0, y1 - y0);
+      npx = (uint32_t)w * (uint32_t)h;
+      if (npx > (uint32_t)kDirectMaxPx) {
+        big = true;
+        npx = 0;
+      } else {
+        s.x0y0[lane] = (uint32_t)x0 | ((uint32_t)y0 << 16);
+        s.w[lane] = (uint32_t)w;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) s.xy[k][lane] = p[k];
+        s.zr[0][lane] = z[2];
+        s.zr[1][lane] = z0mz2;
+        s.zr[2][lane] = z1mz2;
+        s.zr[3][lane] = rcp;
+        s.small[lane] = small_bbox(p, ja.S);
+        s.tri[lane] = (uint32_t)i;
+        s.fbw[lane] = (uint32_t)ja.W;
+        s.fb[lane] = MULTI ? ja.fb_base : 0ull;
+      }
+    }
+  }
+  const unsigned long long km = __ballot(kept);
+  if (lane == 0) s_keep[t >> 6] = (uint32_t)__popcll(km);
+  // large triangles: one append per wave
+  const unsigned long long bm = __ballot(big);
+  if (bm) {
+    const int first = __ffsll((long long)bm) - 1;
+    uint32_t base = 0;
+    if (lane == first) base = atomicAdd(&fs->large, (uint32_t)__popcll(bm));
+    base = __shfl(base, first, 64);
+    if (big) {
+      const uint32_t at = base + (uint32_t)__popcll(bm & ((1ull << lane) - 1ull));
+      if (at < large_cap) large[at] = (uint32_t)i;
+      else atomicOr(&fs->overflow, 2u);
+    }
+  }
+  const uint32_t inc = wave_incl_scan(npx);
+  s.pre[lane] = inc - npx;
+  const uint32_t tot = __shfl(inc, 63, 64);
+  __syncthreads();
+  if (t == 0) {
+    uint32_t kc = 0;
+#pragma unroll
+    for (int w = 0; w < kDirThreads / 64; ++w) kc += s_keep[w];
+    blk_keep[blockIdx.x] = kc;
+  }
+  for (uint32_t cidx = lane; cidx < tot; cidx += 64) {
+    int o = 0;  // owner: largest o with pre[o] <= cidx (o + step stays <= 63)
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1)
+      if (s.pre[o + step] <= cidx) o += step;
+    const uint32_t k = cidx - s.pre[o];
+    const int w = (int)s.w[o];
+    const int q = (int)(((float)k + 0.5f) * __builtin_amdgcn_rcpf((float)w));
+    const int r = (int)k - q * w;
+    const uint32_t org = s.x0y0[o];
+    const int px = (int)(org & 0xffffu) + r, py = (int)(org >> 16) + q;
+    const int xy[6] = {s.xy[0][o], s.xy[1][o], s.xy[2][o], s.xy[3][o], s.xy[4][o], s.xy[5][o]};
+    float alpha, beta;
+    if (covered_at(xy, s.small[o] != 0, px, py, S, s.zr[3][o], alpha, beta)) {
+      const float z = interp(s.zr[1][o], s.zr[2][o], s.zr[0][o], alpha, beta);  // raster.pikostage:272
+      atomicMin((long long *)&vis[s.fb[o] + (size_t)py * s.fbw[o] + px], (long long)vis_key(z, s.tri[o]));
+    }
+  }
+}
+
+// 8x8 tiles of every listed large triangle: exclusive offsets and the total.
+__global__ void __launch_bounds__(1024) k_large_scan(const int4 *__restrict__ geo, const uint32_t *__restrict__ large,
+                                                     uint32_t *__restrict__ tile_off, FrameState *__restrict__ fs,
+                                                     uint32_t large_cap, FrameArgs fa, const FrameArgs *__restrict__ jobs,
+                                                     int njobs, int multi) {
+  __shared__ uint32_t sh[1024 / 64 + 1];
+  __shared__ unsigned long long s_all;
+  if (threadIdx.x == 0) s_all = 0;
+  __syncthreads();
+  const int nl = (int)min(fs->large, large_cap);
+  unsigned long long mine = 0;
+  for (int k = threadIdx.x; k < nl; k += 1024) {
+    const uint32_t tri = large[k];
+    const FrameArgs &ja = multi ? jobs[job_search(jobs, njobs, tri, false)] : fa;
+    const int4 a = geo[3 * (size_t)tri], b = geo[3 * (size_t)tri + 1];
+    const int p[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
+    int x0, y0, x1, y1;
+    clip_rect(p, ja.S, ja.W, ja.H, x0, y0, x1, y1);
+    const uint32_t nt = (uint32_t)((x1 - x0 + kLargeTile - 1) / kLargeTile) * (uint32_t)((y1 - y0 + kLargeTile - 1) / kLargeTile);
+    tile_off[k] = nt;
+    mine += nt;
+  }
+  atomicAdd(&s_all, mine);
+  __syncthreads();
+  const bool fits = s_all <= 0xFFFFFFFFull;  // tile numbers are 32-bit
+  const uint32_t tot = block_scan_inplace(tile_off, nl, [](int i) { return (size_t)i; }, sh);
+  if (threadIdx.x == 0) {
+    fs->large_tiles = fits ? tot : 0u;
+    if (!fits) fs->overflow |= 4u;
+  }
+}
+
+// One wave per 8x8 tile of a large triangle, over all waves of the grid.
+template <bool MULTI>
+__global__ void __launch_bounds__(256) k_large_raster(const int4 *__restrict__ geo,
+                                                      const uint32_t *__restrict__ large,
+                                                      const uint32_t *__restrict__ tile_off,
+                                                      const FrameState *__restrict__ fs, uint32_t large_cap,
+                                                      FrameArgs fa, const FrameArgs *__restrict__ jobs, int njobs,
+                                                      unsigned long long *__restrict__ vis) {
+  const int nl = (int)min(fs->large, large_cap);
+  const uint32_t total = fs->large_tiles;
+  const int lane = threadIdx.x & 63;
+  const uint32_t nw = gridDim.x * (blockDim.x / 64);
+  for (uint32_t g = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); g < total; g += nw) {
+    int lo = 0, hi = nl - 1;  // largest k with tile_off[k] <= g
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tile_off[mid] <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    const uint32_t tri = large[lo];
+    const FrameArgs &ja = MULTI ? jobs[job_search(jobs, njobs, tri, false)] : fa;
+    const int4 a = geo[3 * (size_t)tri], b = geo[3 * (size_t)tri + 1], c = geo[3 * (size_t)tri + 2];
+    const int p[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
+    int x0, y0, x1, y1;
+    clip_rect(p, ja.S, ja.W, ja.H, x0, y0, x1, y1);
+    const uint32_t tw = (uint32_t)(x1 - x0 + kLargeTile - 1) / kLargeTile;
+    const uint32_t local = g - tile_off[lo];
+    const int ty = (int)(local / tw), tx = (int)(local - (uint32_t)ty * tw);
+    const int px = x0 + tx * kLargeTile + (lane & (kLargeTile - 1)), py = y0 + ty * kLargeTile + lane / kLargeTile;
+    if (px >= x1 || py >= y1) continue;
+    float alpha, beta;
+    if (covered_at(p, small_bbox(p, ja.S), px, py, ja.S, __int_as_float(c.y), alpha, beta)) {
+      const float z = interp(__int_as_float(b.w), __int_as_float(c.x), __int_as_float(b.z), alpha, beta);
+      const unsigned long long fbb = MULTI ? ja.fb_base : 0ull;
+      atomicMin((long long *)&vis[fbb + (size_t)py * ja.W + px], (long long)vis_key(z, tri));
+    }
+  }
+}
+
+// Per pixel: depth = the winner's z, colour = Gouraud of the winner (the clear
+// colour where nothing is drawn); the visibility word is reset for the next
+// frame.
+template <bool MULTI>
+__global__ void __launch_bounds__(256) k_resolve(const int4 *__restrict__ geo, const uint4 *__restrict__ colr,
+                                                 unsigned long long *__restrict__ vis, FrameArgs fa,
+                                                 const FrameArgs *__restrict__ jobs, int njobs, long long npix,
+                                                 uint32_t *__restrict__ color, float *__restrict__ depth,
+                                                 FrameState *__restrict__ fs) {
+  __shared__ float s_u8[256];
+  const int t = threadIdx.x;
+  s_u8[t] = kU8Table.v[t];
+  __syncthreads();
+  if (blockIdx.x == 0 && t == 0) {  // the large list of this frame is consumed
+    fs->large_peak = max(fs->large_peak, fs->large);
+    fs->large = 0;
+  }
+  const long long g = (long long)blockIdx.x * blockDim.x + t;
+  if (g >= npix) return;
+  const FrameArgs &ja = MULTI ? jobs[job_by_pixel(jobs, njobs, (unsigned long long)g)] : fa;
+  const unsigned long long key = vis[g];
+  uint32_t col = kClearColor;
+  if (key != kVisSentinel) {
+    vis[g] = kVisSentinel;
+    const long long local = MULTI ? g - (long long)ja.fb_base : g;
+    const int py = (int)(local / ja.W), px = (int)(local - (long long)py * ja.W);
+    const uint32_t tri = 0xFFFFFFFEu - (uint32_t)(key & 0xFFFFFFFFull);
+    const int4 a = geo[3 * (size_t)tri], b = geo[3 * (size_t)tri + 1], c = geo[3 * (size_t)tri + 2];
+    const uint4 cc = colr[tri];
+    const int xy[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
+    float alpha = 0.0f, beta = 0.0f;
+    covered_at(xy, small_bbox(xy, ja.S), px, py, ja.S, __int_as_float(c.y), alpha, beta);
+    col = gouraud(s_u8, cc.x, cc.y, cc.z, alpha, beta);
+  }
+  depth[g] = __int_as_float((int)(key >> 32));
+  color[g] = col;
+}
+
 inline int ilog2i(int v) {
   int l = 0;
   while ((1 << l) < v) ++l;
@@ -758,6 +1041,14 @@ struct piko_raster_pipe {
   uint32_t *d_rowtot = nullptr;  // [256] digit totals
   uint32_t *d_color = nullptr;
   float *d_depth = nullptr;
+  unsigned long long *d_vis = nullptr;  // [cap_pix] direct-path visibility words
+  uint32_t *d_keep_dir = nullptr;       // per direct vertex-stage block: emitted triangles
+  uint32_t *d_large = nullptr;          // [large_cap] large triangles of the frame
+  uint32_t *d_tile_off = nullptr;       // [large_cap] their first 8x8 tile
+  uint32_t large_cap = 0;
+  int mode = PIKO_RASTER_DIRECT;        // frame path
+  int last_mode = PIKO_RASTER_DIRECT;   // path of the last enqueued frame
+  bool bins_current = false;            // the bin queues hold the last frame's pairs
   int final_buf = 0;
   int raster_waves = 0;       // persistent raster grid (default)
   int cus = 256;              // compute units of the device
@@ -819,6 +1110,9 @@ void free_tris(piko_raster_pipe *p) {
   dfree(p->d_cnt);
   dfree(p->d_keep);
   dfree(p->d_tile_pairs);
+  dfree(p->d_keep_dir);
+  dfree(p->d_large);
+  dfree(p->d_tile_off);
   p->device_bytes -= p->tri_bytes;
   p->tri_bytes = 0;
 }
@@ -832,12 +1126,22 @@ int alloc_tris(piko_raster_pipe *p, long long ntris) {
   if ((rc = dalloc(p, &p->d_cnt, p->cap_tris, &p->tri_bytes)) != PIKO_OK) return rc;
   if ((rc = dalloc(p, &p->d_keep, tiles, &p->tri_bytes)) != PIKO_OK) return rc;
   if ((rc = dalloc(p, &p->d_tile_pairs, tiles, &p->tri_bytes)) != PIKO_OK) return rc;
+  if ((rc = dalloc(p, &p->d_keep_dir, cdiv(p->cap_tris, kDirThreads), &p->tri_bytes)) != PIKO_OK) return rc;
+  p->large_cap = (uint32_t)std::min<long long>(std::max<long long>({(long long)p->large_cap, p->cap_tris / 32, 4096}), 1 << 30);
+  if ((rc = dalloc(p, &p->d_large, p->large_cap, &p->tri_bytes)) != PIKO_OK) return rc;
+  if ((rc = dalloc(p, &p->d_tile_off, p->large_cap, &p->tri_bytes)) != PIKO_OK) return rc;
   return PIKO_OK;
+}
+
+__global__ void k_fill_vis(unsigned long long *p, long long n) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+    p[k] = kVisSentinel;
 }
 
 void free_pix(piko_raster_pipe *p) {
   dfree(p->d_color);
   dfree(p->d_depth);
+  dfree(p->d_vis);
   p->device_bytes -= p->pix_bytes;
   p->pix_bytes = 0;
 }
@@ -847,7 +1151,9 @@ int alloc_pix(piko_raster_pipe *p, long long npix) {
   int rc;
   if ((rc = dalloc(p, &p->d_color, p->cap_pix, &p->pix_bytes)) != PIKO_OK) return rc;
   if ((rc = dalloc(p, &p->d_depth, p->cap_pix, &p->pix_bytes)) != PIKO_OK) return rc;
-  return PIKO_OK;
+  if ((rc = dalloc(p, &p->d_vis, p->cap_pix, &p->pix_bytes)) != PIKO_OK) return rc;
+  k_fill_vis<<<(unsigned)std::min<long long>(cdiv(p->cap_pix, 256), 8192), 256, 0, p->stream>>>(p->d_vis, p->cap_pix);
+  return hipGetLastError() == hipSuccess ? PIKO_OK : PIKO_ERR_HIP;
 }
 
 void free_bins(piko_raster_pipe *p) {
@@ -877,6 +1183,8 @@ int radix_passes(int nbins) {
 // Kernel names in launch order (the sort section repeats once per pass, the
 // raster once per bin-size class).
 const char *kernel_name(const piko_raster_pipe *p, int k) {
+  static const char *direct[] = {"k_vs_direct", "k_large_scan", "k_large_raster", "k_resolve"};
+  if (p->last_mode == PIKO_RASTER_DIRECT) return k < 4 ? direct[k] : "";
   static const char *head[] = {"k_vs_count", "k_scan_tiles", "k_emit_pairs"};
   static const char *pass[] = {"k_sort_count", "k_sort_rows", "k_sort_scatter"};
   if (k < 3) return head[k];
@@ -905,14 +1213,20 @@ void launch_raster(piko_raster_pipe *p, int T, int bin_lo, int bin_hi, int waves
 #undef PIKO_RASTER_LAUNCH
 }
 
-// Enqueues one frame of the current job set.  ev (optional) receives an event
-// after each kernel.
-int enqueue_frame(piko_raster_pipe *p, hipEvent_t *ev, int *nev) {
-  int e = 0;
-  auto mark = [&]() {
-    if (ev) (void)hipEventRecord(ev[e], p->stream);
-    ++e;
-  };
+// Records an event after each kernel when profiling.
+struct Marks {
+  hipEvent_t *ev;
+  hipStream_t stream;
+  int n = 0;
+  void operator()() {
+    if (ev) (void)hipEventRecord(ev[n], stream);
+    ++n;
+  }
+};
+
+// Vertex stage, (triangle, bin) pairs, sort by bin and bin bounds: the raster
+// stage's queues for the current job set.
+void enqueue_binning(piko_raster_pipe *p, Marks &mark) {
   const int n = p->ntris;
   const int tiles = (int)std::max<long long>(cdiv(n, kVsTile), 1);
   const int nj = (int)p->jobs.size();
@@ -955,12 +1269,66 @@ int enqueue_frame(piko_raster_pipe *p, hipEvent_t *ev, int *nev) {
   const int gstride = (int)std::min<long long>(std::max<long long>(cdiv(p->cap, kThreads * 8), 1), 1024);
   k_bounds<<<gstride, kThreads, 0, p->stream>>>(p->d_keys[cur], p->d_fs, p->cap, p->d_bstart, p->d_bend);
   mark();
-  for (const RasterClass &c : p->classes) {
-    if (p->multi) launch_raster<true>(p, c.T, c.bin_lo, c.bin_hi, c.waves, p->d_vals[cur]);
-    else launch_raster<false>(p, c.T, c.bin_lo, c.bin_hi, c.waves, p->d_vals[cur]);
-    if (c.bin_hi > c.bin_lo) mark();
+}
+
+// Sort-free frame: vertex stage + small-triangle raster, large-triangle tile
+// offsets, large-triangle tiles, resolve.
+void enqueue_direct(piko_raster_pipe *p, Marks &mark) {
+  const int n = p->ntris;
+  const int nj = (int)p->jobs.size();
+  const int blocks = (int)std::max<long long>(cdiv(n, kDirThreads), 1);
+#define PIKO_VSD_LAUNCH(KIND, MULTI)                                                                              \
+  k_vs_direct<KIND, MULTI><<<blocks, kDirThreads, 0, p->stream>>>(p->d_pos, p->d_attr, n, p->S, p->fa, p->d_jobs, \
+                                                                  nj, p->d_geo, p->d_col, p->d_vis, p->d_large,   \
+                                                                  p->d_fs, p->large_cap, p->d_keep_dir)
+  if (p->kind == PIKO_INPUT_PRETRANSFORMED) {
+    if (p->multi) PIKO_VSD_LAUNCH(PIKO_INPUT_PRETRANSFORMED, true);
+    else PIKO_VSD_LAUNCH(PIKO_INPUT_PRETRANSFORMED, false);
+  } else {
+    if (p->multi) PIKO_VSD_LAUNCH(PIKO_INPUT_WORLD, true);
+    else PIKO_VSD_LAUNCH(PIKO_INPUT_WORLD, false);
   }
-  if (nev) *nev = e;
+#undef PIKO_VSD_LAUNCH
+  mark();
+  k_large_scan<<<1, 1024, 0, p->stream>>>(p->d_geo, p->d_large, p->d_tile_off, p->d_fs, p->large_cap, p->fa,
+                                          p->d_jobs, nj, p->multi ? 1 : 0);
+  mark();
+  const int lgrid = std::max(p->cus, 1) * 8;
+  if (p->multi)
+    k_large_raster<true><<<lgrid, 256, 0, p->stream>>>(p->d_geo, p->d_large, p->d_tile_off, p->d_fs, p->large_cap,
+                                                       p->fa, p->d_jobs, nj, p->d_vis);
+  else
+    k_large_raster<false><<<lgrid, 256, 0, p->stream>>>(p->d_geo, p->d_large, p->d_tile_off, p->d_fs, p->large_cap,
+                                                        p->fa, p->d_jobs, nj, p->d_vis);
+  mark();
+  const long long rblocks = std::max<long long>(cdiv(p->npix, 256), 1);
+  if (p->multi)
+    k_resolve<true><<<(unsigned)rblocks, 256, 0, p->stream>>>(p->d_geo, p->d_col, p->d_vis, p->fa, p->d_jobs, nj,
+                                                              p->npix, p->d_color, p->d_depth, p->d_fs);
+  else
+    k_resolve<false><<<(unsigned)rblocks, 256, 0, p->stream>>>(p->d_geo, p->d_col, p->d_vis, p->fa, p->d_jobs, nj,
+                                                               p->npix, p->d_color, p->d_depth, p->d_fs);
+  mark();
+}
+
+// Enqueues one frame of the current job set.  ev (optional) receives an event
+// after each kernel.
+int enqueue_frame(piko_raster_pipe *p, hipEvent_t *ev, int *nev) {
+  Marks mark{ev, p->stream};
+  p->last_mode = p->mode;
+  if (p->mode == PIKO_RASTER_DIRECT) {
+    enqueue_direct(p, mark);
+    p->bins_current = false;
+  } else {
+    enqueue_binning(p, mark);
+    for (const RasterClass &c : p->classes) {
+      if (p->multi) launch_raster<true>(p, c.T, c.bin_lo, c.bin_hi, c.waves, p->d_vals[p->final_buf]);
+      else launch_raster<false>(p, c.T, c.bin_lo, c.bin_hi, c.waves, p->d_vals[p->final_buf]);
+      if (c.bin_hi > c.bin_lo) mark();
+    }
+    p->bins_current = true;
+  }
+  if (nev) *nev = mark.n;
   return hipGetLastError() == hipSuccess ? PIKO_OK : PIKO_ERR_HIP;
 }
 
@@ -970,23 +1338,57 @@ int read_frame_state(piko_raster_pipe *p, FrameState *fs) {
   return PIKO_OK;
 }
 
-// Reads the overflow flag and the pair total; grows the queues when they
-// overflowed.  *rerun = 1 if the frames must be re-run.
+// Reads the overflow flags; grows the pair queues or the large-triangle list
+// when they overflowed.  *rerun = 1 if the frames must be re-run.
 int check_capacity(piko_raster_pipe *p, int *rerun) {
   FrameState fs;
   int rc = read_frame_state(p, &fs);
   if (rc != PIKO_OK) return rc;
   *rerun = 0;
-  if (fs.overflow || (long long)fs.total > p->cap) {
+  if (fs.overflow & 4u) return PIKO_ERR_CAPACITY;
+  const bool pairs_short = (fs.overflow & 1u) || (p->bins_current && (long long)fs.total > p->cap);
+  const bool large_short = (fs.overflow & 2u) != 0;
+  if (pairs_short) {
     free_pairs(p);
     rc = alloc_pairs(p, (long long)fs.total + (long long)fs.total / 4 + 4096);
     if (rc != PIKO_OK) return rc;
-    const uint32_t zero = 0;
-    HIPOK(hipMemcpyAsync(&p->d_fs->overflow, &zero, 4, hipMemcpyHostToDevice, p->stream));
-    HIPOK(hipStreamSynchronize(p->stream));
-    *rerun = 1;
   }
+  if (large_short) {
+    dfree(p->d_large);
+    dfree(p->d_tile_off);
+    p->tri_bytes -= 8ll * p->large_cap;
+    p->device_bytes -= 8ll * p->large_cap;
+    p->large_cap = (uint32_t)std::min<long long>((long long)fs.large_peak + fs.large_peak / 4 + 4096, 1 << 30);
+    if ((rc = dalloc(p, &p->d_large, p->large_cap, &p->tri_bytes)) != PIKO_OK) return rc;
+    if ((rc = dalloc(p, &p->d_tile_off, p->large_cap, &p->tri_bytes)) != PIKO_OK) return rc;
+  }
+  if (fs.overflow || fs.large_peak) {
+    const uint32_t zero[2] = {0, 0};
+    HIPOK(hipMemcpyAsync(&p->d_fs->overflow, &zero[0], 4, hipMemcpyHostToDevice, p->stream));
+    HIPOK(hipMemcpyAsync(&p->d_fs->large_peak, &zero[1], 4, hipMemcpyHostToDevice, p->stream));
+    HIPOK(hipStreamSynchronize(p->stream));
+  }
+  *rerun = pairs_short || large_short;
   return PIKO_OK;
+}
+
+// Builds the bin queues of the current frame when the last frame did not
+// (a direct frame), growing them until they fit.
+int ensure_bins(piko_raster_pipe *p) {
+  if (p->bins_current) return PIKO_OK;
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    Marks mark{nullptr, p->stream};
+    enqueue_binning(p, mark);
+    HIPOK(hipGetLastError());
+    int rerun = 0;
+    int rc = check_capacity(p, &rerun);
+    if (rc != PIKO_OK) return rc;
+    if (!rerun) {
+      p->bins_current = true;
+      return PIKO_OK;
+    }
+  }
+  return PIKO_ERR_CAPACITY;
 }
 
 bool valid_bin(int T) { return T == 8 || T == 16 || T == 32 || T == 64; }
@@ -1000,6 +1402,7 @@ int validate_opts(const piko_constant_state *cs, int count, const piko_raster_op
   if (opt->subpixel_bits < 1 || opt->subpixel_bits > 8) return PIKO_ERR_INVALID;
   if (opt->input_kind != PIKO_INPUT_PRETRANSFORMED && opt->input_kind != PIKO_INPUT_WORLD)
     return PIKO_ERR_INVALID;
+  if (opt->raster_mode < PIKO_RASTER_AUTO || opt->raster_mode > PIKO_RASTER_DIRECT) return PIKO_ERR_INVALID;
   return PIKO_OK;
 }
 
@@ -1022,11 +1425,16 @@ FrameArgs make_args(const piko_constant_state *cs, int S, int T) {
 }
 
 int create_common(piko_raster_pipe *p, int kind, int S, long long tris, long long pix, int bins,
-                  long long pair_capacity) {
+                  long long pair_capacity, int mode) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PIKO_ERR_NO_DEVICE;
   p->kind = kind;
   p->S = S;
+  if (mode == PIKO_RASTER_AUTO) {
+    const char *m = std::getenv("PIKO_RASTER_MODE");
+    mode = (m && std::strcmp(m, "binned") == 0) ? PIKO_RASTER_BINNED : PIKO_RASTER_DIRECT;
+  }
+  p->mode = p->last_mode = mode;
   {
     int dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
@@ -1089,7 +1497,8 @@ int plain_pipe(piko_raster_pipe **out, const piko_constant_state *cs, const floa
   p->npix = (long long)p->W * p->H;
   p->passes = radix_passes(p->nbins);
   p->jobs.assign(1, p->fa);
-  rc = create_common(p, opt->input_kind, opt->subpixel_bits, count, p->npix, p->nbins, opt->pair_capacity);
+  rc = create_common(p, opt->input_kind, opt->subpixel_bits, count, p->npix, p->nbins, opt->pair_capacity,
+                     opt->raster_mode);
   p->classes.assign(1, RasterClass{p->T, 0, p->nbins, opt->waves_per_cu > 0 ? opt->waves_per_cu * p->cus : 0});
   if (rc == PIKO_OK) {
     if (on_device) {
@@ -1237,8 +1646,10 @@ int piko_raster_get_depth(piko_raster_pipe *p, float *depth) {
 
 long long piko_raster_get_bins(piko_raster_pipe *p, int64_t *bin_start, int32_t *bin_tris, long long cap) {
   if (!p || !bin_start) return PIKO_ERR_INVALID;
+  int rc = ensure_bins(p);
+  if (rc != PIKO_OK) return rc;
   FrameState fs;
-  int rc = read_frame_state(p, &fs);
+  rc = read_frame_state(p, &fs);
   if (rc != PIKO_OK) return rc;
   const long long P = std::min<long long>(fs.total, p->cap);
   uint32_t *keys = (uint32_t *)std::malloc((size_t)std::max<long long>(P, 1) * 4);
@@ -1263,8 +1674,10 @@ long long piko_raster_get_bins(piko_raster_pipe *p, int64_t *bin_start, int32_t 
 
 int piko_raster_get_stats(piko_raster_pipe *p, piko_raster_stats *st) {
   if (!p || !st) return PIKO_ERR_INVALID;
+  int rc = ensure_bins(p);
+  if (rc != PIKO_OK) return rc;
   FrameState fs;
-  int rc = read_frame_state(p, &fs);
+  rc = read_frame_state(p, &fs);
   if (rc != PIKO_OK) return rc;
   const int nkb = (int)std::max<long long>(cdiv(p->ntris, kVsTile), 1);
   uint32_t *kb = (uint32_t *)std::malloc((size_t)nkb * 4);
@@ -1296,6 +1709,7 @@ int piko_batch_create(piko_raster_pipe **out, const float *pos, const void *attr
   if (opt->subpixel_bits < 1 || opt->subpixel_bits > 8) return PIKO_ERR_INVALID;
   if (opt->input_kind != PIKO_INPUT_PRETRANSFORMED && opt->input_kind != PIKO_INPUT_WORLD) return PIKO_ERR_INVALID;
   if (opt->max_jobs <= 0 || opt->max_tris < 0 || opt->max_pixels <= 0) return PIKO_ERR_INVALID;
+  if (opt->raster_mode < PIKO_RASTER_AUTO || opt->raster_mode > PIKO_RASTER_DIRECT) return PIKO_ERR_INVALID;
   if (input_tris > 0 && (!pos || !attr)) return PIKO_ERR_INVALID;
   piko_raster_pipe *p = new (std::nothrow) piko_raster_pipe();
   if (!p) return PIKO_ERR_OOM;
@@ -1304,7 +1718,7 @@ int piko_batch_create(piko_raster_pipe **out, const float *pos, const void *attr
   p->cap_jobs = opt->max_jobs;
   const int bins = (int)std::min<long long>(opt->max_pixels / 64 + 1024, 1ll << 30);
   int rc = create_common(p, opt->input_kind, opt->subpixel_bits, opt->max_tris, opt->max_pixels, bins,
-                         opt->pair_capacity);
+                         opt->pair_capacity, opt->raster_mode);
   if (rc == PIKO_OK) rc = dalloc(p, &p->d_jobs, (long long)p->cap_jobs);
   if (rc == PIKO_OK && hipHostMalloc((void **)&p->h_jobs, (size_t)p->cap_jobs * sizeof(FrameArgs)) != hipSuccess)
     rc = PIKO_ERR_OOM;

@@ -1,6 +1,6 @@
 """GPU parity of the batched form: every job of a batch (one kernel sequence
 for all screens) equals the plain pipe on that screen alone and the oracle,
-bit for bit."""
+bit for bit, in both raster modes."""
 import sys
 
 import numpy as np
@@ -10,6 +10,8 @@ import piko_amd
 from piko_amd import scenes
 
 pytestmark = pytest.mark.gpu
+
+MODES = {"binned": piko_amd.RASTER_BINNED, "direct": piko_amd.RASTER_DIRECT}
 
 
 def log(msg):
@@ -24,7 +26,8 @@ def plain(scene, T, S=8):
         return pipe.pikoScreen.getData(), pipe.get_depth()
 
 
-def test_batch_of_uneven_scenes_matches_plain_and_oracle(oracle_mod):
+@pytest.mark.parametrize("mode", list(MODES))
+def test_batch_of_uneven_scenes_matches_plain_and_oracle(oracle_mod, mode):
     rng = np.random.default_rng(5)
     scs, Ts = [], []
     for k in range(14):
@@ -34,8 +37,9 @@ def test_batch_of_uneven_scenes_matches_plain_and_oracle(oracle_mod):
         scs.append(scenes.tris_uniform(2000 + k, n, W, H))
         Ts.append([8, 16, 32, 64][k % 4])
     pos, attr, parts = piko_amd.concat_scenes(scs)
-    log("batch of %d scenes, %d triangles" % (len(scs), pos.shape[0]))
-    with piko_amd.BatchPipe(pos, attr, "pretransformed", max_jobs=64, max_tris=1000, max_pixels=4096) as bp:
+    log("batch of %d scenes, %d triangles, mode %s" % (len(scs), pos.shape[0], mode))
+    with piko_amd.BatchPipe(pos, attr, "pretransformed", max_jobs=64, max_tris=1000, max_pixels=4096,
+                            raster_mode=MODES[mode]) as bp:
         bp.submit([(n, base, T, piko_amd.ConstantState.for_scene(s)) for (n, base, s), T in zip(parts, Ts)])
         bp.wait()
         for j, (s, T) in enumerate(zip(scs, Ts)):
@@ -50,12 +54,15 @@ def test_batch_of_uneven_scenes_matches_plain_and_oracle(oracle_mod):
         assert np.array_equal(bp.color(0), c0)
 
 
-def test_batch_world_jobs_share_resident_geometry(oracle_mod):
+@pytest.mark.parametrize("mode", list(MODES))
+def test_batch_world_jobs_share_resident_geometry(oracle_mod, mode):
     base = scenes.heightfield(seed=4, grid=120, W=320, H=180)
     cams = [((0.0, 0.3, 1.0), (0.0, -0.3, 0.0), 320, 180, 16),
             ((0.4, 0.5, 1.2), (0.0, 0.0, 0.0), 480, 270, 8),
             ((-0.6, 0.8, 0.9), (0.1, -0.1, 0.0), 200, 200, 32)]
-    with piko_amd.BatchPipe(base.pos, base.nrm, "world", max_jobs=8, max_tris=base.n, max_pixels=1 << 16) as bp:
+    log("world batch, mode %s" % mode)
+    with piko_amd.BatchPipe(base.pos, base.nrm, "world", max_jobs=8, max_tris=base.n, max_pixels=1 << 16,
+                            raster_mode=MODES[mode]) as bp:
         jobs, views = [], []
         for eye, tgt, W, H, T in cams:
             v = scenes.WorldScene(base.pos, base.nrm, W, H,
@@ -70,10 +77,13 @@ def test_batch_world_jobs_share_resident_geometry(oracle_mod):
             assert np.array_equal(bp.color(j), oc) and np.array_equal(bp.depth(j).view(np.int32), od.view(np.int32))
 
 
-def test_batch_resubmit_different_sizes():
+@pytest.mark.parametrize("mode", list(MODES))
+def test_batch_resubmit_different_sizes(mode):
     a = [scenes.tris_uniform(30 + k, 2000 * (k + 1), 100 + 50 * k, 80 + 30 * k) for k in range(3)]
     pos, attr, parts = piko_amd.concat_scenes(a)
-    with piko_amd.BatchPipe(pos, attr, "pretransformed", max_jobs=8, max_tris=10, max_pixels=100) as bp:
+    log("resubmit, mode %s" % mode)
+    with piko_amd.BatchPipe(pos, attr, "pretransformed", max_jobs=8, max_tris=10, max_pixels=100,
+                            raster_mode=MODES[mode]) as bp:
         for order in ([0, 1, 2], [2, 0], [1]):
             bp.submit([(parts[k][0], parts[k][1], 16, piko_amd.ConstantState.for_scene(a[k])) for k in order])
             for j, k in enumerate(order):

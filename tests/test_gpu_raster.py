@@ -1,5 +1,7 @@
 """GPU parity: the HIP raster pipeline against the oracle, bit for bit
-(colour words, depth bits and the per-bin queues), through the C-ABI."""
+(colour words, depth bits and the per-bin queues), through the C-ABI.  Every
+frame is checked in both raster modes: binned (queues sorted by bin, one wave
+per bin) and direct (per-pixel visibility words, no queues)."""
 import sys
 
 import numpy as np
@@ -15,28 +17,33 @@ def log(msg):
     sys.stdout.flush()
 
 
-def gpu_frame(scene, T, S, pair_capacity=0):
+MODES = ("binned", "direct")
+
+
+def gpu_frame(scene, T, S, pair_capacity=0, mode="direct"):
     import piko_amd
-    with piko_amd.RasterPipe().allocate(scene, T, S, pair_capacity) as pipe:
+    rm = piko_amd.RASTER_BINNED if mode == "binned" else piko_amd.RASTER_DIRECT
+    with piko_amd.RasterPipe().allocate(scene, T, S, pair_capacity, raster_mode=rm) as pipe:
         pipe.prepare()
         pipe.run_single()
         return pipe.pikoScreen.getData(), pipe.get_depth(), pipe.get_bins(), pipe.stats()
 
 
-def assert_frame_equal(scene, T, S, oracle_mod, check_bins=True, pair_capacity=0):
-    log("case W=%d H=%d n=%d T=%d S=%d kind=%s" % (scene.W, scene.H, scene.n, T, S, scene.kind))
-    c, d, (bs, bt), st = gpu_frame(scene, T, S, pair_capacity)
+def assert_frame_equal(scene, T, S, oracle_mod, check_bins=True, pair_capacity=0, modes=MODES):
     setup = oracle_mod.vertex_stage(scene, S)
     ostart, otris = oracle_mod.bins(setup, scene.W, scene.H, S, T)
     oc, od = oracle_mod.raster(setup, scene.W, scene.H, S, T, ostart, otris)
-    assert st["emitted"] == int(setup.keep.sum())
-    if check_bins:
-        assert np.array_equal(bs, ostart), "bin starts differ"
-        assert np.array_equal(bt, otris), "bin queues differ"
-    bad = np.argwhere(c != oc)
-    assert bad.size == 0, "colour differs at %d pixels, first %s: gpu %08x oracle %08x" % (
-        len(bad), bad[0], c[tuple(bad[0])], oc[tuple(bad[0])])
-    assert np.array_equal(d.view(np.int32), od.view(np.int32)), "depth bits differ"
+    for mode in modes:
+        log("case W=%d H=%d n=%d T=%d S=%d kind=%s mode=%s" % (scene.W, scene.H, scene.n, T, S, scene.kind, mode))
+        c, d, (bs, bt), st = gpu_frame(scene, T, S, pair_capacity, mode)
+        assert st["emitted"] == int(setup.keep.sum())
+        if check_bins:
+            assert np.array_equal(bs, ostart), "bin starts differ"
+            assert np.array_equal(bt, otris), "bin queues differ"
+        bad = np.argwhere(c != oc)
+        assert bad.size == 0, "%s: colour differs at %d pixels, first %s: gpu %08x oracle %08x" % (
+            mode, len(bad), bad[0], c[tuple(bad[0])], oc[tuple(bad[0])])
+        assert np.array_equal(d.view(np.int32), od.view(np.int32)), "%s: depth bits differ" % mode
     return c, d
 
 
@@ -90,14 +97,50 @@ def test_ties_and_full_screen(oracle_mod):
     assert np.all(c == 0xFF00FFFF)  # last of the tied nearest triangles
 
 
-def test_empty_and_all_culled(oracle_mod):
+@pytest.mark.parametrize("mode", MODES)
+def test_empty_and_all_culled(oracle_mod, mode):
     sc = scenes.TriScene(np.zeros((0, 9), np.float32), np.zeros((0, 3), np.uint32), 64, 48)
-    c, d, (bs, bt), st = gpu_frame(sc, 16, 8)
+    c, d, (bs, bt), st = gpu_frame(sc, 16, 8, mode=mode)
     assert np.all(c == 0xff663313) and np.all(d == 1.0) and st["pairs"] == 0 and len(bt) == 0
     cw = scenes.tris_uniform(2, 500, 64, 48)
     cw.pos = cw.pos[:, [0, 1, 2, 6, 7, 8, 3, 4, 5]].copy()  # reverse winding: all back-facing
-    c, d, (bs, bt), st = gpu_frame(cw, 16, 8)
+    c, d, (bs, bt), st = gpu_frame(cw, 16, 8, mode=mode)
     assert st["emitted"] == 0 and np.all(c == 0xff663313)
+
+
+def sized_tris(seed, n, W, H, lo, hi):
+    """n counter-clockwise triangles with bbox edges of lo..hi pixels, spread
+    over the screen and a margin around it."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform([-0.1 * W, -0.1 * H], [1.1 * W, 1.1 * H], (n, 2))
+    e = rng.uniform(lo, hi, (n, 1))
+    ang = rng.uniform(0, 2 * np.pi, (n, 1)) + np.array([0.0, 2.1, 4.2]) + rng.uniform(-0.4, 0.4, (n, 3))
+    pos = np.zeros((n, 9), np.float32)
+    pos[:, 0::3] = c[:, :1] + 0.6 * e * np.cos(ang)
+    pos[:, 1::3] = c[:, 1:] + 0.6 * e * np.sin(ang)
+    pos[:, 2::3] = rng.uniform(0.05, 0.95, (n, 3))
+    col = rng.integers(0, 2**32, (n, 3), dtype=np.uint64).astype(np.uint32)
+    return scenes.TriScene(pos, col, W, H)
+
+
+def test_direct_size_threshold(oracle_mod):
+    # bboxes from a few pixels to ~80x80: both sides of the 1024-pixel limit of
+    # the direct vertex stage, the rest drawn in 8x8 tiles
+    sc = sized_tris(11, 3000, 400, 300, 4, 80)
+    assert_frame_equal(sc, 16, 8, oracle_mod)
+
+
+def test_direct_large_list_grows(oracle_mod):
+    # more than 4096 large triangles: the large-triangle list overflows,
+    # grows and the frame is re-run
+    sc = sized_tris(12, 6000, 512, 384, 45, 60)
+    assert_frame_equal(sc, 32, 8, oracle_mod, modes=("direct",))
+    import piko_amd
+    with piko_amd.RasterPipe().allocate(sc, 32, 8, raster_mode=piko_amd.RASTER_DIRECT) as pipe:
+        pipe.run_frames(3)
+        a = pipe.pikoScreen.getData().copy()
+        pipe.run_single()
+        assert np.array_equal(a, pipe.pikoScreen.getData())
 
 
 def test_queue_capacity_grows(oracle_mod):
