@@ -82,9 +82,10 @@ struct FrameState {
   uint32_t total;        // (triangle, bin) pairs of this frame (binned path)
   uint32_t overflow;     // sticky: bit 0 pairs exceeded the queues, bit 1 the large-triangle list,
                          // bit 2 the large triangles' 8x8 tiles exceeded 2^32 (cannot grow)
-  uint32_t large;        // large triangles appended this frame (direct path)
-  uint32_t large_tiles;  // their 8x8 tiles
   uint32_t large_peak;   // most large triangles of one frame since the host last looked
+  uint32_t pad;
+  unsigned long long large_acc;  // direct path, this frame: large triangles listed (low
+                                 // half) and their 8x8 tiles (high half)
 };
 
 // One 1024-thread block scans n values in place (exclusive) and returns the
@@ -705,11 +706,14 @@ __global__ void __launch_bounds__(kRT) k_raster(const int4 *__restrict__ geo, co
 // ---- direct path --------------------------------------------------------
 //
 // For triangles whose clipped rounded bbox (rasterMacros.h:48-49, 58-63) has
-// at most kDirectMaxPx pixels -- nearly all of them in dense meshes -- the
-// vertex stage rasterises the triangle itself: the (triangle, pixel)
-// candidates of a wave's 64 triangles are spread evenly over its lanes, and
-// every covered pixel takes a 64-bit atomicMin of (z bits, 0xFFFFFFFE -
-// triangle index) in a per-pixel visibility buffer.  Larger triangles are
+// at most kDirectMaxPx pixels and whose edge functions fit the int32 path
+// (raster.pikostage:223-241) -- nearly all of them in dense meshes -- the
+// vertex stage rasterises the triangle itself.  The bbox columns are cut into
+// spans of kSpan pixels; the spans of a wave's 64 triangles are spread evenly
+// over its lanes (neighbouring lanes on neighbouring columns, so one atomic
+// instruction touches few cache lines), each lane steps the three edge
+// functions down its span, and every covered pixel takes a 64-bit atomicMin
+// of (z bits, 0xFFFFFFFE - triangle index) in a per-pixel visibility buffer.  Other triangles are
 // listed and rasterised in 8x8 tiles by every wave of k_large_raster.  A
 // per-pixel resolve shades the winner (the same arithmetic k_raster uses)
 // and resets the buffer.  With the submission index in the key the result
@@ -718,18 +722,36 @@ __global__ void __launch_bounds__(kRT) k_raster(const int4 *__restrict__ geo, co
 
 constexpr int kDirThreads = 256;     // one triangle per lane
 constexpr int kDirectMaxPx = 1024;   // larger bboxes take the tile path
+constexpr int kSpan = 4;             // pixels per work unit (one bbox column segment)
 constexpr int kLargeTile = 8;
 
+// One staged triangle (64 bytes, read with 16-byte LDS loads).  Edge function
+// k at the centre of pixel (x0 + i, y0 + j) is e[k] + i*dx[k] + j*dy[k], exact
+// in 32-bit arithmetic for triangles of the int32 path.
+struct alignas(16) DirTri {
+  uint32_t x0y0;   // x0 | y0 << 16
+  uint32_t wh;     // bbox width | height << 16
+  uint32_t tri;    // submission index
+  float rcp;       // 1 / barydenom (raster.pikostage:174)
+  int e[3];
+  float z2;
+  int dx[3];
+  float z0mz2;
+  int dy[3];
+  float z1mz2;
+};
+
+template <bool MULTI>
 struct DirStage {  // one wave's triangles
+  uint32_t pre[64];  // exclusive prefix of the triangles' span counts
+  DirTri t[64];
+};
+template <>
+struct DirStage<true> {
   uint32_t pre[64];
-  uint32_t x0y0[64];  // x0 | y0 << 16
-  uint32_t w[64];
-  int xy[6][64];
-  float zr[4][64];  // z2, z0mz2, z1mz2, 1/denom
-  uint32_t small[64];
-  uint32_t tri[64];
-  uint32_t fbw[64];             // framebuffer width of the triangle's job
-  unsigned long long fb[64];    // framebuffer base of the triangle's job
+  DirTri t[64];
+  unsigned long long fb[64];  // framebuffer base of the triangle's job
+  uint32_t fbw[64];           // framebuffer width of the triangle's job
 };
 
 __device__ __forceinline__ void wave_fence() {
@@ -758,18 +780,19 @@ __device__ __forceinline__ int job_by_pixel(const FrameArgs *__restrict__ jobs, 
 
 template <int KIND, bool MULTI>
 __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restrict__ pos, const void *__restrict__ attr,
-                                                            int n, int S, FrameArgs fa, const FrameArgs *__restrict__ jobs,
-                                                            int njobs, int4 *__restrict__ geo,
-                                                            uint4 *__restrict__ colr,
+                                                            int n, int S, FrameArgs fa,
+                                                            const FrameArgs *__restrict__ jobs, int njobs,
+                                                            int4 *__restrict__ geo, uint4 *__restrict__ colr,
                                                             unsigned long long *__restrict__ vis,
-                                                            uint32_t *__restrict__ large, FrameState *__restrict__ fs,
+                                                            uint32_t *__restrict__ large,
+                                                            uint32_t *__restrict__ tile_off, FrameState *__restrict__ fs,
                                                             uint32_t large_cap, uint32_t *__restrict__ blk_keep) {
-  __shared__ DirStage st[kDirThreads / 64];
+  __shared__ DirStage<MULTI> st[kDirThreads / 64];
   __shared__ uint32_t s_keep[kDirThreads / 64];
   const int t = threadIdx.x, lane = t & 63;
-  DirStage &s = st[t >> 6];
+  DirStage<MULTI> &s = st[t >> 6];
   const int i = blockIdx.x * kDirThreads + t;
-  uint32_t npx = 0;
+  uint32_t units = 0, tiles = 0;
   bool big = false, kept = false;
   if (i < n) {
     const FrameArgs &ja = MULTI ? jobs[job_search(jobs, njobs, (uint32_t)i, false)] : fa;
@@ -808,43 +831,73 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
       int x0, y0, x1, y1;
       clip_rect(p, ja.S, ja.W, ja.H, x0, y0, x1, y1);
       const int w = max(0, x1 - x0), h = max(0, y1 - y0);
-      npx = (uint32_t)w * (uint32_t)h;
-      if (npx > (uint32_t)kDirectMaxPx) {
-        big = true;
-        npx = 0;
-      } else {
-        s.x0y0[lane] = (uint32_t)x0 | ((uint32_t)y0 << 16);
-        s.w[lane] = (uint32_t)w;
+      const uint32_t npx = (uint32_t)w * (uint32_t)h;
+      if (npx > (uint32_t)kDirectMaxPx || !small_bbox(p, ja.S)) {
+        big = npx > 0;
+        tiles = (uint32_t)((w + kLargeTile - 1) / kLargeTile) * (uint32_t)((h + kLargeTile - 1) / kLargeTile);
+      } else if (npx > 0) {
+        units = (uint32_t)w * ((uint32_t)(h + kSpan - 1) / kSpan);
+        DirTri &d = s.t[lane];
+        d.x0y0 = (uint32_t)x0 | ((uint32_t)y0 << 16);
+        d.wh = (uint32_t)w | ((uint32_t)h << 16);
+        d.tri = (uint32_t)i;
+        d.rcp = rcp;
+        // edge functions at the centre of pixel (x0, y0) and their steps;
+        // unsigned arithmetic, equal to the int32 evaluation for int32-path
+        // triangles (no intermediate leaves the int32 range)
+        const uint32_t half = 1u << (S - 1);
+        const uint32_t cx = ((uint32_t)x0 << S) + half, cy = ((uint32_t)y0 << S) + half;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) s.xy[k][lane] = p[k];
-        s.zr[0][lane] = z[2];
-        s.zr[1][lane] = z0mz2;
-        s.zr[2][lane] = z1mz2;
-        s.zr[3][lane] = rcp;
-        s.small[lane] = small_bbox(p, ja.S);
-        s.tri[lane] = (uint32_t)i;
-        s.fbw[lane] = (uint32_t)ja.W;
-        s.fb[lane] = MULTI ? ja.fb_base : 0ull;
+        for (int k = 0; k < 3; ++k) {
+          const uint32_t xa = (uint32_t)p[2 * k], ya = (uint32_t)p[2 * k + 1];
+          const uint32_t xb = (uint32_t)p[(2 * k + 2) % 6], yb = (uint32_t)p[(2 * k + 3) % 6];
+          d.e[k] = (int)((cy - ya) * (xb - xa) - (cx - xa) * (yb - ya));
+          d.dx[k] = (int)((0u - (yb - ya)) << S);
+          d.dy[k] = (int)((xb - xa) << S);
+        }
+        d.z2 = z[2];
+        d.z0mz2 = z0mz2;
+        d.z1mz2 = z1mz2;
+        if constexpr (MULTI) {
+          s.fb[lane] = ja.fb_base;
+          s.fbw[lane] = (uint32_t)ja.W;
+        }
       }
     }
   }
   const unsigned long long km = __ballot(kept);
   if (lane == 0) s_keep[t >> 6] = (uint32_t)__popcll(km);
-  // large triangles: one append per wave
+  // large triangles: one append per wave.  A single 64-bit atomic reserves
+  // list slots (low half) and 8x8 tile numbers (high half) together, so the
+  // first tile of the listed triangles increases with the slot.
   const unsigned long long bm = __ballot(big);
   if (bm) {
     const int first = __ffsll((long long)bm) - 1;
-    uint32_t base = 0;
-    if (lane == first) base = atomicAdd(&fs->large, (uint32_t)__popcll(bm));
-    base = __shfl(base, first, 64);
+    const uint32_t tl = big ? tiles : 0u;
+    const uint32_t tinc = wave_incl_scan(tl);
+    const uint32_t tsum = __shfl(tinc, 63, 64);
+    uint32_t bt = 0, btile = 0;
+    if (lane == first) {
+      const unsigned long long old =
+          atomicAdd(&fs->large_acc, ((unsigned long long)tsum << 32) | (unsigned long long)__popcll(bm));
+      bt = (uint32_t)old;
+      btile = (uint32_t)(old >> 32);
+      if ((unsigned long long)btile + tsum > 0xFFFFFFFFull) atomicOr(&fs->overflow, 4u);
+    }
+    bt = __shfl(bt, first, 64);
+    btile = __shfl(btile, first, 64);
     if (big) {
-      const uint32_t at = base + (uint32_t)__popcll(bm & ((1ull << lane) - 1ull));
-      if (at < large_cap) large[at] = (uint32_t)i;
-      else atomicOr(&fs->overflow, 2u);
+      const uint32_t at = bt + (uint32_t)__popcll(bm & ((1ull << lane) - 1ull));
+      if (at < large_cap) {
+        large[at] = (uint32_t)i;
+        tile_off[at] = btile + tinc - tl;
+      } else {
+        atomicOr(&fs->overflow, 2u);
+      }
     }
   }
-  const uint32_t inc = wave_incl_scan(npx);
-  s.pre[lane] = inc - npx;
+  const uint32_t inc = wave_incl_scan(units);
+  s.pre[lane] = inc - units;
   const uint32_t tot = __shfl(inc, 63, 64);
   __syncthreads();
   if (t == 0) {
@@ -858,50 +911,46 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
 #pragma unroll
     for (int step = 32; step > 0; step >>= 1)
       if (s.pre[o + step] <= cidx) o += step;
+    const DirTri &d = s.t[o];
+    const uint4 r0 = *(const uint4 *)&d.x0y0;
+    const int4 re = *(const int4 *)&d.e[0];
+    const int4 rx = *(const int4 *)&d.dx[0];
+    const int4 ry = *(const int4 *)&d.dy[0];
     const uint32_t k = cidx - s.pre[o];
-    const int w = (int)s.w[o];
-    const int q = (int)(((float)k + 0.5f) * __builtin_amdgcn_rcpf((float)w));
-    const int r = (int)k - q * w;
-    const uint32_t org = s.x0y0[o];
-    const int px = (int)(org & 0xffffu) + r, py = (int)(org >> 16) + q;
-    const int xy[6] = {s.xy[0][o], s.xy[1][o], s.xy[2][o], s.xy[3][o], s.xy[4][o], s.xy[5][o]};
-    float alpha, beta;
-    if (covered_at(xy, s.small[o] != 0, px, py, S, s.zr[3][o], alpha, beta)) {
-      const float z = interp(s.zr[1][o], s.zr[2][o], s.zr[0][o], alpha, beta);  // raster.pikostage:272
-      atomicMin((long long *)&vis[s.fb[o] + (size_t)py * s.fbw[o] + px], (long long)vis_key(z, s.tri[o]));
+    const uint32_t w = r0.y & 0xffffu, h = r0.y >> 16;
+    // band and column of unit k: (k + 0.5) / w is at least 0.5 / w from an
+    // integer and w, k <= 1024, so the float quotient truncates exactly
+    const uint32_t q = (uint32_t)(((float)k + 0.5f) * __builtin_amdgcn_rcpf((float)w));
+    const uint32_t sx = k - q * w, sy = q * kSpan;
+    const uint32_t px = (r0.x & 0xffffu) + sx, py = (r0.x >> 16) + sy;
+    const uint32_t nv = min((uint32_t)kSpan, h - sy);
+    uint32_t e0 = (uint32_t)re.x + sx * (uint32_t)rx.x + sy * (uint32_t)ry.x;
+    uint32_t e1 = (uint32_t)re.y + sx * (uint32_t)rx.y + sy * (uint32_t)ry.y;
+    uint32_t e2 = (uint32_t)re.z + sx * (uint32_t)rx.z + sy * (uint32_t)ry.z;
+    const float rcp = __uint_as_float(r0.w);
+    const float z2 = __int_as_float(re.w), z0mz2 = __int_as_float(rx.w), z1mz2 = __int_as_float(ry.w);
+    const uint32_t low = 0xFFFFFFFEu - r0.z;
+    unsigned long long *col;
+    uint32_t pitch;
+    if constexpr (MULTI) {
+      pitch = s.fbw[o];
+      col = vis + s.fb[o] + (size_t)py * pitch + px;
+    } else {
+      pitch = (uint32_t)fa.W;
+      col = vis + (size_t)py * pitch + px;
     }
-  }
-}
-
-// 8x8 tiles of every listed large triangle: exclusive offsets and the total.
-__global__ void __launch_bounds__(1024) k_large_scan(const int4 *__restrict__ geo, const uint32_t *__restrict__ large,
-                                                     uint32_t *__restrict__ tile_off, FrameState *__restrict__ fs,
-                                                     uint32_t large_cap, FrameArgs fa, const FrameArgs *__restrict__ jobs,
-                                                     int njobs, int multi) {
-  __shared__ uint32_t sh[1024 / 64 + 1];
-  __shared__ unsigned long long s_all;
-  if (threadIdx.x == 0) s_all = 0;
-  __syncthreads();
-  const int nl = (int)min(fs->large, large_cap);
-  unsigned long long mine = 0;
-  for (int k = threadIdx.x; k < nl; k += 1024) {
-    const uint32_t tri = large[k];
-    const FrameArgs &ja = multi ? jobs[job_search(jobs, njobs, tri, false)] : fa;
-    const int4 a = geo[3 * (size_t)tri], b = geo[3 * (size_t)tri + 1];
-    const int p[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
-    int x0, y0, x1, y1;
-    clip_rect(p, ja.S, ja.W, ja.H, x0, y0, x1, y1);
-    const uint32_t nt = (uint32_t)((x1 - x0 + kLargeTile - 1) / kLargeTile) * (uint32_t)((y1 - y0 + kLargeTile - 1) / kLargeTile);
-    tile_off[k] = nt;
-    mine += nt;
-  }
-  atomicAdd(&s_all, mine);
-  __syncthreads();
-  const bool fits = s_all <= 0xFFFFFFFFull;  // tile numbers are 32-bit
-  const uint32_t tot = block_scan_inplace(tile_off, nl, [](int i) { return (size_t)i; }, sh);
-  if (threadIdx.x == 0) {
-    fs->large_tiles = fits ? tot : 0u;
-    if (!fits) fs->overflow |= 4u;
+#pragma unroll
+    for (uint32_t j = 0; j < (uint32_t)kSpan; ++j) {
+      if (j < nv && (int)(e0 | e1 | e2) >= 0) {
+        const float alpha = (float)(int)e1 * rcp, beta = (float)(int)e2 * rcp;  // raster.pikostage:269-270
+        const float z = interp(z0mz2, z1mz2, z2, alpha, beta);                    // raster.pikostage:272
+        atomicMin((long long *)(col + (size_t)j * pitch),
+                  (long long)(((unsigned long long)__float_as_uint(z) << 32) | (unsigned long long)low));
+      }
+      e0 += (uint32_t)ry.x;
+      e1 += (uint32_t)ry.y;
+      e2 += (uint32_t)ry.z;
+    }
   }
 }
 
@@ -913,8 +962,11 @@ __global__ void __launch_bounds__(256) k_large_raster(const int4 *__restrict__ g
                                                       const FrameState *__restrict__ fs, uint32_t large_cap,
                                                       FrameArgs fa, const FrameArgs *__restrict__ jobs, int njobs,
                                                       unsigned long long *__restrict__ vis) {
-  const int nl = (int)min(fs->large, large_cap);
-  const uint32_t total = fs->large_tiles;
+  const unsigned long long acc = fs->large_acc;
+  const uint32_t nl32 = (uint32_t)acc;
+  if (nl32 > large_cap || (fs->overflow & 4u)) return;  // the list overflowed: the frame is re-run
+  const int nl = (int)nl32;
+  const uint32_t total = (uint32_t)(acc >> 32);
   const int lane = threadIdx.x & 63;
   const uint32_t nw = gridDim.x * (blockDim.x / 64);
   for (uint32_t g = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); g < total; g += nw) {
@@ -958,8 +1010,8 @@ __global__ void __launch_bounds__(256) k_resolve(const int4 *__restrict__ geo, c
   s_u8[t] = kU8Table.v[t];
   __syncthreads();
   if (blockIdx.x == 0 && t == 0) {  // the large list of this frame is consumed
-    fs->large_peak = max(fs->large_peak, fs->large);
-    fs->large = 0;
+    fs->large_peak = max(fs->large_peak, (uint32_t)fs->large_acc);
+    fs->large_acc = 0;
   }
   const long long g = (long long)blockIdx.x * blockDim.x + t;
   if (g >= npix) return;
@@ -1183,8 +1235,8 @@ int radix_passes(int nbins) {
 // Kernel names in launch order (the sort section repeats once per pass, the
 // raster once per bin-size class).
 const char *kernel_name(const piko_raster_pipe *p, int k) {
-  static const char *direct[] = {"k_vs_direct", "k_large_scan", "k_large_raster", "k_resolve"};
-  if (p->last_mode == PIKO_RASTER_DIRECT) return k < 4 ? direct[k] : "";
+  static const char *direct[] = {"k_vs_direct", "k_large_raster", "k_resolve"};
+  if (p->last_mode == PIKO_RASTER_DIRECT) return k < 3 ? direct[k] : "";
   static const char *head[] = {"k_vs_count", "k_scan_tiles", "k_emit_pairs"};
   static const char *pass[] = {"k_sort_count", "k_sort_rows", "k_sort_scatter"};
   if (k < 3) return head[k];
@@ -1271,8 +1323,8 @@ void enqueue_binning(piko_raster_pipe *p, Marks &mark) {
   mark();
 }
 
-// Sort-free frame: vertex stage + small-triangle raster, large-triangle tile
-// offsets, large-triangle tiles, resolve.
+// Sort-free frame: vertex stage + small-triangle raster, large-triangle tiles,
+// resolve.
 void enqueue_direct(piko_raster_pipe *p, Marks &mark) {
   const int n = p->ntris;
   const int nj = (int)p->jobs.size();
@@ -1280,7 +1332,8 @@ void enqueue_direct(piko_raster_pipe *p, Marks &mark) {
 #define PIKO_VSD_LAUNCH(KIND, MULTI)                                                                              \
   k_vs_direct<KIND, MULTI><<<blocks, kDirThreads, 0, p->stream>>>(p->d_pos, p->d_attr, n, p->S, p->fa, p->d_jobs, \
                                                                   nj, p->d_geo, p->d_col, p->d_vis, p->d_large,   \
-                                                                  p->d_fs, p->large_cap, p->d_keep_dir)
+                                                                  p->d_tile_off, p->d_fs, p->large_cap,           \
+                                                                  p->d_keep_dir)
   if (p->kind == PIKO_INPUT_PRETRANSFORMED) {
     if (p->multi) PIKO_VSD_LAUNCH(PIKO_INPUT_PRETRANSFORMED, true);
     else PIKO_VSD_LAUNCH(PIKO_INPUT_PRETRANSFORMED, false);
@@ -1290,10 +1343,7 @@ void enqueue_direct(piko_raster_pipe *p, Marks &mark) {
   }
 #undef PIKO_VSD_LAUNCH
   mark();
-  k_large_scan<<<1, 1024, 0, p->stream>>>(p->d_geo, p->d_large, p->d_tile_off, p->d_fs, p->large_cap, p->fa,
-                                          p->d_jobs, nj, p->multi ? 1 : 0);
-  mark();
-  const int lgrid = std::max(p->cus, 1) * 8;
+  const int lgrid = std::max(p->cus, 1) * 2;
   if (p->multi)
     k_large_raster<true><<<lgrid, 256, 0, p->stream>>>(p->d_geo, p->d_large, p->d_tile_off, p->d_fs, p->large_cap,
                                                        p->fa, p->d_jobs, nj, p->d_vis);
