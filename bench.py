@@ -221,7 +221,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--bins", default="8,16,32,64", help="bin sizes of the binned raster mode")
-    ap.add_argument("--modes", default="binned,direct", help="raster modes to sweep")
+    ap.add_argument("--modes", default="tiled,direct,binned", help="raster modes to sweep")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="all", choices=["raster", "batch", "serve", "reyes", "all"])
     ap.add_argument("--serve-tris", type=int, default=300_000, help="triangles per resident scene when serving")
@@ -248,12 +248,15 @@ def main():
     best = None
     runs = []
     modes = args.modes.split(",")
+    if "tiled" in modes:  # tile edges of 16 and up (8 is rasterised in 16-pixel tiles)
+        runs += [("tiled", int(t)) for t in args.bins.split(",") if int(t) >= 16]
     if "binned" in modes:
         runs += [("binned", int(t)) for t in args.bins.split(",")]
     if "direct" in modes:
         runs.append(("direct", 16))  # the bin size only shapes the on-demand queues
+    rmodes = {"binned": piko_amd.RASTER_BINNED, "direct": piko_amd.RASTER_DIRECT, "tiled": piko_amd.RASTER_TILED}
     for mode, T in runs:
-        rm = piko_amd.RASTER_BINNED if mode == "binned" else piko_amd.RASTER_DIRECT
+        rm = rmodes[mode]
         pipe = piko_amd.RasterPipe().allocate(scene, T, S, raster_mode=rm)
         pipe.prepare()
         pipe.run_frames(max(args.warmup, 1))
@@ -262,7 +265,7 @@ def main():
         dt = (time.perf_counter() - t0) / args.steps
         kernels, frame_ms = pipe.profile_frames(max(5, min(args.steps, 20)))
         st = pipe.stats()
-        key = "binned-%d" % T if mode == "binned" else "direct"
+        key = "direct" if mode == "direct" else "%s-%d" % (mode, T)
         sweep[key] = {"ms_per_frame": round(dt * 1e3, 4), "event_frame_ms": round(frame_ms, 4),
                       "pairs": st["pairs"], "kernels_ms": [[n, round(m, 4)] for n, m in kernels]}
         log("%s: %.3f ms/frame (events %.3f ms), kernels %s" % (key, dt * 1e3, frame_ms, sweep[key]["kernels_ms"]))
@@ -274,13 +277,23 @@ def main():
     # dominant kernel: the longest single launch of the frame (HIP events on
     # the pipe's stream); the raster kernel at every bin size measured so far
     dom, dom_ms = max(kernels, key=lambda k: k[1])
-    dom_key = "%s<%d>" % (dom, T) if dom == "k_raster" else dom
+    dom_key = "%s<%d>" % (dom, T) if dom in ("k_raster", "k_tile_raster") else dom
     if dom == "k_raster":
         # framebuffer written once (colour + depth), each emitted triangle's
         # setup (3 fixed-point xy pairs, 3 depths, 3 packed colours) read once,
         # each (triangle, bin) queue entry read once
         units = {"pixels": scene.W * scene.H, "emitted_triangles": st["emitted"], "queue_entries": st["pairs"]}
         alg_k = 8 * units["pixels"] + 48 * units["emitted_triangles"] + 4 * units["queue_entries"]
+    elif dom in ("k_tile_bin", "k_vs_direct"):
+        # the input's vertex positions (3 x xyz f32) read once per triangle
+        units = {"input_triangles": scene.n}
+        alg_k = 36 * scene.n
+    elif dom in ("k_tile_raster", "k_resolve"):
+        # the input's vertex attributes (normals 36 B, or packed colours 12 B)
+        # read once per triangle, the framebuffer (colour + depth) written once
+        b_attr = 36 if scene.kind == "world" else 12
+        units = {"input_triangles": scene.n, "pixels": scene.W * scene.H}
+        alg_k = b_attr * scene.n + 8 * scene.W * scene.H
     else:
         units = {"frame": 1}
         alg_k = alg_bytes
@@ -304,7 +317,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "heightfield-2M-1080p raster (VS->bin->raster->depth<=->Gouraud)",
                    "triangles": scene.n, "W": scene.W, "H": scene.H, "subpixel_bits": S, "raster_mode": mode,
-                   "bin_size": T if mode == "binned" else None,
+                   "bin_size": None if mode == "direct" else T,
                    "emitted": st["emitted"], "pairs": st["pairs"],
                    "sweep": sweep},
         "roofline": {"bound": "hbm", "achieved": round(achieved_k, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

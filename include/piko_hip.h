@@ -64,16 +64,22 @@ typedef struct {
 
 enum { PIKO_INPUT_PRETRANSFORMED = 0, PIKO_INPUT_WORLD = 1 };
 
-/* How a frame is rasterised.  Both give the reference's image bit for bit.
+/* How a frame is rasterised.  All give the reference's image bit for bit.
  *   BINNED: vertex stage -> (triangle, bin) queues sorted by bin -> one raster
  *           wave per bin (the reference's schedule, rasterPipe.h:16-45).
  *   DIRECT: the vertex stage rasterises small triangles itself into a per-pixel
  *           (depth, submission index) visibility word; large ones are drawn
- *           in 8x8 tiles; a resolve shades the winner.  No queues; the bin
- *           queues are built on demand when asked for (piko_raster_get_bins).
- *   AUTO:   DIRECT.  The environment variable PIKO_RASTER_MODE=binned|direct
- *           overrides AUTO. */
-enum { PIKO_RASTER_AUTO = 0, PIKO_RASTER_BINNED = 1, PIKO_RASTER_DIRECT = 2 };
+ *           in 8x8 tiles; a resolve shades the winner.
+ *   TILED:  bins are screen tiles (the bin size, at least 16, larger where a
+ *           screen would have more than 8192 tiles).  The vertex stage
+ *           counts, scans and scatters each chunk of 8192 triangles into
+ *           per-tile lists in LDS; one workgroup per tile then resolves every
+ *           fragment in LDS and writes each pixel once.  No global atomics.
+ *   DIRECT and TILED keep no sorted queues: the bin queues are built on
+ *   demand when asked for (piko_raster_get_bins).
+ *   AUTO:   TILED.  The environment variable PIKO_RASTER_MODE=binned|direct|
+ *           tiled overrides AUTO. */
+enum { PIKO_RASTER_AUTO = 0, PIKO_RASTER_BINNED = 1, PIKO_RASTER_DIRECT = 2, PIKO_RASTER_TILED = 3 };
 
 typedef struct {
   int bin_size;           /* raster bin edge T in pixels: 8, 16, 32 or 64 */

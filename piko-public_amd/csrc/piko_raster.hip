@@ -703,6 +703,38 @@ __global__ void __launch_bounds__(kRT) k_raster(const int4 *__restrict__ geo, co
   }
 }
 
+// Vertex positions of input triangle `src` -> fixed-point xy and z.
+template <int KIND>
+__device__ __forceinline__ void tri_positions(const float *__restrict__ pos, size_t src, const FrameArgs &ja,
+                                              int p[6], float z[3]) {
+  const float *v = pos + 9 * src;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (KIND == PIKO_INPUT_PRETRANSFORMED) {
+      p[2 * q] = fixpt(v[3 * q], ja.S);
+      p[2 * q + 1] = fixpt(v[3 * q + 1], ja.S);
+      z[q] = v[3 * q + 2];
+    } else {
+      transform_vertex(v + 3 * q, ja, p[2 * q], p[2 * q + 1], z[q]);
+    }
+  }
+}
+
+// Packed vertex colours of input triangle `src` (lit normals for WORLD input).
+template <int KIND>
+__device__ __forceinline__ void tri_colours(const void *__restrict__ attr, size_t src, const FrameArgs &ja,
+                                            uint32_t c[3]) {
+  if (KIND == PIKO_INPUT_PRETRANSFORMED) {
+    const uint32_t *cc = (const uint32_t *)attr + 3 * src;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) c[q] = cc[q];
+  } else {
+    const float *nn = (const float *)attr + 9 * src;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) c[q] = light_vertex(nn[3 * q], nn[3 * q + 1], nn[3 * q + 2], ja);
+  }
+}
+
 // ---- direct path --------------------------------------------------------
 //
 // For triangles whose clipped rounded bbox (rasterMacros.h:48-49, 58-63) has
@@ -795,29 +827,12 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
   uint32_t units = 0, tiles = 0;
   bool big = false, kept = false;
   if (i < n) {
+    // vertex positions only: the winner's colours are computed in k_resolve
     const FrameArgs &ja = MULTI ? jobs[job_search(jobs, njobs, (uint32_t)i, false)] : fa;
     const size_t src = MULTI ? (size_t)ja.src_base + (size_t)(i - (int)ja.vbase) : (size_t)i;
     int p[6];
     float z[3];
-    uint32_t c[3];
-    const float *v = pos + 9 * src;
-    if (KIND == PIKO_INPUT_PRETRANSFORMED) {
-      const uint32_t *cc = (const uint32_t *)attr + 3 * src;
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        p[2 * q] = fixpt(v[3 * q], ja.S);
-        p[2 * q + 1] = fixpt(v[3 * q + 1], ja.S);
-        z[q] = v[3 * q + 2];
-        c[q] = cc[q];
-      }
-    } else {
-      const float *nn = (const float *)attr + 9 * src;
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        transform_vertex(v + 3 * q, ja, p[2 * q], p[2 * q + 1], z[q]);
-        c[q] = light_vertex(nn[3 * q], nn[3 * q + 1], nn[3 * q + 2], ja);
-      }
-    }
+    tri_positions<KIND>(pos, src, ja, p, z);
     if (vs_keep(p, ja.W, ja.H, ja.S)) {
       kept = true;
       Edges e;
@@ -827,7 +842,6 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
       geo[3 * (size_t)i + 0] = make_int4(p[0], p[1], p[2], p[3]);
       geo[3 * (size_t)i + 1] = make_int4(p[4], p[5], __float_as_int(z[2]), __float_as_int(z0mz2));
       geo[3 * (size_t)i + 2] = make_int4(__float_as_int(z1mz2), __float_as_int(rcp), 0, 0);
-      colr[i] = make_uint4(c[0], c[1], c[2], 0u);
       int x0, y0, x1, y1;
       clip_rect(p, ja.S, ja.W, ja.H, x0, y0, x1, y1);
       const int w = max(0, x1 - x0), h = max(0, y1 - y0);
@@ -999,8 +1013,8 @@ __global__ void __launch_bounds__(256) k_large_raster(const int4 *__restrict__ g
 // Per pixel: depth = the winner's z, colour = Gouraud of the winner (the clear
 // colour where nothing is drawn); the visibility word is reset for the next
 // frame.
-template <bool MULTI>
-__global__ void __launch_bounds__(256) k_resolve(const int4 *__restrict__ geo, const uint4 *__restrict__ colr,
+template <int KIND, bool MULTI>
+__global__ void __launch_bounds__(256) k_resolve(const int4 *__restrict__ geo, const void *__restrict__ attr,
                                                  unsigned long long *__restrict__ vis, FrameArgs fa,
                                                  const FrameArgs *__restrict__ jobs, int njobs, long long npix,
                                                  uint32_t *__restrict__ color, float *__restrict__ depth,
@@ -1024,14 +1038,458 @@ __global__ void __launch_bounds__(256) k_resolve(const int4 *__restrict__ geo, c
     const int py = (int)(local / ja.W), px = (int)(local - (long long)py * ja.W);
     const uint32_t tri = 0xFFFFFFFEu - (uint32_t)(key & 0xFFFFFFFFull);
     const int4 a = geo[3 * (size_t)tri], b = geo[3 * (size_t)tri + 1], c = geo[3 * (size_t)tri + 2];
-    const uint4 cc = colr[tri];
+    const size_t src = MULTI ? (size_t)ja.src_base + (size_t)(tri - ja.vbase) : (size_t)tri;
+    uint32_t cc[3];
+    tri_colours<KIND>(attr, src, ja, cc);
     const int xy[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
     float alpha = 0.0f, beta = 0.0f;
     covered_at(xy, small_bbox(xy, ja.S), px, py, ja.S, __int_as_float(c.y), alpha, beta);
-    col = gouraud(s_u8, cc.x, cc.y, cc.z, alpha, beta);
+    col = gouraud(s_u8, cc[0], cc[1], cc[2], alpha, beta);
   }
   depth[g] = __int_as_float((int)(key >> 32));
   color[g] = col;
+}
+
+// ---- tiled path ---------------------------------------------------------
+//
+// Bins are screen tiles of T x T pixels; every fragment is resolved in the
+// tile's LDS, and no global atomic is used.
+//
+//   k_tile_bin     one block per chunk of up to kChunk consecutive triangles of
+//                  one job.  Vertex stage on the positions only (vertexShader.
+//                  pikostage:36-109), then RasterStage::assignBin's trivial-
+//                  reject test per tile (raster.pikostage:40-117) for triangles
+//                  whose clipped bbox spans at most 2x2 tiles: each accepted
+//                  (tile, triangle) pair takes a slot from an LDS counter of
+//                  its tile.  The counters are scanned, the scan is written as
+//                  the chunk's column of the offset table, and the triangle
+//                  indices are scattered into the chunk's own pair region,
+//                  grouped by tile.  Larger triangles (or ones whose edge
+//                  functions need 64 bits) are listed at the region's end.
+//   k_tile_raster  one block per tile.  It reads where its pairs lie in each
+//                  chunk of its job from the table, recomputes each listed
+//                  triangle's setup from its positions, rasterises it into
+//                  the tile's LDS visibility words (z bits, 0xFFFFFFFE -
+//                  triangle), does the same for the chunks' large triangles
+//                  clipped to the tile, then shades each pixel's winner from
+//                  the winner's vertices and writes colour and depth once.
+//
+// A chunk region holds 4 slots per triangle and a listed triangle takes at
+// most 4, so no queue can overflow.  With the submission index in the key the
+// result is the sequential one whatever the order of pairs within a tile.
+
+constexpr int kChunk = 4096;                      // triangles per chunk
+constexpr int kBinThreads = 1024;
+constexpr int kChunkPer = kChunk / kBinThreads;   // triangles per thread
+constexpr uint32_t kChunkCap = 4u * kChunk;       // pair slots per chunk region
+constexpr int kMaxTilesJob = 8192;                // tiles of one job (LDS counters)
+constexpr int kTRThreads = 256;                   // k_tile_raster block
+constexpr int kTRWinPer = 2;                      // chunks each k_tile_raster thread reads per window
+constexpr int kTRWin = kTRThreads * kTRWinPer;    // chunks read per window
+constexpr int kExactPx = 16;                      // k_tile_bin: exact tile mask up to this many bbox pixels
+constexpr int kLanePx = 16;                       // k_tile_raster: one lane rasterises rects up to this size
+
+struct ChunkDesc {
+  uint32_t job;      // index in the job table
+  uint32_t tri0;     // first triangle (batch numbering)
+  uint32_t count;    // triangles in the chunk
+  uint32_t tab_off;  // first entry of the chunk's column of the offset table
+};
+
+template <int KIND, bool MULTI>
+__global__ void __launch_bounds__(kBinThreads) k_tile_bin(const float *__restrict__ pos, FrameArgs fa,
+                                                           const FrameArgs *__restrict__ jobs,
+                                                           const ChunkDesc *__restrict__ chunks,
+                                                           uint32_t *__restrict__ tab, uint32_t *__restrict__ pairs,
+                                                           uint32_t *__restrict__ chunk_keep,
+                                                           int4 *__restrict__ geo) {
+  __shared__ uint32_t cnt[kMaxTilesJob + 2];
+  __shared__ uint32_t sh[kBinThreads / 64 + 1];
+  const int t = threadIdx.x;
+  const int c = blockIdx.x;
+  const ChunkDesc cd = chunks[c];
+  const FrameArgs &ja = MULTI ? jobs[cd.job] : fa;
+  const int ntiles = ja.nBx * ja.nBy;
+  for (int q = t; q < ntiles + 2; q += kBinThreads) cnt[q] = 0;
+  __syncthreads();
+  const int S = ja.S, lgT = ja.lgT, bsb = S + lgT;
+  // all of the thread's vertex positions first, so their loads overlap
+  float vp[kChunkPer][9];
+#pragma unroll
+  for (int k = 0; k < kChunkPer; ++k) {
+    const uint32_t il = (uint32_t)(k * kBinThreads + t);
+    const uint32_t v = cd.tri0 + min(il, cd.count - 1u);
+    const size_t src = MULTI ? (size_t)ja.src_base + (size_t)(v - ja.vbase) : (size_t)v;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) vp[k][q] = il < cd.count ? pos[9 * src + q] : 0.0f;
+  }
+  // per triangle: ra = kind (bit 0 listed per tile, bit 1 large) | tiles it
+  // is listed in << 2 | tx0 << 8 | ty0 << 20; rb/rc = slots (16 bits each)
+  uint32_t ra[kChunkPer], rb[kChunkPer], rc[kChunkPer];
+  uint32_t keepc = 0;
+#pragma unroll
+  for (int k = 0; k < kChunkPer; ++k) {
+    ra[k] = 0;
+    rb[k] = 0;
+    rc[k] = 0;
+    const uint32_t il = (uint32_t)(k * kBinThreads + t);
+    if (il >= cd.count) continue;
+    int p[6];
+    float z[3];
+    tri_positions<KIND>(vp[k], 0, ja, p, z);
+    if (!vs_keep(p, ja.W, ja.H, S)) continue;
+    ++keepc;
+    int x0, y0, x1, y1;
+    clip_rect(p, S, ja.W, ja.H, x0, y0, x1, y1);
+    if (x1 <= x0 || y1 <= y0) continue;
+    const int tx0 = x0 >> lgT, ty0 = y0 >> lgT, tx1 = (x1 - 1) >> lgT, ty1 = (y1 - 1) >> lgT;
+    if (tx1 - tx0 <= 1 && ty1 - ty0 <= 1 && small_bbox(p, S)) {
+      uint32_t acc = 0;  // tiles that may hold a covered pixel
+      if ((x1 - x0) * (y1 - y0) <= kExactPx) {
+        // few pixels: the tiles that hold a covered pixel centre, exactly
+        // (raster.pikostage:223-241); a triangle covering none is dropped
+        const uint32_t half = 1u << (S - 1);
+        for (int py = y0; py < y1; ++py)
+          for (int px = x0; px < x1; ++px) {
+            const uint32_t cx = ((uint32_t)px << S) + half, cy = ((uint32_t)py << S) + half;
+            bool in = true;
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) {
+              const uint32_t xa = (uint32_t)p[2 * kk], ya = (uint32_t)p[2 * kk + 1];
+              const uint32_t xb = (uint32_t)p[(2 * kk + 2) % 6], yb = (uint32_t)p[(2 * kk + 3) % 6];
+              in &= (int)((cy - ya) * (xb - xa) - (cx - xa) * (yb - ya)) >= 0;
+            }
+            if (in) acc |= 1u << (((px >> lgT) - tx0) | (((py >> lgT) - ty0) << 1));
+          }
+      } else {
+        // RasterStage::assignBin's trivial-reject test (raster.pikostage:77-114)
+        Edges e;
+        e.set(p);
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+          const int bx = tx0 + (d & 1), by = ty0 + (d >> 1);
+          if (bx <= tx1 && by <= ty1 && tr_accept(e, bx, by, bsb)) acc |= 1u << d;
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        if (!((acc >> d) & 1u)) continue;
+        const uint32_t s = atomicAdd(&cnt[(ty0 + (d >> 1)) * ja.nBx + tx0 + (d & 1)], 1u);
+        if (d < 2) rb[k] |= s << (16 * d);
+        else rc[k] |= s << (16 * (d - 2));
+      }
+      ra[k] = 1u | (acc << 2) | ((uint32_t)tx0 << 8) | ((uint32_t)ty0 << 20);
+      if (!acc) continue;
+    } else {
+      rb[k] = atomicAdd(&cnt[ntiles + 1], 1u);
+      ra[k] = 2u;
+    }
+    // the setup record k_tile_raster reads for each listed triangle
+    const size_t gi = 3 * (size_t)(cd.tri0 + il);
+    Edges e;
+    e.set(p);
+    const float rcp = 1.0f / (float)e.denom();  // raster.pikostage:174
+    geo[gi + 0] = make_int4(p[0], p[1], p[2], p[3]);
+    geo[gi + 1] = make_int4(p[4], p[5], __float_as_int(z[2]), __float_as_int(z[0] - z[2]));
+    geo[gi + 2] = make_int4(__float_as_int(z[1] - z[2]), __float_as_int(rcp), 0, 0);
+  }
+  __syncthreads();
+  {  // exclusive scan of cnt[0..ntiles]; cnt[ntiles] becomes the chunk's pair total
+    const int m = ntiles + 1;
+    const int per = (m + kBinThreads - 1) / kBinThreads;
+    const int b = t * per, e = min(m, b + per);
+    uint32_t s = 0;
+    for (int i = b; i < e; ++i) s += cnt[i];
+    uint32_t tot;
+    uint32_t run = block_excl_scan<kBinThreads>(s, sh, tot);
+    for (int i = b; i < e; ++i) {
+      const uint32_t x = cnt[i];
+      cnt[i] = run;
+      run += x;
+    }
+  }
+  __syncthreads();
+  for (int q = t; q < ntiles + 2; q += kBinThreads) tab[cd.tab_off + q] = cnt[q];
+  uint32_t *reg = pairs + (size_t)c * kChunkCap;
+#pragma unroll
+  for (int k = 0; k < kChunkPer; ++k) {
+    if (!ra[k]) continue;
+    const uint32_t v = cd.tri0 + (uint32_t)(k * kBinThreads + t);
+    if (ra[k] & 1u) {
+      const int tx0 = (int)((ra[k] >> 8) & 0xfffu), ty0 = (int)(ra[k] >> 20);
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        if (!((ra[k] >> (2 + d)) & 1u)) continue;
+        const int tile = (ty0 + (d >> 1)) * ja.nBx + tx0 + (d & 1);
+        const uint32_t s = ((d < 2 ? rb[k] : rc[k]) >> (16 * (d & 1))) & 0xffffu;
+        reg[cnt[tile] + s] = v;
+      }
+    } else {
+      reg[kChunkCap - 1u - rb[k]] = v;
+    }
+  }
+  uint32_t kt;
+  block_excl_scan<kBinThreads>(keepc, sh, kt);
+  if (t == 0) chunk_keep[c] = kt;
+}
+
+// One wave's staged triangles in k_tile_raster (structure of arrays so that
+// lanes reading neighbouring triangles hit neighbouring banks).
+struct TileStage {
+  uint32_t pre[64];  // exclusive prefix of the triangles' span counts
+  int4 xy01[64];     // x0 y0 x1 y1 (fixed point)
+  int4 xy2z[64];     // x2 y2, z2 and z0 - z2 (float bits)
+  uint4 misc[64];    // z1 - z2 and 1/barydenom (float bits), rect x | y << 16,
+                     // rect w | h << 16 | (int32 edges) << 31
+  uint32_t tri[64];
+};
+
+// Rasterises staged triangle o's span k (a column segment of kSpan pixels of
+// its bbox clipped to the tile) into the tile's visibility words.
+template <int T>
+__device__ __forceinline__ void tile_span(const TileStage &s, int o, uint32_t k, int S, int xlo, int ylo,
+                                          unsigned long long *vis) {
+  const int4 a = s.xy01[o];
+  const int4 b = s.xy2z[o];
+  const uint4 m = s.misc[o];
+  const uint32_t w = m.w & 0xffffu, h = (m.w >> 16) & 0x7fffu;
+  // band and column of span k: (k + 0.5) / w is at least 0.5 / w from an
+  // integer and k < 1024, w <= 64, so the float quotient truncates exactly
+  const uint32_t q = (uint32_t)(((float)k + 0.5f) * __builtin_amdgcn_rcpf((float)w));
+  const uint32_t sx = k - q * w, sy = q * kSpan;
+  const int px = (int)(m.z & 0xffffu) + (int)sx, py0 = (int)(m.z >> 16) + (int)sy;
+  const uint32_t nv = min((uint32_t)kSpan, h - sy);
+  const float rcp = __uint_as_float(m.y);
+  const float z2 = __int_as_float(b.z), z0mz2 = __int_as_float(b.w), z1mz2 = __uint_as_float(m.x);
+  const unsigned long long low = (unsigned long long)(0xFFFFFFFEu - s.tri[o]);
+  unsigned long long *col = vis + (py0 - ylo) * T + (px - xlo);
+  const int xy[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
+  if (m.w >> 31) {
+    // int32-path triangle: the edge functions at the span's pixel centres in
+    // wrapping 32-bit arithmetic equal the exact values (all fit in int32)
+    const uint32_t half = 1u << (S - 1);
+    const uint32_t cx = ((uint32_t)px << S) + half, cy = ((uint32_t)py0 << S) + half;
+    uint32_t e[3], dy[3];
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk) {
+      const uint32_t xa = (uint32_t)xy[2 * kk], ya = (uint32_t)xy[2 * kk + 1];
+      const uint32_t xb = (uint32_t)xy[(2 * kk + 2) % 6], yb = (uint32_t)xy[(2 * kk + 3) % 6];
+      e[kk] = (cy - ya) * (xb - xa) - (cx - xa) * (yb - ya);
+      dy[kk] = (xb - xa) << S;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < (uint32_t)kSpan; ++j) {
+      if (j < nv && (int)(e[0] | e[1] | e[2]) >= 0) {
+        const float alpha = (float)(int)e[1] * rcp, beta = (float)(int)e[2] * rcp;  // raster.pikostage:269-270
+        const float z = interp(z0mz2, z1mz2, z2, alpha, beta);                      // raster.pikostage:272
+        atomicMin((long long *)(col + j * T), (long long)(((unsigned long long)__float_as_uint(z) << 32) | low));
+      }
+#pragma unroll
+      for (int kk = 0; kk < 3; ++kk) e[kk] += dy[kk];
+    }
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < (uint32_t)kSpan; ++j) {
+      float alpha, beta;
+      if (j < nv && covered_at(xy, false, px, py0 + (int)j, S, rcp, alpha, beta)) {
+        const float z = interp(z0mz2, z1mz2, z2, alpha, beta);
+        atomicMin((long long *)(col + j * T), (long long)(((unsigned long long)__float_as_uint(z) << 32) | low));
+      }
+    }
+  }
+}
+
+template <int T, int KIND, bool MULTI>
+__global__ void __launch_bounds__(kTRThreads) k_tile_raster(const int4 *__restrict__ geo,
+                                                             const void *__restrict__ attr, FrameArgs fa,
+                                                             const FrameArgs *__restrict__ jobs, int njobs,
+                                                             const uint32_t *__restrict__ tab,
+                                                             const uint32_t *__restrict__ pairs, int tile_lo,
+                                                             int tile_hi, uint32_t *__restrict__ color,
+                                                             float *__restrict__ depth) {
+  __shared__ unsigned long long vis[T * T];
+  __shared__ uint32_t w_pre[2][kTRWin];   // window prefix: pairs, large triangles
+  __shared__ uint32_t w_base[2][kTRWin];  // where the chunk's run starts in `pairs`
+  __shared__ TileStage st[kTRThreads / 64];
+  __shared__ uint32_t sh[kTRThreads / 64 + 1];
+  __shared__ float s_u8[256];
+  const int t = threadIdx.x, lane = t & 63;
+  TileStage &s = st[t >> 6];
+  // block i runs on XCD i % 8: give each XCD a contiguous range of tiles, so
+  // the table entries of neighbouring tiles share its L2
+  const int nt = tile_hi - tile_lo;
+  const int per = (nt + 7) / 8;
+  const int tb = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+  if (tb >= nt) return;
+  const int tile = tile_lo + tb;
+  const FrameArgs &ja = MULTI ? jobs[job_search(jobs, njobs, (uint32_t)tile, true)] : fa;
+  const int lt = MULTI ? tile - (int)ja.bin_base : tile;
+  const int ntiles = ja.nBx * ja.nBy;
+  const int tx = lt % ja.nBx, ty = lt / ja.nBx;
+  const int xlo = tx * T, ylo = ty * T, xhi = min(xlo + T, ja.W), yhi = min(ylo + T, ja.H);
+  const int S = ja.S;
+  for (int q = t; q < 256; q += kTRThreads) s_u8[q] = kU8Table.v[q];
+  for (int q = t; q < T * T; q += kTRThreads) vis[q] = kVisSentinel;
+  const uint32_t nch = ja.nchunks, col_h = (uint32_t)ntiles + 2u;
+  for (uint32_t w0 = 0; w0 < nch; w0 += kTRWin) {
+    // thread t reads chunks w0 + t*kTRWinPer ... (consecutive, so the scan
+    // keeps chunk order)
+    uint32_t cnt[2][kTRWinPer], base[2][kTRWinPer], sum[2] = {0, 0};
+#pragma unroll
+    for (int k = 0; k < kTRWinPer; ++k) {
+      const uint32_t c = w0 + (uint32_t)(t * kTRWinPer + k);
+      cnt[0][k] = cnt[1][k] = base[0][k] = base[1][k] = 0;
+      if (c < nch) {
+        const uint32_t *colm = tab + ja.tab_base + (size_t)c * col_h;
+        const uint32_t o = colm[lt], o1 = colm[lt + 1], nl = colm[ntiles + 1];
+        const uint32_t reg = (ja.chunk_base + c) * kChunkCap;
+        cnt[0][k] = o1 - o;
+        base[0][k] = reg + o;
+        cnt[1][k] = nl;
+        base[1][k] = reg + kChunkCap - nl;
+      }
+      sum[0] += cnt[0][k];
+      sum[1] += cnt[1][k];
+    }
+    uint32_t tot[2];
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+      uint32_t run = block_excl_scan<kTRThreads>(sum[l], sh, tot[l]);
+#pragma unroll
+      for (int k = 0; k < kTRWinPer; ++k) {
+        w_pre[l][t * kTRWinPer + k] = run;
+        w_base[l][t * kTRWinPer + k] = base[l][k];
+        run += cnt[l][k];
+      }
+    }
+    __syncthreads();
+    for (int l = 0; l < 2; ++l) {  // the tile's pairs, then the large triangles
+      // entry j of the window's list -> triangle index (~0u past the end)
+      auto entry = [&](uint32_t j) -> uint32_t {
+        if (j >= tot[l]) return ~0u;
+        int lo = 0;  // chunk of entry j: largest k with w_pre[k] <= j
+#pragma unroll
+        for (int step = kTRWin / 2; step > 0; step >>= 1)
+          if (w_pre[l][lo + step] <= j) lo += step;
+        return pairs[w_base[l][lo] + (j - w_pre[l][lo])];
+      };
+      auto setup = [&](uint32_t v, int4 *out) {
+        if (v != ~0u) {
+#pragma unroll
+          for (int q = 0; q < 3; ++q) out[q] = geo[3 * (size_t)v + q];
+        } else {
+#pragma unroll
+          for (int q = 0; q < 3; ++q) out[q] = make_int4(0, 0, 0, 0);
+        }
+      };
+      // software pipeline: the setup records of the next batch and the
+      // entries of the one after are in flight while a batch is rasterised
+      uint32_t v_cur = entry((uint32_t)t);
+      int4 gc[3];
+      setup(v_cur, gc);
+      uint32_t v_nxt = entry((uint32_t)(kTRThreads + t));
+      for (uint32_t b0 = 0; b0 < tot[l]; b0 += kTRThreads) {
+        int4 gn[3];
+        setup(v_nxt, gn);
+        const uint32_t v_nn = entry(b0 + 2u * kTRThreads + (uint32_t)t);
+        uint32_t units = 0;
+        if (v_cur != ~0u) {
+          const int p[6] = {gc[0].x, gc[0].y, gc[0].z, gc[0].w, gc[1].x, gc[1].y};
+          int x0, y0, x1, y1;
+          clip_rect(p, S, ja.W, ja.H, x0, y0, x1, y1);
+          x0 = max(x0, xlo);
+          y0 = max(y0, ylo);
+          x1 = min(x1, xhi);
+          y1 = min(y1, yhi);
+          const int w = max(0, x1 - x0), h = max(0, y1 - y0);
+          if (w > 0 && h > 0) {
+            const float rcp = __int_as_float(gc[2].y);
+            const float z2 = __int_as_float(gc[1].z), z0mz2 = __int_as_float(gc[1].w), z1mz2 = __int_as_float(gc[2].x);
+            const bool small = small_bbox(p, S);
+            if (small && w * h <= kLanePx) {
+              // a few pixels: this lane tests them itself (raster.pikostage:223-241)
+              const uint32_t half = 1u << (S - 1);
+              const uint32_t cx = ((uint32_t)x0 << S) + half, cy = ((uint32_t)y0 << S) + half;
+              uint32_t er[3], dx[3], dy[3];
+#pragma unroll
+              for (int kk = 0; kk < 3; ++kk) {
+                const uint32_t xa = (uint32_t)p[2 * kk], ya = (uint32_t)p[2 * kk + 1];
+                const uint32_t xb = (uint32_t)p[(2 * kk + 2) % 6], yb = (uint32_t)p[(2 * kk + 3) % 6];
+                er[kk] = (cy - ya) * (xb - xa) - (cx - xa) * (yb - ya);
+                dx[kk] = (0u - (yb - ya)) << S;
+                dy[kk] = (xb - xa) << S;
+              }
+              const unsigned long long low = (unsigned long long)(0xFFFFFFFEu - v_cur);
+              for (int r = 0; r < h; ++r) {
+                uint32_t e0 = er[0], e1 = er[1], e2 = er[2];
+                for (int c = 0; c < w; ++c) {
+                  if ((int)(e0 | e1 | e2) >= 0) {
+                    const float alpha = (float)(int)e1 * rcp, beta = (float)(int)e2 * rcp;  // raster.pikostage:269-270
+                    const float zz = interp(z0mz2, z1mz2, z2, alpha, beta);                // raster.pikostage:272
+                    atomicMin((long long *)&vis[(y0 - ylo + r) * T + (x0 - xlo + c)],
+                              (long long)(((unsigned long long)__float_as_uint(zz) << 32) | low));
+                  }
+                  e0 += dx[0];
+                  e1 += dx[1];
+                  e2 += dx[2];
+                }
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk) er[kk] += dy[kk];
+              }
+            } else {
+              units = (uint32_t)w * (uint32_t)((h + kSpan - 1) / kSpan);
+              s.xy01[lane] = make_int4(p[0], p[1], p[2], p[3]);
+              s.xy2z[lane] = make_int4(p[4], p[5], gc[1].z, gc[1].w);
+              s.misc[lane] = make_uint4(__float_as_uint(z1mz2), __float_as_uint(rcp),
+                                        (uint32_t)x0 | ((uint32_t)y0 << 16),
+                                        (uint32_t)w | ((uint32_t)h << 16) | (small ? 0x80000000u : 0u));
+              s.tri[lane] = v_cur;
+            }
+          }
+        }
+        v_cur = v_nxt;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) gc[q] = gn[q];
+        v_nxt = v_nn;
+        const uint32_t inc = wave_incl_scan(units);
+        s.pre[lane] = inc - units;
+        const uint32_t wtot = __shfl(inc, 63, 64);
+        wave_fence();
+        for (uint32_t cidx = (uint32_t)lane; cidx < wtot; cidx += 64) {
+          int o = 0;  // owner: largest o with pre[o] <= cidx
+#pragma unroll
+          for (int step = 32; step > 0; step >>= 1)
+            if (s.pre[o + step] <= cidx) o += step;
+          tile_span<T>(s, o, cidx - s.pre[o], S, xlo, ylo, vis);
+        }
+        wave_fence();
+      }
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  // resolve: depth = the winner's z, colour = Gouraud of the winner
+  const size_t fbb = MULTI ? (size_t)ja.fb_base : 0;
+  for (int q = t; q < T * T; q += kTRThreads) {
+    const int px = xlo + (q % T), py = ylo + (q / T);
+    if (px >= xhi || py >= yhi) continue;
+    const unsigned long long key = vis[q];
+    uint32_t col = kClearColor;
+    if (key != kVisSentinel) {
+      const uint32_t v = 0xFFFFFFFEu - (uint32_t)(key & 0xFFFFFFFFull);
+      const size_t src = MULTI ? (size_t)ja.src_base + (size_t)(v - ja.vbase) : (size_t)v;
+      const int4 a = geo[3 * (size_t)v], b = geo[3 * (size_t)v + 1], cg = geo[3 * (size_t)v + 2];
+      const int p[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
+      uint32_t c[3];
+      tri_colours<KIND>(attr, src, ja, c);
+      float alpha = 0.0f, beta = 0.0f;
+      covered_at(p, small_bbox(p, S), px, py, S, __int_as_float(cg.y), alpha, beta);
+      col = gouraud(s_u8, c[0], c[1], c[2], alpha, beta);
+    }
+    const size_t pid = fbb + (size_t)py * ja.W + px;
+    depth[pid] = __int_as_float((int)(key >> 32));
+    color[pid] = col;
+  }
 }
 
 inline int ilog2i(int v) {
@@ -1098,8 +1556,18 @@ struct piko_raster_pipe {
   uint32_t *d_large = nullptr;          // [large_cap] large triangles of the frame
   uint32_t *d_tile_off = nullptr;       // [large_cap] their first 8x8 tile
   uint32_t large_cap = 0;
-  int mode = PIKO_RASTER_DIRECT;        // frame path
-  int last_mode = PIKO_RASTER_DIRECT;   // path of the last enqueued frame
+  // tiled path: chunk table, offset table, per-chunk pair regions
+  ChunkDesc *d_chunks = nullptr;
+  ChunkDesc *h_chunks = nullptr;        // pinned staging for the chunk table
+  uint32_t *d_tab = nullptr;            // [cap_tab] chunk-major tile offsets
+  uint32_t *d_tpairs = nullptr;         // [cap_chunks * kChunkCap] triangle indices
+  uint32_t *d_chunk_keep = nullptr;     // [cap_chunks] emitted triangles per chunk
+  long long cap_chunks = 0, cap_tab = 0, chunk_bytes = 0;
+  int nchunks = 0;
+  FrameArgs fa_t{};                     // plain pipe job in tile numbering
+  RasterClass tclass{};                 // plain pipe: its one tile class
+  int mode = PIKO_RASTER_TILED;         // frame path
+  int last_mode = PIKO_RASTER_TILED;    // path of the last enqueued frame
   bool bins_current = false;            // the bin queues hold the last frame's pairs
   int final_buf = 0;
   int raster_waves = 0;       // persistent raster grid (default)
@@ -1208,6 +1676,83 @@ int alloc_pix(piko_raster_pipe *p, long long npix) {
   return hipGetLastError() == hipSuccess ? PIKO_OK : PIKO_ERR_HIP;
 }
 
+void free_chunks(piko_raster_pipe *p) {
+  dfree(p->d_chunks);
+  dfree(p->d_tab);
+  dfree(p->d_tpairs);
+  dfree(p->d_chunk_keep);
+  if (p->h_chunks) (void)hipHostFree(p->h_chunks);
+  p->h_chunks = nullptr;
+  p->device_bytes -= p->chunk_bytes;
+  p->chunk_bytes = 0;
+  p->cap_chunks = p->cap_tab = 0;
+}
+
+int alloc_chunks(piko_raster_pipe *p, long long nch, long long ntab) {
+  p->cap_chunks = std::max<long long>(nch, 1);
+  p->cap_tab = std::max<long long>(ntab, 1);
+  int rc;
+  if ((rc = dalloc(p, &p->d_chunks, p->cap_chunks, &p->chunk_bytes)) != PIKO_OK) return rc;
+  if ((rc = dalloc(p, &p->d_tab, p->cap_tab, &p->chunk_bytes)) != PIKO_OK) return rc;
+  if ((rc = dalloc(p, &p->d_tpairs, p->cap_chunks * (long long)kChunkCap, &p->chunk_bytes)) != PIKO_OK) return rc;
+  if ((rc = dalloc(p, &p->d_chunk_keep, p->cap_chunks, &p->chunk_bytes)) != PIKO_OK) return rc;
+  if (hipHostMalloc((void **)&p->h_chunks, (size_t)p->cap_chunks * sizeof(ChunkDesc)) != hipSuccess) {
+    p->h_chunks = nullptr;
+    return PIKO_ERR_OOM;
+  }
+  return PIKO_OK;
+}
+
+// Tile edge of the tiled path for a job asked to bin at T: at least 16, and
+// large enough that one job has at most kMaxTilesJob tiles (64 always is for
+// screens up to 4095 x 4095).
+int tiled_T(int T, int W, int H) {
+  int tt = std::max(T, 16);
+  while (tt < 64 && cdiv(W, tt) * cdiv(H, tt) > kMaxTilesJob) tt *= 2;
+  return tt;
+}
+
+// Chunks of the jobs (device order; triangle counts from the vbase steps and
+// the batch total): sets each job's chunk_base / nchunks / tab_base, grows
+// the tables when needed and enqueues the chunk table's upload.
+int setup_chunks(piko_raster_pipe *p, FrameArgs *jobs, int njobs, long long total_tris) {
+  long long nch = 0, ntab = 0;
+  for (int j = 0; j < njobs; ++j) {
+    const long long end = j + 1 < njobs ? (long long)jobs[j + 1].vbase : total_tris;
+    const long long cnt = end - (long long)jobs[j].vbase;
+    const long long nc = cdiv(cnt, kChunk);
+    jobs[j].chunk_base = (uint32_t)nch;
+    jobs[j].nchunks = (uint32_t)nc;
+    jobs[j].tab_base = (uint32_t)ntab;
+    nch += nc;
+    ntab += nc * ((long long)jobs[j].nBx * jobs[j].nBy + 2);
+  }
+  if (nch > 0xFFFFFFFFll / kChunkCap || ntab > 0xFFFFFFFFll) return PIKO_ERR_CAPACITY;
+  HIPOK(hipStreamSynchronize(p->stream));  // the staging buffer may still be in use
+  if (nch > p->cap_chunks || ntab > p->cap_tab || !p->h_chunks) {
+    free_chunks(p);
+    int rc = alloc_chunks(p, nch, ntab);
+    if (rc != PIKO_OK) return rc;
+  }
+  long long k = 0;
+  for (int j = 0; j < njobs; ++j) {
+    const uint32_t ntl = (uint32_t)(jobs[j].nBx * jobs[j].nBy) + 2u;
+    const long long end = j + 1 < njobs ? (long long)jobs[j + 1].vbase : total_tris;
+    for (uint32_t c = 0; c < jobs[j].nchunks; ++c, ++k) {
+      ChunkDesc &d = p->h_chunks[k];
+      d.job = (uint32_t)j;
+      d.tri0 = jobs[j].vbase + c * (uint32_t)kChunk;
+      d.count = (uint32_t)std::min<long long>(kChunk, end - (long long)d.tri0);
+      d.tab_off = jobs[j].tab_base + c * ntl;
+    }
+  }
+  p->nchunks = (int)nch;
+  if (nch > 0)
+    HIPOK(hipMemcpyAsync(p->d_chunks, p->h_chunks, (size_t)nch * sizeof(ChunkDesc), hipMemcpyHostToDevice,
+                         p->stream));
+  return PIKO_OK;
+}
+
 void free_bins(piko_raster_pipe *p) {
   dfree(p->d_bstart);
   dfree(p->d_bend);
@@ -1237,6 +1782,7 @@ int radix_passes(int nbins) {
 const char *kernel_name(const piko_raster_pipe *p, int k) {
   static const char *direct[] = {"k_vs_direct", "k_large_raster", "k_resolve"};
   if (p->last_mode == PIKO_RASTER_DIRECT) return k < 3 ? direct[k] : "";
+  if (p->last_mode == PIKO_RASTER_TILED) return k == 0 ? "k_tile_bin" : "k_tile_raster";
   static const char *head[] = {"k_vs_count", "k_scan_tiles", "k_emit_pairs"};
   static const char *pass[] = {"k_sort_count", "k_sort_rows", "k_sort_scatter"};
   if (k < 3) return head[k];
@@ -1352,13 +1898,69 @@ void enqueue_direct(piko_raster_pipe *p, Marks &mark) {
                                                         p->fa, p->d_jobs, nj, p->d_vis);
   mark();
   const long long rblocks = std::max<long long>(cdiv(p->npix, 256), 1);
-  if (p->multi)
-    k_resolve<true><<<(unsigned)rblocks, 256, 0, p->stream>>>(p->d_geo, p->d_col, p->d_vis, p->fa, p->d_jobs, nj,
-                                                              p->npix, p->d_color, p->d_depth, p->d_fs);
-  else
-    k_resolve<false><<<(unsigned)rblocks, 256, 0, p->stream>>>(p->d_geo, p->d_col, p->d_vis, p->fa, p->d_jobs, nj,
-                                                               p->npix, p->d_color, p->d_depth, p->d_fs);
+#define PIKO_RES_LAUNCH(KIND, MULTI)                                                                          \
+  k_resolve<KIND, MULTI><<<(unsigned)rblocks, 256, 0, p->stream>>>(p->d_geo, p->d_attr, p->d_vis, p->fa, p->d_jobs, \
+                                                                   nj, p->npix, p->d_color, p->d_depth, p->d_fs)
+  if (p->kind == PIKO_INPUT_PRETRANSFORMED) {
+    if (p->multi) PIKO_RES_LAUNCH(PIKO_INPUT_PRETRANSFORMED, true);
+    else PIKO_RES_LAUNCH(PIKO_INPUT_PRETRANSFORMED, false);
+  } else {
+    if (p->multi) PIKO_RES_LAUNCH(PIKO_INPUT_WORLD, true);
+    else PIKO_RES_LAUNCH(PIKO_INPUT_WORLD, false);
+  }
+#undef PIKO_RES_LAUNCH
   mark();
+}
+
+template <bool MULTI>
+void launch_tile_raster(piko_raster_pipe *p, const RasterClass &c, const FrameArgs &fa) {
+  const int nt = c.bin_hi - c.bin_lo;
+  if (nt <= 0) return;
+  const unsigned grid = 8u * (unsigned)((nt + 7) / 8);
+  const int nj = (int)p->jobs.size();
+#define PIKO_TR_LAUNCH(TT, KIND)                                                                                 \
+  k_tile_raster<TT, KIND, MULTI><<<grid, kTRThreads, 0, p->stream>>>(p->d_geo, p->d_attr, fa, p->d_jobs, nj,    \
+                                                                     p->d_tab, p->d_tpairs, c.bin_lo, c.bin_hi, \
+                                                                     p->d_color, p->d_depth)
+#define PIKO_TR_KIND(TT)                                                       \
+  if (p->kind == PIKO_INPUT_PRETRANSFORMED) PIKO_TR_LAUNCH(TT, PIKO_INPUT_PRETRANSFORMED); \
+  else PIKO_TR_LAUNCH(TT, PIKO_INPUT_WORLD)
+  switch (c.T) {
+    case 16: PIKO_TR_KIND(16); break;
+    case 32: PIKO_TR_KIND(32); break;
+    default: PIKO_TR_KIND(64); break;
+  }
+#undef PIKO_TR_KIND
+#undef PIKO_TR_LAUNCH
+}
+
+// Tiled frame: chunk binning, then one raster launch per tile-size class.
+void enqueue_tiled(piko_raster_pipe *p, Marks &mark) {
+  const FrameArgs &fa = p->multi ? p->fa : p->fa_t;
+  if (p->nchunks > 0) {
+#define PIKO_TB_LAUNCH(KIND, MULTI)                                                                          \
+  k_tile_bin<KIND, MULTI><<<p->nchunks, kBinThreads, 0, p->stream>>>(p->d_pos, fa, p->d_jobs, p->d_chunks, \
+                                                                     p->d_tab, p->d_tpairs, p->d_chunk_keep,   \
+                                                                     p->d_geo)
+    if (p->kind == PIKO_INPUT_PRETRANSFORMED) {
+      if (p->multi) PIKO_TB_LAUNCH(PIKO_INPUT_PRETRANSFORMED, true);
+      else PIKO_TB_LAUNCH(PIKO_INPUT_PRETRANSFORMED, false);
+    } else {
+      if (p->multi) PIKO_TB_LAUNCH(PIKO_INPUT_WORLD, true);
+      else PIKO_TB_LAUNCH(PIKO_INPUT_WORLD, false);
+    }
+#undef PIKO_TB_LAUNCH
+  }
+  mark();
+  if (p->multi) {
+    for (const RasterClass &c : p->classes) {
+      launch_tile_raster<true>(p, c, fa);
+      if (c.bin_hi > c.bin_lo) mark();
+    }
+  } else {
+    launch_tile_raster<false>(p, p->tclass, fa);
+    mark();
+  }
 }
 
 // Enqueues one frame of the current job set.  ev (optional) receives an event
@@ -1366,7 +1968,10 @@ void enqueue_direct(piko_raster_pipe *p, Marks &mark) {
 int enqueue_frame(piko_raster_pipe *p, hipEvent_t *ev, int *nev) {
   Marks mark{ev, p->stream};
   p->last_mode = p->mode;
-  if (p->mode == PIKO_RASTER_DIRECT) {
+  if (p->mode == PIKO_RASTER_TILED) {
+    enqueue_tiled(p, mark);
+    p->bins_current = false;
+  } else if (p->mode == PIKO_RASTER_DIRECT) {
     enqueue_direct(p, mark);
     p->bins_current = false;
   } else {
@@ -1452,7 +2057,7 @@ int validate_opts(const piko_constant_state *cs, int count, const piko_raster_op
   if (opt->subpixel_bits < 1 || opt->subpixel_bits > 8) return PIKO_ERR_INVALID;
   if (opt->input_kind != PIKO_INPUT_PRETRANSFORMED && opt->input_kind != PIKO_INPUT_WORLD)
     return PIKO_ERR_INVALID;
-  if (opt->raster_mode < PIKO_RASTER_AUTO || opt->raster_mode > PIKO_RASTER_DIRECT) return PIKO_ERR_INVALID;
+  if (opt->raster_mode < PIKO_RASTER_AUTO || opt->raster_mode > PIKO_RASTER_TILED) return PIKO_ERR_INVALID;
   return PIKO_OK;
 }
 
@@ -1482,7 +2087,9 @@ int create_common(piko_raster_pipe *p, int kind, int S, long long tris, long lon
   p->S = S;
   if (mode == PIKO_RASTER_AUTO) {
     const char *m = std::getenv("PIKO_RASTER_MODE");
-    mode = (m && std::strcmp(m, "binned") == 0) ? PIKO_RASTER_BINNED : PIKO_RASTER_DIRECT;
+    mode = PIKO_RASTER_TILED;
+    if (m && std::strcmp(m, "binned") == 0) mode = PIKO_RASTER_BINNED;
+    if (m && std::strcmp(m, "direct") == 0) mode = PIKO_RASTER_DIRECT;
   }
   p->mode = p->last_mode = mode;
   {
@@ -1551,6 +2158,12 @@ int plain_pipe(piko_raster_pipe **out, const piko_constant_state *cs, const floa
                      opt->raster_mode);
   p->classes.assign(1, RasterClass{p->T, 0, p->nbins, opt->waves_per_cu > 0 ? opt->waves_per_cu * p->cus : 0});
   if (rc == PIKO_OK) {
+    const int tt = tiled_T(p->T, p->W, p->H);
+    p->fa_t = make_args(cs, opt->subpixel_bits, tt);
+    p->tclass = RasterClass{tt, 0, (int)p->fa_t.nbins, 0};
+    rc = setup_chunks(p, &p->fa_t, 1, count);
+  }
+  if (rc == PIKO_OK) {
     if (on_device) {
       p->d_pos = pos;
       p->d_attr = attr;
@@ -1590,6 +2203,7 @@ int piko_raster_destroy(piko_raster_pipe *p) {
   free_tris(p);
   free_pix(p);
   free_bins(p);
+  free_chunks(p);
   dfree(p->d_rowtot);
   dfree(p->d_fs);
   dfree(p->d_jobs);
@@ -1618,6 +2232,11 @@ int piko_raster_prepare(piko_raster_pipe *p, const piko_constant_state *cs) {
   if (cs->screenSizeX != p->W || cs->screenSizeY != p->H) return PIKO_ERR_INVALID;
   p->fa = make_args(cs, p->S, p->T);
   p->jobs.assign(1, p->fa);
+  FrameArgs ft = make_args(cs, p->S, p->tclass.T);
+  ft.chunk_base = p->fa_t.chunk_base;
+  ft.nchunks = p->fa_t.nchunks;
+  ft.tab_base = p->fa_t.tab_base;
+  p->fa_t = ft;
   return PIKO_OK;
 }
 
@@ -1759,7 +2378,7 @@ int piko_batch_create(piko_raster_pipe **out, const float *pos, const void *attr
   if (opt->subpixel_bits < 1 || opt->subpixel_bits > 8) return PIKO_ERR_INVALID;
   if (opt->input_kind != PIKO_INPUT_PRETRANSFORMED && opt->input_kind != PIKO_INPUT_WORLD) return PIKO_ERR_INVALID;
   if (opt->max_jobs <= 0 || opt->max_tris < 0 || opt->max_pixels <= 0) return PIKO_ERR_INVALID;
-  if (opt->raster_mode < PIKO_RASTER_AUTO || opt->raster_mode > PIKO_RASTER_DIRECT) return PIKO_ERR_INVALID;
+  if (opt->raster_mode < PIKO_RASTER_AUTO || opt->raster_mode > PIKO_RASTER_TILED) return PIKO_ERR_INVALID;
   if (input_tris > 0 && (!pos || !attr)) return PIKO_ERR_INVALID;
   piko_raster_pipe *p = new (std::nothrow) piko_raster_pipe();
   if (!p) return PIKO_ERR_OOM;
@@ -1811,6 +2430,12 @@ int piko_batch_query(piko_raster_pipe *p) {
   return hipStreamQuery(p->stream) == hipSuccess ? 1 : 0;
 }
 
+// Bin (tile) edge a job is rasterised with: its own in the binned and direct
+// paths, the tiled path's tile edge otherwise.
+static int job_T(const piko_raster_pipe *p, const piko_raster_job &jb) {
+  return p->mode == PIKO_RASTER_TILED ? tiled_T(jb.bin_size, jb.cs.screenSizeX, jb.cs.screenSizeY) : jb.bin_size;
+}
+
 int piko_batch_submit(piko_raster_pipe *p, const piko_raster_job *jobs, int njobs) {
   if (!p || !p->multi || !jobs || njobs <= 0 || njobs > p->cap_jobs) return PIKO_ERR_INVALID;
   int rc = piko_batch_wait(p);  // one batch in flight per pipe
@@ -1820,10 +2445,10 @@ int piko_batch_submit(piko_raster_pipe *p, const piko_raster_job *jobs, int njob
     const piko_raster_job &jb = jobs[j];
     if (jb.count < 0 || jb.src_base < 0 || jb.src_base + jb.count > p->input_tris) return PIKO_ERR_INVALID;
     if (!valid_bin(jb.bin_size) || !valid_screen(jb.cs.screenSizeX, jb.cs.screenSizeY)) return PIKO_ERR_INVALID;
+    const int T = job_T(p, jb);
     tris += jb.count;
     pix += (long long)jb.cs.screenSizeX * jb.cs.screenSizeY;
-    bins += (long long)((jb.cs.screenSizeX + jb.bin_size - 1) / jb.bin_size) *
-            ((jb.cs.screenSizeY + jb.bin_size - 1) / jb.bin_size);
+    bins += (long long)((jb.cs.screenSizeX + T - 1) / T) * ((jb.cs.screenSizeY + T - 1) / T);
   }
   if (tris > 0x7FFFFFFFll || bins > 0x7FFFFFFFll) return PIKO_ERR_CAPACITY;
   // grow per-triangle, per-pixel and per-bin storage when this batch needs it
@@ -1854,7 +2479,7 @@ int piko_batch_submit(piko_raster_pipe *p, const piko_raster_job *jobs, int njob
     const uint32_t lo = bb;
     int wpc = 0;
     for (int j = 0; j < njobs; ++j) {
-      if (jobs[j].bin_size != T) continue;
+      if (job_T(p, jobs[j]) != T) continue;
       wpc = std::max(wpc, jobs[j].waves_per_cu);
       const double qd = jobs[j].queue_depth > 0.0f ? (double)jobs[j].queue_depth : 2.0;
       want_pairs += (long long)(qd * jobs[j].count) + 64;
@@ -1879,6 +2504,7 @@ int piko_batch_submit(piko_raster_pipe *p, const piko_raster_job *jobs, int njob
   p->nbins = (int)bb;
   p->npix = (long long)fbb;
   p->passes = radix_passes(p->nbins);
+  if ((rc = setup_chunks(p, p->jobs.data(), njobs, tris)) != PIKO_OK) return rc;
   std::memcpy(p->h_jobs, p->jobs.data(), (size_t)njobs * sizeof(FrameArgs));
   HIPOK(hipMemcpyAsync(p->d_jobs, p->h_jobs, (size_t)njobs * sizeof(FrameArgs), hipMemcpyHostToDevice, p->stream));
   rc = enqueue_frame(p, nullptr, nullptr);

@@ -26,12 +26,15 @@ constexpr unsigned long long kVisSentinel =
 //   src_base first triangle of the job in the input buffer
 //   bin_base first bin of the job in the batch's bin numbering
 //   fb_base  first pixel of the job's framebuffer in the batch's buffers
+// The tiled path also reads where the job's chunks and their offset table
+// start (chunk_base, tab_base) and how many chunks it has (nchunks).
 struct FrameArgs {
   int W, H, S, lgT, nBx, nBy;
   float halfW, halfH;
   float vp[16];
   float light[3], matcol[3], ambient[3];
   uint32_t vbase, src_base, bin_base, nbins;
+  uint32_t chunk_base, nchunks, tab_base, pad_;
   unsigned long long fb_base;
 };
 

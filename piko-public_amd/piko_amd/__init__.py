@@ -26,7 +26,7 @@ _lib = None
 
 PIKO_INPUT_PRETRANSFORMED = 0
 PIKO_INPUT_WORLD = 1
-RASTER_AUTO, RASTER_BINNED, RASTER_DIRECT = 0, 1, 2   # PIKO_RASTER_* (include/piko_hip.h)
+RASTER_AUTO, RASTER_BINNED, RASTER_DIRECT, RASTER_TILED = 0, 1, 2, 3   # PIKO_RASTER_* (include/piko_hip.h)
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "no HIP device",
           -4: "queue capacity exceeded", -5: "device out of memory", -6: "admission refused"}
@@ -192,7 +192,7 @@ class RasterPipe:
     def allocate(self, scene, bin_size=16, subpixel_bits=8, pair_capacity=0, waves_per_cu=0, raster_mode=0):
         """RasterPipe::allocate (PTXBackend.cpp:649-789): uploads the input and
         sizes every stage.  `scene` is a scenes.TriScene or scenes.WorldScene.
-        raster_mode: RASTER_AUTO, RASTER_BINNED or RASTER_DIRECT (same image)."""
+        raster_mode: RASTER_AUTO, RASTER_BINNED, RASTER_DIRECT or RASTER_TILED (same image)."""
         if self._h is not None:
             raise RuntimeError("pipe already allocated")
         self.W, self.H, self.T, self.S = scene.W, scene.H, bin_size, subpixel_bits

@@ -1,11 +1,12 @@
 """Per-scene-class autotuner for the raster pipeline.
 
 A scene class is (log2 of the triangle count, log2 of the pixel-count edge).
-For one representative scene per class, every candidate (bin size T, raster
-waves per CU) is timed on the GPU with the plain pipe; the fastest wins.  The
-queue depth recorded for the class is the measured (triangle, bin) pairs per
-triangle at the chosen T, with 25% headroom, so batches can size their queues
-up front instead of growing them after an overflow.
+For one representative scene per class, every candidate is timed on the GPU
+with the plain pipe; the fastest wins.  Candidates are the bin (tile) size T
+and, for the binned raster, the persistent raster waves per CU.  The queue
+depth recorded for the class is the measured (triangle, bin) pairs per
+triangle at the chosen T, with 25% headroom, so binned batches can size their
+queues up front instead of growing them after an overflow.
 """
 import json
 import math
@@ -13,18 +14,27 @@ import time
 
 import numpy as np
 
-from . import RasterPipe
+from . import RASTER_AUTO, RASTER_BINNED, RasterPipe
 
 BIN_SIZES = (8, 16, 32, 64)
 WAVES_PER_CU = (8, 16, 24)
+TILE_SIZES = (16, 32, 64)  # the tiled raster's tile edges (8 is drawn in 16-pixel tiles)
+
+
+def candidates(raster_mode):
+    """(bin sizes, waves per CU) worth timing for a raster mode: the waves only
+    shape the binned raster's persistent grid."""
+    if raster_mode == RASTER_BINNED:
+        return BIN_SIZES, WAVES_PER_CU
+    return TILE_SIZES, (0,)
 
 
 def scene_class(n, W, H):
     return (int(round(math.log2(max(n, 1)))), int(round(math.log2(max(W * H, 1)) / 2)))
 
 
-def time_plain(scene, T, waves, S=8, frames=4):
-    with RasterPipe().allocate(scene, T, S, waves_per_cu=waves) as pipe:
+def time_plain(scene, T, waves, S=8, frames=4, raster_mode=RASTER_AUTO):
+    with RasterPipe().allocate(scene, T, S, waves_per_cu=waves, raster_mode=raster_mode) as pipe:
         pipe.prepare()
         pipe.run_frames(1)
         t0 = time.perf_counter()
@@ -34,12 +44,15 @@ def time_plain(scene, T, waves, S=8, frames=4):
     return dt, st["pairs"] / max(scene.n, 1)
 
 
-def tune(scene, bin_sizes=BIN_SIZES, waves=WAVES_PER_CU, S=8):
+def tune(scene, bin_sizes=None, waves=None, S=8, raster_mode=RASTER_AUTO):
+    cb, cw = candidates(raster_mode)
+    bin_sizes = bin_sizes or cb
+    waves = waves or cw
     best = None
     trials = []
     for T in bin_sizes:
         for w in waves:
-            dt, qd = time_plain(scene, T, w, S)
+            dt, qd = time_plain(scene, T, w, S, raster_mode=raster_mode)
             trials.append((T, w, dt))
             if best is None or dt < best[2]:
                 best = (T, w, dt, qd)
@@ -48,7 +61,7 @@ def tune(scene, bin_sizes=BIN_SIZES, waves=WAVES_PER_CU, S=8):
             "trials": trials}
 
 
-def autotune(scene_list, S=8, log=None):
+def autotune(scene_list, S=8, log=None, raster_mode=RASTER_AUTO):
     """Returns (per-scene settings list, class table).  The representative of
     a class is its scene with the median triangle count."""
     groups = {}
@@ -58,7 +71,7 @@ def autotune(scene_list, S=8, log=None):
     for cls, members in sorted(groups.items()):
         members = sorted(members, key=lambda k: scene_list[k].n)
         rep = scene_list[members[len(members) // 2]]
-        table[cls] = tune(rep, S=S)
+        table[cls] = tune(rep, S=S, raster_mode=raster_mode)
         if log:
             log("class %s (%d scenes): T=%d waves/CU=%d queue depth %.2f (%.3f ms)" % (
                 cls, len(members), table[cls]["bin_size"], table[cls]["waves_per_cu"], table[cls]["queue_depth"],
@@ -73,4 +86,4 @@ def save(table, path):
                   indent=1)
 
 
-__all__ = ["scene_class", "tune", "autotune", "save", "np"]
+__all__ = ["scene_class", "candidates", "tune", "autotune", "save", "np"]

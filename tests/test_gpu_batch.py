@@ -1,6 +1,6 @@
 """GPU parity of the batched form: every job of a batch (one kernel sequence
 for all screens) equals the plain pipe on that screen alone and the oracle,
-bit for bit, in both raster modes."""
+bit for bit, in every raster mode."""
 import sys
 
 import numpy as np
@@ -11,7 +11,7 @@ from piko_amd import scenes
 
 pytestmark = pytest.mark.gpu
 
-MODES = {"binned": piko_amd.RASTER_BINNED, "direct": piko_amd.RASTER_DIRECT}
+MODES = {"binned": piko_amd.RASTER_BINNED, "direct": piko_amd.RASTER_DIRECT, "tiled": piko_amd.RASTER_TILED}
 
 
 def log(msg):

@@ -1,7 +1,8 @@
 """GPU parity: the HIP raster pipeline against the oracle, bit for bit
 (colour words, depth bits and the per-bin queues), through the C-ABI.  Every
-frame is checked in both raster modes: binned (queues sorted by bin, one wave
-per bin) and direct (per-pixel visibility words, no queues)."""
+frame is checked in every raster mode: binned (queues sorted by bin, one wave
+per bin), direct (per-pixel visibility words, no queues) and tiled (per-chunk
+tile lists, LDS visibility per tile)."""
 import sys
 
 import numpy as np
@@ -17,12 +18,12 @@ def log(msg):
     sys.stdout.flush()
 
 
-MODES = ("binned", "direct")
+MODES = ("binned", "direct", "tiled")
 
 
-def gpu_frame(scene, T, S, pair_capacity=0, mode="direct"):
+def gpu_frame(scene, T, S, pair_capacity=0, mode="tiled"):
     import piko_amd
-    rm = piko_amd.RASTER_BINNED if mode == "binned" else piko_amd.RASTER_DIRECT
+    rm = {"binned": piko_amd.RASTER_BINNED, "direct": piko_amd.RASTER_DIRECT, "tiled": piko_amd.RASTER_TILED}[mode]
     with piko_amd.RasterPipe().allocate(scene, T, S, pair_capacity, raster_mode=rm) as pipe:
         pipe.prepare()
         pipe.run_single()
@@ -141,6 +142,20 @@ def test_direct_large_list_grows(oracle_mod):
         a = pipe.pikoScreen.getData().copy()
         pipe.run_single()
         assert np.array_equal(a, pipe.pikoScreen.getData())
+
+
+def test_tiled_large_lists(oracle_mod):
+    # every triangle spans more than 2x2 tiles of 16 pixels: all of them go to
+    # the chunks' large lists and are clipped to each tile they touch
+    sc = sized_tris(13, 9000, 512, 384, 45, 90)
+    assert_frame_equal(sc, 16, 8, oracle_mod, check_bins=False, modes=("tiled",))
+
+
+def test_tiled_tile_size_grows_with_screen(oracle_mod):
+    # 2000x1200 at bin size 8 or 16 would be more than 8192 tiles: the tiled
+    # path rasterises it in 32-pixel tiles, with the same image
+    sc = scenes.tris_uniform(14, 20000, 2000, 1200)
+    assert_frame_equal(sc, 8, 8, oracle_mod, check_bins=False, modes=("tiled",))
 
 
 def test_queue_capacity_grows(oracle_mod):
