@@ -112,11 +112,14 @@ def bench_batch(log, reps=5):
     ntris = sum(s.n for s in scs)
     npix = sum(s.W * s.H for s in scs)
     log("batch: %d scenes, %d triangles, %d pixels; autotuning" % (len(scs), ntris, npix))
-    per, table = autotune.autotune(scs, log=None)
+    # plain pipes: the fastest mode and bin size per class; the batch (one
+    # tiled launch sequence): the fastest tile size per class
+    per, table = autotune.autotune(scs, log=None, modes=(piko_amd.RASTER_DIRECT, piko_amd.RASTER_TILED))
     pipes = []
     for s, cfg in zip(scs, per):
         pipes.append(piko_amd.RasterPipe().allocate(s, cfg["bin_size"], 8, waves_per_cu=cfg["waves_per_cu"],
-                                                    pair_capacity=int(cfg["queue_depth"] * s.n) + 64))
+                                                    pair_capacity=int(cfg["queue_depth"] * s.n) + 64,
+                                                    raster_mode=cfg["raster_mode"]))
     for p in pipes:
         p.run_frames(1)
     t0 = time.perf_counter()
@@ -125,9 +128,10 @@ def bench_batch(log, reps=5):
             p.run_single()
     t_seq = (time.perf_counter() - t0) / reps
     pos, attr, parts = piko_amd.concat_scenes(scs)
-    bp = piko_amd.BatchPipe(pos, attr, "pretransformed", max_jobs=len(scs), max_tris=ntris, max_pixels=npix)
-    bp.submit([(n, base, cfg["bin_size"], piko_amd.ConstantState.for_scene(s), cfg["waves_per_cu"],
-                cfg["queue_depth"]) for (n, base, s), cfg in zip(parts, per)])
+    bp = piko_amd.BatchPipe(pos, attr, "pretransformed", max_jobs=len(scs), max_tris=ntris, max_pixels=npix,
+                            raster_mode=piko_amd.RASTER_TILED)
+    bp.submit([(n, base, cfg["best_bin_size"][piko_amd.RASTER_TILED], piko_amd.ConstantState.for_scene(s),
+                cfg["waves_per_cu"], cfg["queue_depth"]) for (n, base, s), cfg in zip(parts, per)])
     bp.wait()
     bp.run_frames(1)
     t0 = time.perf_counter()
@@ -140,11 +144,13 @@ def bench_batch(log, reps=5):
     for p in pipes:
         p.destroy()
     out = {"workload": "256 uniform-soup scenes, seeds 1000-1255, N log-uniform [1K,500K], "
-                       "pixels log-uniform [256^2,2048^2], aspect {1,4:3,16:9}, autotuned per class",
+                       "pixels log-uniform [256^2,2048^2], aspect {1,4:3,16:9}, autotuned per class (plain: mode and bin size; batch: tile size)",
            "scenes": len(scs), "triangles": ntris, "pixels": npix, "classes": len(table),
            "seq_ms_per_batch": round(t_seq * 1e3, 3), "batch_ms": round(t_batch * 1e3, 3),
            "seq_frames_per_s": round(len(scs) / t_seq, 1), "batch_frames_per_s": round(len(scs) / t_batch, 1),
            "speedup": round(t_seq / t_batch, 2), "batch_Mtri_per_s": round(ntris / t_batch / 1e6, 1),
+           "plain_modes": {"direct": sum(1 for c in per if c["raster_mode"] == piko_amd.RASTER_DIRECT),
+                           "tiled": sum(1 for c in per if c["raster_mode"] == piko_amd.RASTER_TILED)},
            "bit_exact_vs_sequential": exact, "scenes_compared": len(checked)}
     log("batch: %s" % out)
     return out

@@ -77,8 +77,8 @@ enum { PIKO_INPUT_PRETRANSFORMED = 0, PIKO_INPUT_WORLD = 1 };
  *           fragment in LDS and writes each pixel once.  No global atomics.
  *   DIRECT and TILED keep no sorted queues: the bin queues are built on
  *   demand when asked for (piko_raster_get_bins).
- *   AUTO:   TILED.  The environment variable PIKO_RASTER_MODE=binned|direct|
- *           tiled overrides AUTO. */
+ *   AUTO:   DIRECT for a plain pipe, TILED for a batch.  The environment
+ *           variable PIKO_RASTER_MODE=binned|direct|tiled overrides AUTO. */
 enum { PIKO_RASTER_AUTO = 0, PIKO_RASTER_BINNED = 1, PIKO_RASTER_DIRECT = 2, PIKO_RASTER_TILED = 3 };
 
 typedef struct {

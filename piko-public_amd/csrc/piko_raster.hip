@@ -83,7 +83,7 @@ struct FrameState {
   uint32_t overflow;     // sticky: bit 0 pairs exceeded the queues, bit 1 the large-triangle list,
                          // bit 2 the large triangles' 8x8 tiles exceeded 2^32 (cannot grow)
   uint32_t large_peak;   // most large triangles of one frame since the host last looked
-  uint32_t pad;
+  uint32_t tiles_peak;   // most 8x8 tiles of large triangles of one frame since then
   unsigned long long large_acc;  // direct path, this frame: large triangles listed (low
                                  // half) and their 8x8 tiles (high half)
 };
@@ -107,6 +107,19 @@ __device__ uint32_t block_scan_inplace(uint32_t *__restrict__ v, int n, Addr add
     run += x;
   }
   return tot;
+}
+
+// Setup record of an emitted triangle, 48 bytes (3 x int4), written by the
+// vertex stage and gathered by the raster and resolve kernels:
+//   [0] x0 y0 x1 y1 (fixed point)
+//   [1] x2 y2, 1/barydenom (raster.pikostage:174), z2
+//   [2] z0 - z2, z1 - z2
+// Colour needs the first 32 bytes only.
+__device__ __forceinline__ void put_setup(int4 *__restrict__ geo, size_t i, const int p[6], float rcp,
+                                          const float z[3]) {
+  geo[3 * i + 0] = make_int4(p[0], p[1], p[2], p[3]);
+  geo[3 * i + 1] = make_int4(p[4], p[5], __float_as_int(rcp), __float_as_int(z[2]));
+  geo[3 * i + 2] = make_int4(__float_as_int(z[0] - z[2]), __float_as_int(z[1] - z[2]), 0, 0);
 }
 
 // ---- vertex stage + bin count ---------------------------------------------
@@ -188,10 +201,7 @@ __global__ void __launch_bounds__(kVsThreads) k_vs_count(const float *__restrict
       Edges e;
       e.set(p);
       const float rcp = 1.0f / (float)e.denom();  // raster.pikostage:174
-      const float z0mz2 = z[0] - z[2], z1mz2 = z[1] - z[2];
-      geo[3 * (size_t)i + 0] = make_int4(p[0], p[1], p[2], p[3]);
-      geo[3 * (size_t)i + 1] = make_int4(p[4], p[5], __float_as_int(z[2]), __float_as_int(z0mz2));
-      geo[3 * (size_t)i + 2] = make_int4(__float_as_int(z1mz2), __float_as_int(rcp), 0, 0);
+      put_setup(geo, (size_t)i, p, rcp, z);
       colr[i] = make_uint4(c[0], c[1], c[2], 0u);
       int bx1, by1, bx2, by2;
       bin_rect(p, bsb, ja.nBx, ja.nBy, bx1, by1, bx2, by2);
@@ -507,7 +517,7 @@ __device__ __forceinline__ uint32_t gouraud(const float *u8, uint32_t c0, uint32
 
 // Setup record of one triangle as the raster kernel consumes it.
 struct TriRec {
-  int4 a, b, c;  // x0 y0 x1 y1 | x2 y2 z2 z0mz2 | z1mz2 rcp - -
+  int4 a, b, c;  // the setup record (put_setup)
   uint4 col;     // c0 c1 c2 -
 };
 
@@ -538,10 +548,10 @@ __device__ __forceinline__ uint32_t stage_tri(RasterBatch &bt, int t, bool valid
   bt.rect[t] = (uint32_t)(x0 & 0xfff) | ((uint32_t)(y0 & 0xfff) << 12) | ((uint32_t)w << 24);
 #pragma unroll
   for (int k = 0; k < 6; ++k) bt.xy[k][t] = p[k];
-  bt.zr[0][t] = __int_as_float(r.b.z);
-  bt.zr[1][t] = __int_as_float(r.b.w);
-  bt.zr[2][t] = __int_as_float(r.c.x);
-  bt.zr[3][t] = __int_as_float(r.c.y);
+  bt.zr[0][t] = __int_as_float(r.b.w);
+  bt.zr[1][t] = __int_as_float(r.c.x);
+  bt.zr[2][t] = __int_as_float(r.c.y);
+  bt.zr[3][t] = __int_as_float(r.b.z);
   bt.col[0][t] = r.col.x;
   bt.col[1][t] = r.col.y;
   bt.col[2][t] = r.col.z;
@@ -683,7 +693,7 @@ __global__ void __launch_bounds__(kRT) k_raster(const int4 *__restrict__ geo, co
         } else {
           const TriRec r = load_tri(geo, colr, bin_tris[b0 + pos]);
           xy[0] = r.a.x; xy[1] = r.a.y; xy[2] = r.a.z; xy[3] = r.a.w; xy[4] = r.b.x; xy[5] = r.b.y;
-          rcp = __int_as_float(r.c.y);
+          rcp = __int_as_float(r.b.z);
           c0 = r.col.x;
           c1 = r.col.y;
           c2 = r.col.z;
@@ -760,28 +770,25 @@ constexpr int kLargeTile = 8;
 // One staged triangle (64 bytes, read with 16-byte LDS loads).  Edge function
 // k at the centre of pixel (x0 + i, y0 + j) is e[k] + i*dx[k] + j*dy[k], exact
 // in 32-bit arithmetic for triangles of the int32 path.
-struct alignas(16) DirTri {
-  uint32_t x0y0;   // x0 | y0 << 16
-  uint32_t wh;     // bbox width | height << 16
-  uint32_t tri;    // submission index
-  float rcp;       // 1 / barydenom (raster.pikostage:174)
-  int e[3];
-  float z2;
-  int dx[3];
-  float z0mz2;
-  int dy[3];
-  float z1mz2;
-};
-
+// One wave's staged triangles, as structure of arrays (lanes reading
+// neighbouring triangles hit neighbouring LDS banks).  Edge function k at the
+// centre of pixel (x0 + i, y0 + j) is e[k] + i*dx[k] + j*dy[k], exact in
+// 32-bit arithmetic for triangles of the int32 path.
 template <bool MULTI>
-struct DirStage {  // one wave's triangles
+struct DirStage {
   uint32_t pre[64];  // exclusive prefix of the triangles' span counts
-  DirTri t[64];
+  uint4 r0[64];      // x0 | y0 << 16, w | h << 16, submission index, 1/barydenom
+  int4 re[64];       // e[0..2] at (x0, y0), z2 (float bits)
+  int4 rx[64];       // dx[0..2], z0 - z2
+  int4 ry[64];       // dy[0..2], z1 - z2
 };
 template <>
 struct DirStage<true> {
   uint32_t pre[64];
-  DirTri t[64];
+  uint4 r0[64];
+  int4 re[64];
+  int4 rx[64];
+  int4 ry[64];
   unsigned long long fb[64];  // framebuffer base of the triangle's job
   uint32_t fbw[64];           // framebuffer width of the triangle's job
 };
@@ -811,14 +818,14 @@ __device__ __forceinline__ int job_by_pixel(const FrameArgs *__restrict__ jobs, 
 }
 
 template <int KIND, bool MULTI>
-__global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restrict__ pos, const void *__restrict__ attr,
-                                                            int n, int S, FrameArgs fa,
+__global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restrict__ pos, int n, int S, FrameArgs fa,
                                                             const FrameArgs *__restrict__ jobs, int njobs,
-                                                            int4 *__restrict__ geo, uint4 *__restrict__ colr,
+                                                            int4 *__restrict__ geo,
                                                             unsigned long long *__restrict__ vis,
                                                             uint32_t *__restrict__ large,
                                                             uint32_t *__restrict__ tile_off, FrameState *__restrict__ fs,
-                                                            uint32_t large_cap, uint32_t *__restrict__ blk_keep) {
+                                                            uint32_t large_cap, uint32_t *__restrict__ blk_keep,
+                                                            int inline_large) {
   __shared__ DirStage<MULTI> st[kDirThreads / 64];
   __shared__ uint32_t s_keep[kDirThreads / 64];
   const int t = threadIdx.x, lane = t & 63;
@@ -826,6 +833,12 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
   const int i = blockIdx.x * kDirThreads + t;
   uint32_t units = 0, tiles = 0;
   bool big = false, kept = false;
+  // a large triangle's setup, kept for the wave that rasterises it inline
+  int lp[6] = {0, 0, 0, 0, 0, 0};
+  int lrect = 0, lwh = 0;
+  float lrcp = 0.0f, lz[3] = {0.0f, 0.0f, 0.0f};
+  unsigned long long lfb = 0;
+  int lW = 0;
   if (i < n) {
     // vertex positions only: the winner's colours are computed in k_resolve
     const FrameArgs &ja = MULTI ? jobs[job_search(jobs, njobs, (uint32_t)i, false)] : fa;
@@ -838,10 +851,7 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
       Edges e;
       e.set(p);
       const float rcp = 1.0f / (float)e.denom();  // raster.pikostage:174
-      const float z0mz2 = z[0] - z[2], z1mz2 = z[1] - z[2];
-      geo[3 * (size_t)i + 0] = make_int4(p[0], p[1], p[2], p[3]);
-      geo[3 * (size_t)i + 1] = make_int4(p[4], p[5], __float_as_int(z[2]), __float_as_int(z0mz2));
-      geo[3 * (size_t)i + 2] = make_int4(__float_as_int(z1mz2), __float_as_int(rcp), 0, 0);
+      put_setup(geo, (size_t)i, p, rcp, z);
       int x0, y0, x1, y1;
       clip_rect(p, ja.S, ja.W, ja.H, x0, y0, x1, y1);
       const int w = max(0, x1 - x0), h = max(0, y1 - y0);
@@ -849,29 +859,36 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
       if (npx > (uint32_t)kDirectMaxPx || !small_bbox(p, ja.S)) {
         big = npx > 0;
         tiles = (uint32_t)((w + kLargeTile - 1) / kLargeTile) * (uint32_t)((h + kLargeTile - 1) / kLargeTile);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) lp[k] = p[k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) lz[k] = z[k];
+        lrcp = rcp;
+        lrect = x0 | (y0 << 16);
+        lwh = w | (h << 16);
+        lfb = MULTI ? ja.fb_base : 0ull;
+        lW = ja.W;
       } else if (npx > 0) {
         units = (uint32_t)w * ((uint32_t)(h + kSpan - 1) / kSpan);
-        DirTri &d = s.t[lane];
-        d.x0y0 = (uint32_t)x0 | ((uint32_t)y0 << 16);
-        d.wh = (uint32_t)w | ((uint32_t)h << 16);
-        d.tri = (uint32_t)i;
-        d.rcp = rcp;
         // edge functions at the centre of pixel (x0, y0) and their steps;
         // unsigned arithmetic, equal to the int32 evaluation for int32-path
         // triangles (no intermediate leaves the int32 range)
         const uint32_t half = 1u << (S - 1);
         const uint32_t cx = ((uint32_t)x0 << S) + half, cy = ((uint32_t)y0 << S) + half;
+        uint32_t ee[3], dx[3], dy[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
           const uint32_t xa = (uint32_t)p[2 * k], ya = (uint32_t)p[2 * k + 1];
           const uint32_t xb = (uint32_t)p[(2 * k + 2) % 6], yb = (uint32_t)p[(2 * k + 3) % 6];
-          d.e[k] = (int)((cy - ya) * (xb - xa) - (cx - xa) * (yb - ya));
-          d.dx[k] = (int)((0u - (yb - ya)) << S);
-          d.dy[k] = (int)((xb - xa) << S);
+          ee[k] = (cy - ya) * (xb - xa) - (cx - xa) * (yb - ya);
+          dx[k] = (0u - (yb - ya)) << S;
+          dy[k] = (xb - xa) << S;
         }
-        d.z2 = z[2];
-        d.z0mz2 = z0mz2;
-        d.z1mz2 = z1mz2;
+        s.r0[lane] = make_uint4((uint32_t)x0 | ((uint32_t)y0 << 16), (uint32_t)w | ((uint32_t)h << 16), (uint32_t)i,
+                                __float_as_uint(rcp));
+        s.re[lane] = make_int4((int)ee[0], (int)ee[1], (int)ee[2], __float_as_int(z[2]));
+        s.rx[lane] = make_int4((int)dx[0], (int)dx[1], (int)dx[2], __float_as_int(z[0] - z[2]));
+        s.ry[lane] = make_int4((int)dy[0], (int)dy[1], (int)dy[2], __float_as_int(z[1] - z[2]));
         if constexpr (MULTI) {
           s.fb[lane] = ja.fb_base;
           s.fbw[lane] = (uint32_t)ja.W;
@@ -883,7 +900,9 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
   if (lane == 0) s_keep[t >> 6] = (uint32_t)__popcll(km);
   // large triangles: one append per wave.  A single 64-bit atomic reserves
   // list slots (low half) and 8x8 tile numbers (high half) together, so the
-  // first tile of the listed triangles increases with the slot.
+  // first tile of the listed triangles increases with the slot.  With
+  // inline_large (the last frame had none) the counts are still added, for
+  // the host's next launch decision, and the wave draws them itself.
   const unsigned long long bm = __ballot(big);
   if (bm) {
     const int first = __ffsll((long long)bm) - 1;
@@ -896,17 +915,46 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
           atomicAdd(&fs->large_acc, ((unsigned long long)tsum << 32) | (unsigned long long)__popcll(bm));
       bt = (uint32_t)old;
       btile = (uint32_t)(old >> 32);
-      if ((unsigned long long)btile + tsum > 0xFFFFFFFFull) atomicOr(&fs->overflow, 4u);
+      if (!inline_large && (unsigned long long)btile + tsum > 0xFFFFFFFFull) atomicOr(&fs->overflow, 4u);
     }
-    bt = __shfl(bt, first, 64);
-    btile = __shfl(btile, first, 64);
-    if (big) {
-      const uint32_t at = bt + (uint32_t)__popcll(bm & ((1ull << lane) - 1ull));
-      if (at < large_cap) {
-        large[at] = (uint32_t)i;
-        tile_off[at] = btile + tinc - tl;
-      } else {
-        atomicOr(&fs->overflow, 2u);
+    if (inline_large) {
+      for (unsigned long long m = bm; m; m &= m - 1) {
+        const int L = __ffsll((long long)m) - 1;
+        int xy[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) xy[k] = __shfl(lp[k], L, 64);
+        const float rcp = __shfl(lrcp, L, 64);
+        const float z2 = __shfl(lz[2], L, 64);
+        const float z0mz2 = __shfl(lz[0], L, 64) - z2, z1mz2 = __shfl(lz[1], L, 64) - z2;
+        const int rect = __shfl(lrect, L, 64), wh = __shfl(lwh, L, 64);
+        const int W = __shfl(lW, L, 64);
+        const unsigned long long fbb = __shfl(lfb, L, 64);
+        const uint32_t tri = (uint32_t)(blockIdx.x * kDirThreads + (t & ~63) + L);
+        const int x0 = rect & 0xffff, y0 = rect >> 16, w = wh & 0xffff, h = wh >> 16;
+        const bool sm = small_bbox(xy, S);
+        const int tw = (w + kLargeTile - 1) / kLargeTile, th = (h + kLargeTile - 1) / kLargeTile;
+        for (int g = 0; g < tw * th; ++g) {
+          const int ty = g / tw, tx = g - ty * tw;
+          const int px = x0 + tx * kLargeTile + (lane & (kLargeTile - 1));
+          const int py = y0 + ty * kLargeTile + lane / kLargeTile;
+          float alpha, beta;
+          if (px < x0 + w && py < y0 + h && covered_at(xy, sm, px, py, S, rcp, alpha, beta)) {
+            const float z = interp(z0mz2, z1mz2, z2, alpha, beta);
+            atomicMin((long long *)&vis[fbb + (size_t)py * W + px], (long long)vis_key(z, tri));
+          }
+        }
+      }
+    } else {
+      bt = __shfl(bt, first, 64);
+      btile = __shfl(btile, first, 64);
+      if (big) {
+        const uint32_t at = bt + (uint32_t)__popcll(bm & ((1ull << lane) - 1ull));
+        if (at < large_cap) {
+          large[at] = (uint32_t)i;
+          tile_off[at] = btile + tinc - tl;
+        } else {
+          atomicOr(&fs->overflow, 2u);
+        }
       }
     }
   }
@@ -925,11 +973,10 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
 #pragma unroll
     for (int step = 32; step > 0; step >>= 1)
       if (s.pre[o + step] <= cidx) o += step;
-    const DirTri &d = s.t[o];
-    const uint4 r0 = *(const uint4 *)&d.x0y0;
-    const int4 re = *(const int4 *)&d.e[0];
-    const int4 rx = *(const int4 *)&d.dx[0];
-    const int4 ry = *(const int4 *)&d.dy[0];
+    const uint4 r0 = s.r0[o];
+    const int4 re = s.re[o];
+    const int4 rx = s.rx[o];
+    const int4 ry = s.ry[o];
     const uint32_t k = cidx - s.pre[o];
     const uint32_t w = r0.y & 0xffffu, h = r0.y >> 16;
     // band and column of unit k: (k + 0.5) / w is at least 0.5 / w from an
@@ -1002,8 +1049,8 @@ __global__ void __launch_bounds__(256) k_large_raster(const int4 *__restrict__ g
     const int px = x0 + tx * kLargeTile + (lane & (kLargeTile - 1)), py = y0 + ty * kLargeTile + lane / kLargeTile;
     if (px >= x1 || py >= y1) continue;
     float alpha, beta;
-    if (covered_at(p, small_bbox(p, ja.S), px, py, ja.S, __int_as_float(c.y), alpha, beta)) {
-      const float z = interp(__int_as_float(b.w), __int_as_float(c.x), __int_as_float(b.z), alpha, beta);
+    if (covered_at(p, small_bbox(p, ja.S), px, py, ja.S, __int_as_float(b.z), alpha, beta)) {
+      const float z = interp(__int_as_float(c.x), __int_as_float(c.y), __int_as_float(b.w), alpha, beta);
       const unsigned long long fbb = MULTI ? ja.fb_base : 0ull;
       atomicMin((long long *)&vis[fbb + (size_t)py * ja.W + px], (long long)vis_key(z, tri));
     }
@@ -1025,6 +1072,7 @@ __global__ void __launch_bounds__(256) k_resolve(const int4 *__restrict__ geo, c
   __syncthreads();
   if (blockIdx.x == 0 && t == 0) {  // the large list of this frame is consumed
     fs->large_peak = max(fs->large_peak, (uint32_t)fs->large_acc);
+    fs->tiles_peak = max(fs->tiles_peak, (uint32_t)(fs->large_acc >> 32));
     fs->large_acc = 0;
   }
   const long long g = (long long)blockIdx.x * blockDim.x + t;
@@ -1037,13 +1085,13 @@ __global__ void __launch_bounds__(256) k_resolve(const int4 *__restrict__ geo, c
     const long long local = MULTI ? g - (long long)ja.fb_base : g;
     const int py = (int)(local / ja.W), px = (int)(local - (long long)py * ja.W);
     const uint32_t tri = 0xFFFFFFFEu - (uint32_t)(key & 0xFFFFFFFFull);
-    const int4 a = geo[3 * (size_t)tri], b = geo[3 * (size_t)tri + 1], c = geo[3 * (size_t)tri + 2];
+    const int4 a = geo[3 * (size_t)tri], b = geo[3 * (size_t)tri + 1];
     const size_t src = MULTI ? (size_t)ja.src_base + (size_t)(tri - ja.vbase) : (size_t)tri;
     uint32_t cc[3];
     tri_colours<KIND>(attr, src, ja, cc);
     const int xy[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
     float alpha = 0.0f, beta = 0.0f;
-    covered_at(xy, small_bbox(xy, ja.S), px, py, ja.S, __int_as_float(c.y), alpha, beta);
+    covered_at(xy, small_bbox(xy, ja.S), px, py, ja.S, __int_as_float(b.z), alpha, beta);
     col = gouraud(s_u8, cc[0], cc[1], cc[2], alpha, beta);
   }
   depth[g] = __int_as_float((int)(key >> 32));
@@ -1185,13 +1233,9 @@ __global__ void __launch_bounds__(kBinThreads) k_tile_bin(const float *__restric
       ra[k] = 2u;
     }
     // the setup record k_tile_raster reads for each listed triangle
-    const size_t gi = 3 * (size_t)(cd.tri0 + il);
     Edges e;
     e.set(p);
-    const float rcp = 1.0f / (float)e.denom();  // raster.pikostage:174
-    geo[gi + 0] = make_int4(p[0], p[1], p[2], p[3]);
-    geo[gi + 1] = make_int4(p[4], p[5], __float_as_int(z[2]), __float_as_int(z[0] - z[2]));
-    geo[gi + 2] = make_int4(__float_as_int(z[1] - z[2]), __float_as_int(rcp), 0, 0);
+    put_setup(geo, (size_t)(cd.tri0 + il), p, 1.0f / (float)e.denom(), z);  // raster.pikostage:174
   }
   __syncthreads();
   {  // exclusive scan of cnt[0..ntiles]; cnt[ntiles] becomes the chunk's pair total
@@ -1403,8 +1447,8 @@ __global__ void __launch_bounds__(kTRThreads) k_tile_raster(const int4 *__restri
           y1 = min(y1, yhi);
           const int w = max(0, x1 - x0), h = max(0, y1 - y0);
           if (w > 0 && h > 0) {
-            const float rcp = __int_as_float(gc[2].y);
-            const float z2 = __int_as_float(gc[1].z), z0mz2 = __int_as_float(gc[1].w), z1mz2 = __int_as_float(gc[2].x);
+            const float rcp = __int_as_float(gc[1].z);
+            const float z2 = __int_as_float(gc[1].w), z0mz2 = __int_as_float(gc[2].x), z1mz2 = __int_as_float(gc[2].y);
             const bool small = small_bbox(p, S);
             if (small && w * h <= kLanePx) {
               // a few pixels: this lane tests them itself (raster.pikostage:223-241)
@@ -1439,7 +1483,7 @@ __global__ void __launch_bounds__(kTRThreads) k_tile_raster(const int4 *__restri
             } else {
               units = (uint32_t)w * (uint32_t)((h + kSpan - 1) / kSpan);
               s.xy01[lane] = make_int4(p[0], p[1], p[2], p[3]);
-              s.xy2z[lane] = make_int4(p[4], p[5], gc[1].z, gc[1].w);
+              s.xy2z[lane] = make_int4(p[4], p[5], gc[1].w, gc[2].x);
               s.misc[lane] = make_uint4(__float_as_uint(z1mz2), __float_as_uint(rcp),
                                         (uint32_t)x0 | ((uint32_t)y0 << 16),
                                         (uint32_t)w | ((uint32_t)h << 16) | (small ? 0x80000000u : 0u));
@@ -1478,12 +1522,12 @@ __global__ void __launch_bounds__(kTRThreads) k_tile_raster(const int4 *__restri
     if (key != kVisSentinel) {
       const uint32_t v = 0xFFFFFFFEu - (uint32_t)(key & 0xFFFFFFFFull);
       const size_t src = MULTI ? (size_t)ja.src_base + (size_t)(v - ja.vbase) : (size_t)v;
-      const int4 a = geo[3 * (size_t)v], b = geo[3 * (size_t)v + 1], cg = geo[3 * (size_t)v + 2];
+      const int4 a = geo[3 * (size_t)v], b = geo[3 * (size_t)v + 1];
       const int p[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
       uint32_t c[3];
       tri_colours<KIND>(attr, src, ja, c);
       float alpha = 0.0f, beta = 0.0f;
-      covered_at(p, small_bbox(p, S), px, py, S, __int_as_float(cg.y), alpha, beta);
+      covered_at(p, small_bbox(p, S), px, py, S, __int_as_float(b.z), alpha, beta);
       col = gouraud(s_u8, c[0], c[1], c[2], alpha, beta);
     }
     const size_t pid = fbb + (size_t)py * ja.W + px;
@@ -1573,6 +1617,8 @@ struct piko_raster_pipe {
   int raster_waves = 0;       // persistent raster grid (default)
   int cus = 256;              // compute units of the device
   int raster_persistent = 1;  // 1: persistent pipelined waves; 0: one block per bin
+  long long large_tiles_hint = -1;  // 8x8 tiles of large triangles last seen (-1: unknown)
+  bool last_inline = false;         // the last direct frame drew large triangles in the vertex stage
   long long device_bytes = 0;
   long long pair_bytes = 0, tri_bytes = 0, pix_bytes = 0, bin_bytes = 0;
 };
@@ -1781,7 +1827,10 @@ int radix_passes(int nbins) {
 // raster once per bin-size class).
 const char *kernel_name(const piko_raster_pipe *p, int k) {
   static const char *direct[] = {"k_vs_direct", "k_large_raster", "k_resolve"};
-  if (p->last_mode == PIKO_RASTER_DIRECT) return k < 3 ? direct[k] : "";
+  if (p->last_mode == PIKO_RASTER_DIRECT) {
+    if (p->last_inline) return k == 0 ? direct[0] : (k == 1 ? direct[2] : "");
+    return k < 3 ? direct[k] : "";
+  }
   if (p->last_mode == PIKO_RASTER_TILED) return k == 0 ? "k_tile_bin" : "k_tile_raster";
   static const char *head[] = {"k_vs_count", "k_scan_tiles", "k_emit_pairs"};
   static const char *pass[] = {"k_sort_count", "k_sort_rows", "k_sort_scatter"};
@@ -1875,11 +1924,14 @@ void enqueue_direct(piko_raster_pipe *p, Marks &mark) {
   const int n = p->ntris;
   const int nj = (int)p->jobs.size();
   const int blocks = (int)std::max<long long>(cdiv(n, kDirThreads), 1);
-#define PIKO_VSD_LAUNCH(KIND, MULTI)                                                                              \
-  k_vs_direct<KIND, MULTI><<<blocks, kDirThreads, 0, p->stream>>>(p->d_pos, p->d_attr, n, p->S, p->fa, p->d_jobs, \
-                                                                  nj, p->d_geo, p->d_col, p->d_vis, p->d_large,   \
-                                                                  p->d_tile_off, p->d_fs, p->large_cap,           \
-                                                                  p->d_keep_dir)
+  // the last frame looked at had no large triangle: draw any in the vertex
+  // stage and skip the large-triangle launch
+  const int inl = p->large_tiles_hint == 0 ? 1 : 0;
+  p->last_inline = inl != 0;
+#define PIKO_VSD_LAUNCH(KIND, MULTI)                                                                           \
+  k_vs_direct<KIND, MULTI><<<blocks, kDirThreads, 0, p->stream>>>(p->d_pos, n, p->S, p->fa, p->d_jobs, nj,     \
+                                                                  p->d_geo, p->d_vis, p->d_large, p->d_tile_off, \
+                                                                  p->d_fs, p->large_cap, p->d_keep_dir, inl)
   if (p->kind == PIKO_INPUT_PRETRANSFORMED) {
     if (p->multi) PIKO_VSD_LAUNCH(PIKO_INPUT_PRETRANSFORMED, true);
     else PIKO_VSD_LAUNCH(PIKO_INPUT_PRETRANSFORMED, false);
@@ -1889,7 +1941,14 @@ void enqueue_direct(piko_raster_pipe *p, Marks &mark) {
   }
 #undef PIKO_VSD_LAUNCH
   mark();
-  const int lgrid = std::max(p->cus, 1) * 2;
+  if (!inl) {
+  // grid-stride over the frame's large-triangle tiles: sized from the last
+  // frame that was looked at (4 tiles per block in flight), the full grid
+  // when unknown
+  const int full = std::max(p->cus, 1) * 2;
+  const int lgrid = p->large_tiles_hint < 0
+                        ? full
+                        : (int)std::min<long long>(std::max<long long>(cdiv(p->large_tiles_hint, 4), 1), full);
   if (p->multi)
     k_large_raster<true><<<lgrid, 256, 0, p->stream>>>(p->d_geo, p->d_large, p->d_tile_off, p->d_fs, p->large_cap,
                                                        p->fa, p->d_jobs, nj, p->d_vis);
@@ -1897,6 +1956,7 @@ void enqueue_direct(piko_raster_pipe *p, Marks &mark) {
     k_large_raster<false><<<lgrid, 256, 0, p->stream>>>(p->d_geo, p->d_large, p->d_tile_off, p->d_fs, p->large_cap,
                                                         p->fa, p->d_jobs, nj, p->d_vis);
   mark();
+  }
   const long long rblocks = std::max<long long>(cdiv(p->npix, 256), 1);
 #define PIKO_RES_LAUNCH(KIND, MULTI)                                                                          \
   k_resolve<KIND, MULTI><<<(unsigned)rblocks, 256, 0, p->stream>>>(p->d_geo, p->d_attr, p->d_vis, p->fa, p->d_jobs, \
@@ -2017,10 +2077,11 @@ int check_capacity(piko_raster_pipe *p, int *rerun) {
     if ((rc = dalloc(p, &p->d_large, p->large_cap, &p->tri_bytes)) != PIKO_OK) return rc;
     if ((rc = dalloc(p, &p->d_tile_off, p->large_cap, &p->tri_bytes)) != PIKO_OK) return rc;
   }
-  if (fs.overflow || fs.large_peak) {
-    const uint32_t zero[2] = {0, 0};
+  if (p->last_mode == PIKO_RASTER_DIRECT) p->large_tiles_hint = fs.tiles_peak;
+  if (fs.overflow || fs.large_peak || fs.tiles_peak) {
+    const uint32_t zero[3] = {0, 0, 0};
     HIPOK(hipMemcpyAsync(&p->d_fs->overflow, &zero[0], 4, hipMemcpyHostToDevice, p->stream));
-    HIPOK(hipMemcpyAsync(&p->d_fs->large_peak, &zero[1], 4, hipMemcpyHostToDevice, p->stream));
+    HIPOK(hipMemcpyAsync(&p->d_fs->large_peak, &zero[1], 8, hipMemcpyHostToDevice, p->stream));
     HIPOK(hipStreamSynchronize(p->stream));
   }
   *rerun = pairs_short || large_short;
@@ -2086,8 +2147,10 @@ int create_common(piko_raster_pipe *p, int kind, int S, long long tris, long lon
   p->kind = kind;
   p->S = S;
   if (mode == PIKO_RASTER_AUTO) {
+    // plain frames: direct (fewest passes over one scene); batches: tiled
+    // (many fragments per pixel across uneven screens stay in LDS)
     const char *m = std::getenv("PIKO_RASTER_MODE");
-    mode = PIKO_RASTER_TILED;
+    mode = p->multi ? PIKO_RASTER_TILED : PIKO_RASTER_DIRECT;
     if (m && std::strcmp(m, "binned") == 0) mode = PIKO_RASTER_BINNED;
     if (m && std::strcmp(m, "direct") == 0) mode = PIKO_RASTER_DIRECT;
   }

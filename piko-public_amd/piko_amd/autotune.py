@@ -44,26 +44,33 @@ def time_plain(scene, T, waves, S=8, frames=4, raster_mode=RASTER_AUTO):
     return dt, st["pairs"] / max(scene.n, 1)
 
 
-def tune(scene, bin_sizes=None, waves=None, S=8, raster_mode=RASTER_AUTO):
-    cb, cw = candidates(raster_mode)
-    bin_sizes = bin_sizes or cb
-    waves = waves or cw
+def tune(scene, bin_sizes=None, waves=None, S=8, raster_mode=RASTER_AUTO, modes=None):
+    """Times every candidate of `raster_mode` (or of each mode in `modes`) on
+    `scene`.  Returns the fastest setting overall ("raster_mode", "bin_size",
+    "waves_per_cu", "queue_depth", "ms") and, per mode, its fastest bin size
+    ("best_bin_size": {mode: T})."""
     best = None
+    per_mode = {}
     trials = []
-    for T in bin_sizes:
-        for w in waves:
-            dt, qd = time_plain(scene, T, w, S, raster_mode=raster_mode)
-            trials.append((T, w, dt))
-            if best is None or dt < best[2]:
-                best = (T, w, dt, qd)
-    T, w, dt, qd = best
-    return {"bin_size": T, "waves_per_cu": w, "queue_depth": round(1.25 * qd + 0.05, 3), "ms": dt * 1e3,
-            "trials": trials}
+    for mode in (modes or (raster_mode,)):
+        cb, cw = candidates(mode)
+        for T in bin_sizes or cb:
+            for w in waves or cw:
+                dt, qd = time_plain(scene, T, w, S, raster_mode=mode)
+                trials.append((mode, T, w, dt))
+                if mode not in per_mode or dt < per_mode[mode][1]:
+                    per_mode[mode] = (T, dt)
+                if best is None or dt < best[3]:
+                    best = (mode, T, w, dt, qd)
+    mode, T, w, dt, qd = best
+    return {"raster_mode": mode, "bin_size": T, "waves_per_cu": w, "queue_depth": round(1.25 * qd + 0.05, 3),
+            "ms": dt * 1e3, "best_bin_size": {m: v[0] for m, v in per_mode.items()}, "trials": trials}
 
 
-def autotune(scene_list, S=8, log=None, raster_mode=RASTER_AUTO):
+def autotune(scene_list, S=8, log=None, raster_mode=RASTER_AUTO, modes=None):
     """Returns (per-scene settings list, class table).  The representative of
-    a class is its scene with the median triangle count."""
+    a class is its scene with the median triangle count.  With `modes`, the
+    raster mode is tuned too."""
     groups = {}
     for k, sc in enumerate(scene_list):
         groups.setdefault(scene_class(sc.n, sc.W, sc.H), []).append(k)
@@ -71,19 +78,19 @@ def autotune(scene_list, S=8, log=None, raster_mode=RASTER_AUTO):
     for cls, members in sorted(groups.items()):
         members = sorted(members, key=lambda k: scene_list[k].n)
         rep = scene_list[members[len(members) // 2]]
-        table[cls] = tune(rep, S=S, raster_mode=raster_mode)
+        table[cls] = tune(rep, S=S, raster_mode=raster_mode, modes=modes)
         if log:
-            log("class %s (%d scenes): T=%d waves/CU=%d queue depth %.2f (%.3f ms)" % (
-                cls, len(members), table[cls]["bin_size"], table[cls]["waves_per_cu"], table[cls]["queue_depth"],
-                table[cls]["ms"]))
+            log("class %s (%d scenes): mode %d T=%d waves/CU=%d queue depth %.2f (%.3f ms)" % (
+                cls, len(members), table[cls]["raster_mode"], table[cls]["bin_size"], table[cls]["waves_per_cu"],
+                table[cls]["queue_depth"], table[cls]["ms"]))
     per_scene = [table[scene_class(sc.n, sc.W, sc.H)] for sc in scene_list]
     return per_scene, table
 
 
 def save(table, path):
     with open(path, "w") as f:
-        json.dump({"%d,%d" % k: {kk: vv for kk, vv in v.items() if kk != "trials"} for k, v in table.items()}, f,
-                  indent=1)
+        json.dump({"%d,%d" % k: {kk: (vv if kk != "best_bin_size" else {str(m): t for m, t in vv.items()})
+                                 for kk, vv in v.items() if kk != "trials"} for k, v in table.items()}, f, indent=1)
 
 
 __all__ = ["scene_class", "candidates", "tune", "autotune", "save", "np"]

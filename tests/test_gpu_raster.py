@@ -144,6 +144,30 @@ def test_direct_large_list_grows(oracle_mod):
         assert np.array_equal(a, pipe.pikoScreen.getData())
 
 
+def test_direct_large_after_frames_without(oracle_mod):
+    # after a frame without large triangles the next frame draws any large
+    # ones inside the vertex stage (no large-triangle launch); a camera that
+    # brings large triangles in must give the oracle's image on that frame
+    # and on the next (launched again)
+    import piko_amd
+    far = scenes.heightfield(seed=5, grid=60, W=320, H=200, eye=(0.0, 2.5, 3.0), target=(0.0, 0.0, 0.0))
+    near = scenes.heightfield(seed=5, grid=60, W=320, H=200, eye=(0.0, 0.12, 0.25), target=(0.0, 0.0, -0.3))
+    fs, ns = oracle_mod.vertex_stage(far, 8), oracle_mod.vertex_stage(near, 8)
+    big = lambda st: int(((np.ptp(st.xy[st.keep.astype(bool)][:, 0::2], 1) >> 8) *
+                          (np.ptp(st.xy[st.keep.astype(bool)][:, 1::2], 1) >> 8) > 1024).sum())
+    assert big(fs) == 0 and big(ns) > 0
+    oc, od, _ = oracle_mod.frame(near, 8, 16)
+    with piko_amd.RasterPipe().allocate(far, 16, 8, raster_mode=piko_amd.RASTER_DIRECT) as pipe:
+        pipe.prepare()
+        pipe.run_single()
+        for rep in range(2):
+            log("near camera, frame %d" % rep)
+            pipe.prepare(near)
+            pipe.run_single()
+            c, d = pipe.pikoScreen.getData(), pipe.get_depth()
+            assert np.array_equal(c, oc) and np.array_equal(d.view(np.int32), od.view(np.int32)), rep
+
+
 def test_tiled_large_lists(oracle_mod):
     # every triangle spans more than 2x2 tiles of 16 pixels: all of them go to
     # the chunks' large lists and are clipped to each tile they touch
