@@ -764,7 +764,8 @@ __device__ __forceinline__ void tri_colours(const void *__restrict__ attr, size_
 
 constexpr int kDirThreads = 256;     // one triangle per lane
 constexpr int kDirectMaxPx = 1024;   // larger bboxes take the tile path
-constexpr int kSpan = 4;             // pixels per work unit (one bbox column segment)
+constexpr int kSpan = 8;             // pixels per work unit (one bbox column segment)
+constexpr int kTileSpan = 4;         // the same in k_tile_raster
 constexpr int kLargeTile = 8;
 
 // One staged triangle (64 bytes, read with 16-byte LDS loads).  Edge function
@@ -824,15 +825,14 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
                                                             unsigned long long *__restrict__ vis,
                                                             uint32_t *__restrict__ large,
                                                             uint32_t *__restrict__ tile_off, FrameState *__restrict__ fs,
-                                                            uint32_t large_cap, uint32_t *__restrict__ blk_keep,
-                                                            int inline_large) {
+                                                            uint32_t large_cap, int inline_large) {
+  // every wave works on its own 64 triangles: no block barrier
   __shared__ DirStage<MULTI> st[kDirThreads / 64];
-  __shared__ uint32_t s_keep[kDirThreads / 64];
   const int t = threadIdx.x, lane = t & 63;
   DirStage<MULTI> &s = st[t >> 6];
   const int i = blockIdx.x * kDirThreads + t;
   uint32_t units = 0, tiles = 0;
-  bool big = false, kept = false;
+  bool big = false;
   // a large triangle's setup, kept for the wave that rasterises it inline
   int lp[6] = {0, 0, 0, 0, 0, 0};
   int lrect = 0, lwh = 0;
@@ -847,7 +847,6 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
     float z[3];
     tri_positions<KIND>(pos, src, ja, p, z);
     if (vs_keep(p, ja.W, ja.H, ja.S)) {
-      kept = true;
       Edges e;
       e.set(p);
       const float rcp = 1.0f / (float)e.denom();  // raster.pikostage:174
@@ -896,8 +895,6 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
       }
     }
   }
-  const unsigned long long km = __ballot(kept);
-  if (lane == 0) s_keep[t >> 6] = (uint32_t)__popcll(km);
   // large triangles: one append per wave.  A single 64-bit atomic reserves
   // list slots (low half) and 8x8 tile numbers (high half) together, so the
   // first tile of the listed triangles increases with the slot.  With
@@ -961,13 +958,7 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
   const uint32_t inc = wave_incl_scan(units);
   s.pre[lane] = inc - units;
   const uint32_t tot = __shfl(inc, 63, 64);
-  __syncthreads();
-  if (t == 0) {
-    uint32_t kc = 0;
-#pragma unroll
-    for (int w = 0; w < kDirThreads / 64; ++w) kc += s_keep[w];
-    blk_keep[blockIdx.x] = kc;
-  }
+  wave_fence();
   for (uint32_t cidx = lane; cidx < tot; cidx += 64) {
     int o = 0;  // owner: largest o with pre[o] <= cidx (o + step stays <= 63)
 #pragma unroll
@@ -1288,7 +1279,7 @@ struct TileStage {
   uint32_t tri[64];
 };
 
-// Rasterises staged triangle o's span k (a column segment of kSpan pixels of
+// Rasterises staged triangle o's span k (a column segment of kTileSpan pixels of
 // its bbox clipped to the tile) into the tile's visibility words.
 template <int T>
 __device__ __forceinline__ void tile_span(const TileStage &s, int o, uint32_t k, int S, int xlo, int ylo,
@@ -1300,9 +1291,9 @@ __device__ __forceinline__ void tile_span(const TileStage &s, int o, uint32_t k,
   // band and column of span k: (k + 0.5) / w is at least 0.5 / w from an
   // integer and k < 1024, w <= 64, so the float quotient truncates exactly
   const uint32_t q = (uint32_t)(((float)k + 0.5f) * __builtin_amdgcn_rcpf((float)w));
-  const uint32_t sx = k - q * w, sy = q * kSpan;
+  const uint32_t sx = k - q * w, sy = q * kTileSpan;
   const int px = (int)(m.z & 0xffffu) + (int)sx, py0 = (int)(m.z >> 16) + (int)sy;
-  const uint32_t nv = min((uint32_t)kSpan, h - sy);
+  const uint32_t nv = min((uint32_t)kTileSpan, h - sy);
   const float rcp = __uint_as_float(m.y);
   const float z2 = __int_as_float(b.z), z0mz2 = __int_as_float(b.w), z1mz2 = __uint_as_float(m.x);
   const unsigned long long low = (unsigned long long)(0xFFFFFFFEu - s.tri[o]);
@@ -1322,7 +1313,7 @@ __device__ __forceinline__ void tile_span(const TileStage &s, int o, uint32_t k,
       dy[kk] = (xb - xa) << S;
     }
 #pragma unroll
-    for (uint32_t j = 0; j < (uint32_t)kSpan; ++j) {
+    for (uint32_t j = 0; j < (uint32_t)kTileSpan; ++j) {
       if (j < nv && (int)(e[0] | e[1] | e[2]) >= 0) {
         const float alpha = (float)(int)e[1] * rcp, beta = (float)(int)e[2] * rcp;  // raster.pikostage:269-270
         const float z = interp(z0mz2, z1mz2, z2, alpha, beta);                      // raster.pikostage:272
@@ -1333,7 +1324,7 @@ __device__ __forceinline__ void tile_span(const TileStage &s, int o, uint32_t k,
     }
   } else {
 #pragma unroll
-    for (uint32_t j = 0; j < (uint32_t)kSpan; ++j) {
+    for (uint32_t j = 0; j < (uint32_t)kTileSpan; ++j) {
       float alpha, beta;
       if (j < nv && covered_at(xy, false, px, py0 + (int)j, S, rcp, alpha, beta)) {
         const float z = interp(z0mz2, z1mz2, z2, alpha, beta);
@@ -1481,7 +1472,7 @@ __global__ void __launch_bounds__(kTRThreads) k_tile_raster(const int4 *__restri
                 for (int kk = 0; kk < 3; ++kk) er[kk] += dy[kk];
               }
             } else {
-              units = (uint32_t)w * (uint32_t)((h + kSpan - 1) / kSpan);
+              units = (uint32_t)w * (uint32_t)((h + kTileSpan - 1) / kTileSpan);
               s.xy01[lane] = make_int4(p[0], p[1], p[2], p[3]);
               s.xy2z[lane] = make_int4(p[4], p[5], gc[1].w, gc[2].x);
               s.misc[lane] = make_uint4(__float_as_uint(z1mz2), __float_as_uint(rcp),
@@ -1596,7 +1587,6 @@ struct piko_raster_pipe {
   uint32_t *d_color = nullptr;
   float *d_depth = nullptr;
   unsigned long long *d_vis = nullptr;  // [cap_pix] direct-path visibility words
-  uint32_t *d_keep_dir = nullptr;       // per direct vertex-stage block: emitted triangles
   uint32_t *d_large = nullptr;          // [large_cap] large triangles of the frame
   uint32_t *d_tile_off = nullptr;       // [large_cap] their first 8x8 tile
   uint32_t large_cap = 0;
@@ -1676,7 +1666,6 @@ void free_tris(piko_raster_pipe *p) {
   dfree(p->d_cnt);
   dfree(p->d_keep);
   dfree(p->d_tile_pairs);
-  dfree(p->d_keep_dir);
   dfree(p->d_large);
   dfree(p->d_tile_off);
   p->device_bytes -= p->tri_bytes;
@@ -1692,7 +1681,6 @@ int alloc_tris(piko_raster_pipe *p, long long ntris) {
   if ((rc = dalloc(p, &p->d_cnt, p->cap_tris, &p->tri_bytes)) != PIKO_OK) return rc;
   if ((rc = dalloc(p, &p->d_keep, tiles, &p->tri_bytes)) != PIKO_OK) return rc;
   if ((rc = dalloc(p, &p->d_tile_pairs, tiles, &p->tri_bytes)) != PIKO_OK) return rc;
-  if ((rc = dalloc(p, &p->d_keep_dir, cdiv(p->cap_tris, kDirThreads), &p->tri_bytes)) != PIKO_OK) return rc;
   p->large_cap = (uint32_t)std::min<long long>(std::max<long long>({(long long)p->large_cap, p->cap_tris / 32, 4096}), 1 << 30);
   if ((rc = dalloc(p, &p->d_large, p->large_cap, &p->tri_bytes)) != PIKO_OK) return rc;
   if ((rc = dalloc(p, &p->d_tile_off, p->large_cap, &p->tri_bytes)) != PIKO_OK) return rc;
@@ -1717,6 +1705,7 @@ int alloc_pix(piko_raster_pipe *p, long long npix) {
   int rc;
   if ((rc = dalloc(p, &p->d_color, p->cap_pix, &p->pix_bytes)) != PIKO_OK) return rc;
   if ((rc = dalloc(p, &p->d_depth, p->cap_pix, &p->pix_bytes)) != PIKO_OK) return rc;
+  if (p->mode != PIKO_RASTER_DIRECT) return PIKO_OK;  // per-pixel visibility words: direct path only
   if ((rc = dalloc(p, &p->d_vis, p->cap_pix, &p->pix_bytes)) != PIKO_OK) return rc;
   k_fill_vis<<<(unsigned)std::min<long long>(cdiv(p->cap_pix, 256), 8192), 256, 0, p->stream>>>(p->d_vis, p->cap_pix);
   return hipGetLastError() == hipSuccess ? PIKO_OK : PIKO_ERR_HIP;
@@ -1931,7 +1920,7 @@ void enqueue_direct(piko_raster_pipe *p, Marks &mark) {
 #define PIKO_VSD_LAUNCH(KIND, MULTI)                                                                           \
   k_vs_direct<KIND, MULTI><<<blocks, kDirThreads, 0, p->stream>>>(p->d_pos, n, p->S, p->fa, p->d_jobs, nj,     \
                                                                   p->d_geo, p->d_vis, p->d_large, p->d_tile_off, \
-                                                                  p->d_fs, p->large_cap, p->d_keep_dir, inl)
+                                                                  p->d_fs, p->large_cap, inl)
   if (p->kind == PIKO_INPUT_PRETRANSFORMED) {
     if (p->multi) PIKO_VSD_LAUNCH(PIKO_INPUT_PRETRANSFORMED, true);
     else PIKO_VSD_LAUNCH(PIKO_INPUT_PRETRANSFORMED, false);
@@ -2171,7 +2160,11 @@ int create_common(piko_raster_pipe *p, int kind, int S, long long tris, long lon
   if ((rc = alloc_tris(p, tris)) != PIKO_OK) return rc;
   if ((rc = alloc_pix(p, pix)) != PIKO_OK) return rc;
   if ((rc = alloc_bins(p, bins)) != PIKO_OK) return rc;
-  const long long cap = pair_capacity > 0 ? pair_capacity : std::max<long long>(2ll * tris + 1024, 1 << 16);
+  // (triangle, bin) queues: sized for the frame in the binned path; the
+  // other paths build them only when asked for, growing them then
+  const long long cap = pair_capacity > 0 ? pair_capacity
+                                          : (mode == PIKO_RASTER_BINNED ? std::max<long long>(2ll * tris + 1024, 1 << 16)
+                                                                        : (1 << 16));
   if ((rc = alloc_pairs(p, cap)) != PIKO_OK) return rc;
   HIPOK(hipMemsetAsync(p->d_fs, 0, sizeof(FrameState), p->stream));
   HIPOK(hipStreamSynchronize(p->stream));
@@ -2220,7 +2213,7 @@ int plain_pipe(piko_raster_pipe **out, const piko_constant_state *cs, const floa
   rc = create_common(p, opt->input_kind, opt->subpixel_bits, count, p->npix, p->nbins, opt->pair_capacity,
                      opt->raster_mode);
   p->classes.assign(1, RasterClass{p->T, 0, p->nbins, opt->waves_per_cu > 0 ? opt->waves_per_cu * p->cus : 0});
-  if (rc == PIKO_OK) {
+  if (rc == PIKO_OK && p->mode == PIKO_RASTER_TILED) {
     const int tt = tiled_T(p->T, p->W, p->H);
     p->fa_t = make_args(cs, opt->subpixel_bits, tt);
     p->tclass = RasterClass{tt, 0, (int)p->fa_t.nbins, 0};
@@ -2295,11 +2288,13 @@ int piko_raster_prepare(piko_raster_pipe *p, const piko_constant_state *cs) {
   if (cs->screenSizeX != p->W || cs->screenSizeY != p->H) return PIKO_ERR_INVALID;
   p->fa = make_args(cs, p->S, p->T);
   p->jobs.assign(1, p->fa);
-  FrameArgs ft = make_args(cs, p->S, p->tclass.T);
-  ft.chunk_base = p->fa_t.chunk_base;
-  ft.nchunks = p->fa_t.nchunks;
-  ft.tab_base = p->fa_t.tab_base;
-  p->fa_t = ft;
+  if (p->mode == PIKO_RASTER_TILED) {
+    FrameArgs ft = make_args(cs, p->S, p->tclass.T);
+    ft.chunk_base = p->fa_t.chunk_base;
+    ft.nchunks = p->fa_t.nchunks;
+    ft.tab_base = p->fa_t.tab_base;
+    p->fa_t = ft;
+  }
   return PIKO_OK;
 }
 
@@ -2559,7 +2554,7 @@ int piko_batch_submit(piko_raster_pipe *p, const piko_raster_job *jobs, int njob
     }
     if (bb > lo) p->classes.push_back(RasterClass{T, (int)lo, (int)bb, wpc > 0 ? wpc * p->cus : 0});
   }
-  if (want_pairs > p->cap) {  // preset the queues from the jobs' expected depth
+  if (want_pairs > p->cap && p->mode == PIKO_RASTER_BINNED) {  // preset the queues from the jobs' expected depth
     free_pairs(p);
     if ((rc = alloc_pairs(p, want_pairs)) != PIKO_OK) return rc;
   }
@@ -2567,7 +2562,7 @@ int piko_batch_submit(piko_raster_pipe *p, const piko_raster_job *jobs, int njob
   p->nbins = (int)bb;
   p->npix = (long long)fbb;
   p->passes = radix_passes(p->nbins);
-  if ((rc = setup_chunks(p, p->jobs.data(), njobs, tris)) != PIKO_OK) return rc;
+  if (p->mode == PIKO_RASTER_TILED && (rc = setup_chunks(p, p->jobs.data(), njobs, tris)) != PIKO_OK) return rc;
   std::memcpy(p->h_jobs, p->jobs.data(), (size_t)njobs * sizeof(FrameArgs));
   HIPOK(hipMemcpyAsync(p->d_jobs, p->h_jobs, (size_t)njobs * sizeof(FrameArgs), hipMemcpyHostToDevice, p->stream));
   rc = enqueue_frame(p, nullptr, nullptr);

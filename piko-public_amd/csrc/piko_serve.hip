@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <new>
@@ -40,9 +41,11 @@ namespace {
 // (64 B) + count/queue entries per triangle (2 x 8 B pairs, double-buffered
 // radix keys/values).
 long long footprint(const piko_server *s, const piko_request &r) {
+  // colour + depth + the direct path's visibility word per pixel; setup
+  // record, bin count and queue entries per triangle
   const long long px = (long long)r.cs.screenSizeX * r.cs.screenSizeY;
   const long long n = s->scene_count[r.scene];
-  return px * 8 + n * (64 + 4 + 4 * 16);
+  return px * 16 + n * (64 + 4 + 4 * 16);
 }
 
 double now_s(std::chrono::steady_clock::time_point t0) {
@@ -95,6 +98,9 @@ int piko_server_create(piko_server **out, const float *pos, const void *attr, lo
   bo.max_tris = max_scene;
   bo.max_pixels = 1920 * 1080;
   bo.pair_capacity = 0;
+  // a request is one screen of one resident scene: the direct path, which
+  // measured the lower latency on such frames; PIKO_RASTER_MODE overrides
+  bo.raster_mode = std::getenv("PIKO_RASTER_MODE") ? PIKO_RASTER_AUTO : PIKO_RASTER_DIRECT;
   for (int k = 0; rc == PIKO_OK && k < opt->pipes; ++k) {
     piko_raster_pipe *p = nullptr;
     rc = piko_batch_create(&p, s->d_pos, s->d_attr, tris, 1, &bo);
