@@ -11,8 +11,10 @@ Prints one JSON line.  `value` is Mtri/s at the fastest bin size.  The same
 line carries the batched form (256 uneven scenes in one launch sequence vs
 sequential plain launches, autotuned per scene class) under "batch" and the
 served form (resident scenes, Poisson request stream, continuous batching,
-p50/p99 latency) under "serve", and the Reyes pipeline (split, dice,
-sample, resolve of 4,096 Bezier patches at 720p, 4 spp) under "reyes".
+p50/p99 latency) under "serve", the served form at saturation against
+sequential plain launches of the same requests under "serve_saturated", and
+the Reyes pipeline (split, dice, sample, resolve of 4,096 Bezier patches at
+720p, 4 spp) under "reyes".
 """
 import argparse
 import json
@@ -174,6 +176,69 @@ def bench_serve(log, tris_per_scene, nreq, rate_hz=2000.0):
            "frames_per_s": round(st["frames_per_s"], 1), "p50_ms": round(st["p50_ms"], 3),
            "p99_ms": round(st["p99_ms"], 3), "batches": st["batches"], "mean_batch": round(st["mean_batch"], 2)}
     log("serve: %s" % out)
+    return out
+
+
+def bench_serve_saturated(log, tris_per_scene, nreq=512):
+    """The served form at saturation (every request has arrived before the
+    first batch ends) against sequential plain launches of the same requests:
+    one plain pipe per (scene, resolution) over the same resident geometry,
+    prepare + run_single per request in arrival order."""
+    import numpy as np
+
+    import piko_amd
+    from piko_amd import serve
+    pos, nrm, base, count = serve.resident_scenes(64, tris_per_scene)
+    reqs, _ = serve.poisson_trace(nreq, 1e7, 64, seed=13, bin_size=8)
+    warm, _ = serve.poisson_trace(64, 1e7, 64, seed=14, bin_size=8)
+    rng = np.random.default_rng(1)
+    sample = sorted(int(k) for k in rng.choice(nreq, 16, replace=False))
+    log("serve (saturated): %d requests at once over 64 resident scenes" % nreq)
+    # 8 requests per batch on 4 pipes: the fastest shape of a sweep over 4..64
+    # requests per batch on these fragment-heavy scenes
+    with serve.Server(pos, nrm, base, count, pipes=4, max_jobs=8, budget_bytes=4 << 30) as srv:
+        srv.replay(warm)
+        _, st, cols = srv.replay(reqs, keep=sample)
+    dpos, dnrm = piko_amd.DeviceArray(pos), piko_amd.DeviceArray(nrm)
+    pipes = {}
+    try:
+        def pipe_of(r):
+            key = (r.scene, r.cs.screenSizeX, r.cs.screenSizeY)
+            if key not in pipes:
+                b, n = int(base[r.scene]), int(count[r.scene])
+                pipes[key] = piko_amd.RasterPipe().allocate_device(dpos.ptr + 36 * b, dnrm.ptr + 36 * b, n, r.cs,
+                                                                   bin_size=8, raster_mode=piko_amd.RASTER_DIRECT)
+            return pipes[key]
+        for r in reqs:
+            pipe_of(r)
+        for p in pipes.values():
+            p.run_single()
+        t0 = time.perf_counter()
+        for r in reqs:
+            p = pipe_of(r)
+            p.prepare(cs=r.cs)
+            p.run_single()
+        t_seq = time.perf_counter() - t0
+        exact = True
+        for k in sample:
+            p = pipe_of(reqs[k])
+            p.prepare(cs=reqs[k].cs)
+            p.run_single()
+            exact = exact and bool(np.array_equal(p.pikoScreen.getData(), cols[k]))
+    finally:
+        for p in pipes.values():
+            p.destroy()
+        dpos.free()
+        dnrm.free()
+    seq_fps = nreq / t_seq
+    out = {"workload": "%d requests arriving at once (Poisson at 1e7 req/s) over 64 resident world-space scenes, "
+                       "camera radius U[2,5], FOV U[40,90], {540p,720p,1080p}" % nreq,
+           "requests": nreq, "served_frames_per_s": round(st["frames_per_s"], 1),
+           "sequential_plain_frames_per_s": round(seq_fps, 1),
+           "speedup": round(st["frames_per_s"] / seq_fps, 2), "p50_ms": round(st["p50_ms"], 3),
+           "p99_ms": round(st["p99_ms"], 3), "batches": st["batches"], "mean_batch": round(st["mean_batch"], 2),
+           "bit_exact_vs_plain": exact, "requests_compared": len(sample)}
+    log("serve (saturated): %s" % out)
     return out
 
 
@@ -342,6 +407,7 @@ def main():
         out["batch"] = bench_batch(log)
     if args.workload in ("serve", "all"):
         out["serve"] = bench_serve(log, args.serve_tris, args.serve_requests)
+        out["serve_saturated"] = bench_serve_saturated(log, args.serve_tris)
     if args.workload in ("reyes", "all"):
         out["reyes"] = bench_reyes(log, args.steps, args.warmup, cpu=not args.no_cpu)
     print(json.dumps(out), flush=True)

@@ -763,7 +763,7 @@ __device__ __forceinline__ void tri_colours(const void *__restrict__ attr, size_
 // are needed; the binned path remains for the queues themselves.
 
 constexpr int kDirThreads = 256;     // one triangle per lane
-constexpr int kDirectMaxPx = 1024;   // larger bboxes take the tile path
+constexpr int kDirectMaxPx = 16384;  // larger bboxes (never int32-path ones) take the tile path
 constexpr int kSpan = 8;             // pixels per work unit (one bbox column segment)
 constexpr int kTileSpan = 4;         // the same in k_tile_raster
 constexpr int kLargeTile = 8;
@@ -808,6 +808,52 @@ __device__ __forceinline__ void clip_rect(const int *p, int S, int W, int H, int
   y1 = min(((imax3(p[1], p[3], p[5]) + 0xff) & (int)0xffffff00) >> S, H);
 }
 
+// The jobs of one block's items (triangles or pixels, consecutive in the
+// batch numbering): the job of the block's first item and the next ones,
+// copied to LDS once, so each item finds its job with a short forward scan
+// and reads the job's fields from LDS.  Jobs past the staged ones (a block
+// spanning more than kStageJobs tiny jobs) are read from global memory.
+constexpr int kStageJobs = 4;
+struct JobStage {
+  FrameArgs a[kStageJobs];
+  int first, count;
+};
+
+// by_pixel: items are pixels (fb_base order), else triangles (vbase order).
+__device__ __forceinline__ void stage_jobs(JobStage &js, const FrameArgs *__restrict__ jobs, int njobs,
+                                           unsigned long long first_item, bool by_pixel) {
+  if (threadIdx.x == 0) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      const unsigned long long v = by_pixel ? jobs[mid].fb_base : (unsigned long long)jobs[mid].vbase;
+      if (v <= first_item) lo = mid;
+      else hi = mid - 1;
+    }
+    js.first = lo;
+    js.count = min(kStageJobs, njobs - lo);
+  }
+  __syncthreads();
+  const int nw = (int)(sizeof(FrameArgs) / 4);
+  for (int q = threadIdx.x; q < js.count * nw; q += blockDim.x)
+    ((uint32_t *)js.a)[q] = ((const uint32_t *)(jobs + js.first))[q];
+  __syncthreads();
+}
+
+// The job of `item` (at or after the block's first item).
+__device__ __forceinline__ const FrameArgs &staged_job(const JobStage &js, const FrameArgs *__restrict__ jobs,
+                                                       int njobs, unsigned long long item, bool by_pixel) {
+  int j = js.first;
+  for (;;) {
+    if (j + 1 >= njobs) break;
+    const int k = j + 1 - js.first;
+    const FrameArgs &nx = k < js.count ? js.a[k] : jobs[j + 1];
+    if ((by_pixel ? nx.fb_base : (unsigned long long)nx.vbase) > item) break;
+    ++j;
+  }
+  return j - js.first < js.count ? js.a[j - js.first] : jobs[j];
+}
+
 __device__ __forceinline__ int job_by_pixel(const FrameArgs *__restrict__ jobs, int njobs, unsigned long long g) {
   int lo = 0, hi = njobs - 1;
   while (lo < hi) {
@@ -828,9 +874,11 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
                                                             uint32_t large_cap, int inline_large) {
   // every wave works on its own 64 triangles: no block barrier
   __shared__ DirStage<MULTI> st[kDirThreads / 64];
+  __shared__ JobStage js;
   const int t = threadIdx.x, lane = t & 63;
   DirStage<MULTI> &s = st[t >> 6];
   const int i = blockIdx.x * kDirThreads + t;
+  if (MULTI) stage_jobs(js, jobs, njobs, (unsigned long long)blockIdx.x * kDirThreads, false);
   uint32_t units = 0, tiles = 0;
   bool big = false;
   // a large triangle's setup, kept for the wave that rasterises it inline
@@ -841,7 +889,7 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
   int lW = 0;
   if (i < n) {
     // vertex positions only: the winner's colours are computed in k_resolve
-    const FrameArgs &ja = MULTI ? jobs[job_search(jobs, njobs, (uint32_t)i, false)] : fa;
+    const FrameArgs &ja = MULTI ? staged_job(js, jobs, njobs, (unsigned long long)i, false) : fa;
     const size_t src = MULTI ? (size_t)ja.src_base + (size_t)(i - (int)ja.vbase) : (size_t)i;
     int p[6];
     float z[3];
@@ -971,7 +1019,9 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
     const uint32_t k = cidx - s.pre[o];
     const uint32_t w = r0.y & 0xffffu, h = r0.y >> 16;
     // band and column of unit k: (k + 0.5) / w is at least 0.5 / w from an
-    // integer and w, k <= 1024, so the float quotient truncates exactly
+    // integer, and the float product is within 3 * 2^-24 * (k + 0.5) / w of
+    // it (k + 0.5 is exact, rcpf and the product round once each), which is
+    // below 0.5 / w for every k < 2^21: the quotient truncates exactly
     const uint32_t q = (uint32_t)(((float)k + 0.5f) * __builtin_amdgcn_rcpf((float)w));
     const uint32_t sx = k - q * w, sy = q * kSpan;
     const uint32_t px = (r0.x & 0xffffu) + sx, py = (r0.x >> 16) + sy;
@@ -1058,8 +1108,10 @@ __global__ void __launch_bounds__(256) k_resolve(const int4 *__restrict__ geo, c
                                                  uint32_t *__restrict__ color, float *__restrict__ depth,
                                                  FrameState *__restrict__ fs) {
   __shared__ float s_u8[256];
+  __shared__ JobStage js;
   const int t = threadIdx.x;
   s_u8[t] = kU8Table.v[t];
+  if (MULTI) stage_jobs(js, jobs, njobs, (unsigned long long)blockIdx.x * blockDim.x, true);
   __syncthreads();
   if (blockIdx.x == 0 && t == 0) {  // the large list of this frame is consumed
     fs->large_peak = max(fs->large_peak, (uint32_t)fs->large_acc);
@@ -1068,7 +1120,7 @@ __global__ void __launch_bounds__(256) k_resolve(const int4 *__restrict__ geo, c
   }
   const long long g = (long long)blockIdx.x * blockDim.x + t;
   if (g >= npix) return;
-  const FrameArgs &ja = MULTI ? jobs[job_by_pixel(jobs, njobs, (unsigned long long)g)] : fa;
+  const FrameArgs &ja = MULTI ? staged_job(js, jobs, njobs, (unsigned long long)g, true) : fa;
   const unsigned long long key = vis[g];
   uint32_t col = kClearColor;
   if (key != kVisSentinel) {

@@ -207,10 +207,31 @@ class RasterPipe:
         self._h = h
         return self
 
-    def prepare(self, scene=None):
-        """RasterPipe::prepare (PTXBackend.cpp:791-809): per-frame constant state."""
+    def allocate_device(self, d_pos, d_attr, count, cs, kind="world", bin_size=16, subpixel_bits=8,
+                        pair_capacity=0, waves_per_cu=0, raster_mode=0):
+        """RasterPipe::allocate over input that already lives on the device
+        (piko_raster_allocate_device): d_pos / d_attr are device addresses of
+        `count` triangles; cs is the ConstantState (screen size and camera)."""
+        if self._h is not None:
+            raise RuntimeError("pipe already allocated")
+        self.W, self.H, self.T, self.S = cs.screenSizeX, cs.screenSizeY, bin_size, subpixel_bits
+        self.kind = PIKO_INPUT_PRETRANSFORMED if kind == "pretransformed" else PIKO_INPUT_WORLD
+        self.cs = cs
+        opt = Options(bin_size, subpixel_bits, self.kind, pair_capacity, waves_per_cu, raster_mode)
+        h = ctypes.c_void_p()
+        _check(lib().piko_raster_allocate_device(ctypes.byref(h), ctypes.byref(self.cs), ctypes.c_void_p(d_pos),
+                                                 ctypes.c_void_p(d_attr), int(count), ctypes.byref(opt)),
+               "allocate_device")
+        self._h = h
+        return self
+
+    def prepare(self, scene=None, cs=None):
+        """RasterPipe::prepare (PTXBackend.cpp:791-809): per-frame constant state
+        (from a scene, or a ConstantState)."""
         if scene is not None:
             self.cs = ConstantState.for_scene(scene)
+        if cs is not None:
+            self.cs = cs
         _check(lib().piko_raster_prepare(self._h, ctypes.byref(self.cs)), "prepare")
 
     def run_single(self):
@@ -434,6 +455,32 @@ def render_reyes(scene, keep_samples=False):
         pipe.prepare()
         pipe.run_single()
         return pipe.color(), pipe.depth(), pipe.stats()
+
+
+class DeviceArray:
+    """A device copy of a numpy array through the runtime shim (piko_rt_malloc
+    + piko_rt_memcpy_htod); `ptr` is its device address."""
+
+    def __init__(self, arr):
+        L = lib()
+        L.piko_rt_malloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        L.piko_rt_memcpy_htod.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        L.piko_rt_free.argtypes = [ctypes.c_void_p]
+        a = np.ascontiguousarray(arr)
+        self.nbytes = a.nbytes
+        self._d = ctypes.c_void_p()
+        _check(L.piko_rt_malloc(ctypes.byref(self._d), max(a.nbytes, 4)), "rt_malloc")
+        if a.nbytes:
+            _check(L.piko_rt_memcpy_htod(self._d, _p(a), a.nbytes), "rt_memcpy_htod")
+
+    @property
+    def ptr(self):
+        return self._d.value
+
+    def free(self):
+        if self._d:
+            lib().piko_rt_free(self._d)
+            self._d = ctypes.c_void_p()
 
 
 def concat_scenes(scene_list):

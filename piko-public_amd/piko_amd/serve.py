@@ -110,14 +110,17 @@ class Server:
         self._h = h
 
     def replay(self, reqs, keep=False):
-        """Returns (done_s [n], stats dict, colours or None)."""
+        """Returns (done_s [n], stats dict, colours or None).  keep: True for
+        every request's framebuffer, or a list of request indices."""
         n = len(reqs)
         done = np.zeros(n, np.float64)
         cols = None
         keep_arr = None
-        if keep:
-            cols = [np.empty(r.cs.screenSizeX * r.cs.screenSizeY, np.uint32) for r in reqs]
-            keep_arr = (ctypes.c_void_p * n)(*[c.ctypes.data for c in cols])
+        if keep is not False:
+            idx = set(range(n)) if keep is True else set(int(k) for k in keep)
+            cols = [np.empty(r.cs.screenSizeX * r.cs.screenSizeY, np.uint32) if i in idx else None
+                    for i, r in enumerate(reqs)]
+            keep_arr = (ctypes.c_void_p * n)(*[c.ctypes.data if c is not None else None for c in cols])
         st = ServerStats()
         rc = _bind().piko_server_replay(self._h, reqs, n, _p(done), keep_arr, ctypes.byref(st))
         if rc != 0:
@@ -129,8 +132,9 @@ class Server:
                  "frames_per_s": n / st.wall_s if st.wall_s > 0 else 0.0,
                  "p50_ms": float(np.percentile(lat, 50) * 1e3) if n else 0.0,
                  "p99_ms": float(np.percentile(lat, 99) * 1e3) if n else 0.0}
-        if keep:
-            cols = [c.reshape(r.cs.screenSizeY, r.cs.screenSizeX) for c, r in zip(cols, reqs)]
+        if keep is not False:
+            cols = [c.reshape(r.cs.screenSizeY, r.cs.screenSizeX) if c is not None else None
+                    for c, r in zip(cols, reqs)]
         return done, stats, cols
 
     def destroy(self):

@@ -125,16 +125,16 @@ def sized_tris(seed, n, W, H, lo, hi):
 
 
 def test_direct_size_threshold(oracle_mod):
-    # bboxes from a few pixels to ~80x80: both sides of the 1024-pixel limit of
-    # the direct vertex stage, the rest drawn in 8x8 tiles
-    sc = sized_tris(11, 3000, 400, 300, 4, 80)
+    # bboxes from a few pixels to ~190x190: both sides of the 16384-pixel
+    # limit of the direct vertex stage, the rest drawn in 8x8 tiles
+    sc = sized_tris(11, 1500, 400, 300, 4, 190)
     assert_frame_equal(sc, 16, 8, oracle_mod)
 
 
 def test_direct_large_list_grows(oracle_mod):
     # more than 4096 large triangles: the large-triangle list overflows,
     # grows and the frame is re-run
-    sc = sized_tris(12, 6000, 512, 384, 45, 60)
+    sc = sized_tris(12, 9000, 512, 384, 170, 200)
     assert_frame_equal(sc, 32, 8, oracle_mod, modes=("direct",))
     import piko_amd
     with piko_amd.RasterPipe().allocate(sc, 32, 8, raster_mode=piko_amd.RASTER_DIRECT) as pipe:
