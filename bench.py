@@ -130,10 +130,10 @@ def bench_batch(log, reps=5):
             p.run_single()
     t_seq = (time.perf_counter() - t0) / reps
     pos, attr, parts = piko_amd.concat_scenes(scs)
+    tiles, btable = autotune.tune_batch(scs, pos, attr, parts)
     bp = piko_amd.BatchPipe(pos, attr, "pretransformed", max_jobs=len(scs), max_tris=ntris, max_pixels=npix,
                             raster_mode=piko_amd.RASTER_TILED)
-    bp.submit([(n, base, cfg["best_bin_size"][piko_amd.RASTER_TILED], piko_amd.ConstantState.for_scene(s),
-                cfg["waves_per_cu"], cfg["queue_depth"]) for (n, base, s), cfg in zip(parts, per)])
+    bp.submit([(n, base, T, piko_amd.ConstantState.for_scene(s)) for (n, base, s), T in zip(parts, tiles)])
     bp.wait()
     bp.run_frames(1)
     t0 = time.perf_counter()
@@ -146,13 +146,14 @@ def bench_batch(log, reps=5):
     for p in pipes:
         p.destroy()
     out = {"workload": "256 uniform-soup scenes, seeds 1000-1255, N log-uniform [1K,500K], "
-                       "pixels log-uniform [256^2,2048^2], aspect {1,4:3,16:9}, autotuned per class (plain: mode and bin size; batch: tile size)",
+                       "pixels log-uniform [256^2,2048^2], aspect {1,4:3,16:9}, autotuned per class (plain pipes: raster path and bin size; batch: tile size timed in the batch)",
            "scenes": len(scs), "triangles": ntris, "pixels": npix, "classes": len(table),
            "seq_ms_per_batch": round(t_seq * 1e3, 3), "batch_ms": round(t_batch * 1e3, 3),
            "seq_frames_per_s": round(len(scs) / t_seq, 1), "batch_frames_per_s": round(len(scs) / t_batch, 1),
            "speedup": round(t_seq / t_batch, 2), "batch_Mtri_per_s": round(ntris / t_batch / 1e6, 1),
            "plain_modes": {"direct": sum(1 for c in per if c["raster_mode"] == piko_amd.RASTER_DIRECT),
                            "tiled": sum(1 for c in per if c["raster_mode"] == piko_amd.RASTER_TILED)},
+           "batch_tile_sizes": {str(T): tiles.count(T) for T in sorted(set(tiles))},
            "bit_exact_vs_sequential": exact, "scenes_compared": len(checked)}
     log("batch: %s" % out)
     return out

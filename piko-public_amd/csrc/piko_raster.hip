@@ -1110,15 +1110,22 @@ __global__ void __launch_bounds__(256) k_resolve(const int4 *__restrict__ geo, c
   __shared__ float s_u8[256];
   __shared__ JobStage js;
   const int t = threadIdx.x;
+  // block i runs on XCD i % 8: each XCD resolves a contiguous run of pixel
+  // blocks (whole bands of rows), so the records of a triangle that covers
+  // pixels in neighbouring rows are gathered into one XCD's L2
+  const unsigned nblk = (unsigned)((npix + 255) / 256);
+  const unsigned per = (nblk + 7) / 8;
+  const unsigned b = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  if (b >= nblk) return;
   s_u8[t] = kU8Table.v[t];
-  if (MULTI) stage_jobs(js, jobs, njobs, (unsigned long long)blockIdx.x * blockDim.x, true);
+  if (MULTI) stage_jobs(js, jobs, njobs, (unsigned long long)b * blockDim.x, true);
   __syncthreads();
-  if (blockIdx.x == 0 && t == 0) {  // the large list of this frame is consumed
+  if (b == 0 && t == 0) {  // the large list of this frame is consumed
     fs->large_peak = max(fs->large_peak, (uint32_t)fs->large_acc);
     fs->tiles_peak = max(fs->tiles_peak, (uint32_t)(fs->large_acc >> 32));
     fs->large_acc = 0;
   }
-  const long long g = (long long)blockIdx.x * blockDim.x + t;
+  const long long g = (long long)b * blockDim.x + t;
   if (g >= npix) return;
   const FrameArgs &ja = MULTI ? staged_job(js, jobs, njobs, (unsigned long long)g, true) : fa;
   const unsigned long long key = vis[g];
@@ -1998,7 +2005,7 @@ void enqueue_direct(piko_raster_pipe *p, Marks &mark) {
                                                         p->fa, p->d_jobs, nj, p->d_vis);
   mark();
   }
-  const long long rblocks = std::max<long long>(cdiv(p->npix, 256), 1);
+  const long long rblocks = 8 * cdiv(std::max<long long>(cdiv(p->npix, 256), 1), 8);  // see k_resolve
 #define PIKO_RES_LAUNCH(KIND, MULTI)                                                                          \
   k_resolve<KIND, MULTI><<<(unsigned)rblocks, 256, 0, p->stream>>>(p->d_geo, p->d_attr, p->d_vis, p->fa, p->d_jobs, \
                                                                    nj, p->npix, p->d_color, p->d_depth, p->d_fs)

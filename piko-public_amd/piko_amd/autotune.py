@@ -87,10 +87,47 @@ def autotune(scene_list, S=8, log=None, raster_mode=RASTER_AUTO, modes=None):
     return per_scene, table
 
 
+def tune_batch(scene_list, pos, attr, parts, S=8, tile_sizes=TILE_SIZES, frames=3, log=None):
+    """Tile size of the batched (tiled) form per scene class, timed in the
+    batch itself: each class's scenes are run as one sub-batch at every
+    candidate tile size and the fastest wins.  `pos`/`attr`/`parts` are
+    piko_amd.concat_scenes(scene_list).  Returns (per-scene tile sizes,
+    {class: (T, ms)})."""
+    from . import BatchPipe, ConstantState, RASTER_TILED
+    groups = {}
+    for k, sc in enumerate(scene_list):
+        groups.setdefault(scene_class(sc.n, sc.W, sc.H), []).append(k)
+    kind = scene_list[0].kind if scene_list else "pretransformed"
+    ntris = sum(sc.n for sc in scene_list)
+    npix = sum(sc.W * sc.H for sc in scene_list)
+    table = {}
+    bp = BatchPipe(pos, attr, kind, subpixel_bits=S, max_jobs=max(len(scene_list), 1), max_tris=ntris,
+                   max_pixels=npix, raster_mode=RASTER_TILED)
+    try:
+        for cls, members in sorted(groups.items()):
+            best = None
+            for T in tile_sizes:
+                bp.submit([(parts[k][0], parts[k][1], T, ConstantState.for_scene(scene_list[k])) for k in members])
+                bp.wait()
+                bp.run_frames(1)
+                t0 = time.perf_counter()
+                bp.run_frames(frames)
+                dt = (time.perf_counter() - t0) / frames
+                if best is None or dt < best[1]:
+                    best = (T, dt)
+            table[cls] = (best[0], best[1] * 1e3)
+            if log:
+                log("batch class %s (%d scenes): T=%d (%.3f ms)" % (cls, len(members), best[0], best[1] * 1e3))
+    finally:
+        bp.destroy()
+    per_scene = [table[scene_class(sc.n, sc.W, sc.H)][0] for sc in scene_list]
+    return per_scene, table
+
+
 def save(table, path):
     with open(path, "w") as f:
         json.dump({"%d,%d" % k: {kk: (vv if kk != "best_bin_size" else {str(m): t for m, t in vv.items()})
                                  for kk, vv in v.items() if kk != "trials"} for k, v in table.items()}, f, indent=1)
 
 
-__all__ = ["scene_class", "candidates", "tune", "autotune", "save", "np"]
+__all__ = ["scene_class", "candidates", "tune", "autotune", "tune_batch", "save", "np"]
