@@ -115,11 +115,13 @@ __device__ uint32_t block_scan_inplace(uint32_t *__restrict__ v, int n, Addr add
 //   [1] x2 y2, 1/barydenom (raster.pikostage:174), z2
 //   [2] z0 - z2, z1 - z2
 // Colour needs the first 32 bytes only.
+// colour_only: write the first 32 bytes (a triangle whose visibility is
+// decided without gathering its depths again).
 __device__ __forceinline__ void put_setup(int4 *__restrict__ geo, size_t i, const int p[6], float rcp,
-                                          const float z[3]) {
+                                          const float z[3], bool colour_only = false) {
   geo[3 * i + 0] = make_int4(p[0], p[1], p[2], p[3]);
   geo[3 * i + 1] = make_int4(p[4], p[5], __float_as_int(rcp), __float_as_int(z[2]));
-  geo[3 * i + 2] = make_int4(__float_as_int(z[0] - z[2]), __float_as_int(z[1] - z[2]), 0, 0);
+  if (!colour_only) geo[3 * i + 2] = make_int4(__float_as_int(z[0] - z[2]), __float_as_int(z[1] - z[2]), 0, 0);
 }
 
 // ---- vertex stage + bin count ---------------------------------------------
@@ -898,12 +900,15 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
       Edges e;
       e.set(p);
       const float rcp = 1.0f / (float)e.denom();  // raster.pikostage:174
-      put_setup(geo, (size_t)i, p, rcp, z);
       int x0, y0, x1, y1;
       clip_rect(p, ja.S, ja.W, ja.H, x0, y0, x1, y1);
       const int w = max(0, x1 - x0), h = max(0, y1 - y0);
       const uint32_t npx = (uint32_t)w * (uint32_t)h;
-      if (npx > (uint32_t)kDirectMaxPx || !small_bbox(p, ja.S)) {
+      const bool listed = npx > (uint32_t)kDirectMaxPx || !small_bbox(p, ja.S);
+      // the large-triangle raster gathers the depths again; spans and the
+      // resolve need only the colour part
+      put_setup(geo, (size_t)i, p, rcp, z, !listed);
+      if (listed) {
         big = npx > 0;
         tiles = (uint32_t)((w + kLargeTile - 1) / kLargeTile) * (uint32_t)((h + kLargeTile - 1) / kLargeTile);
 #pragma unroll
