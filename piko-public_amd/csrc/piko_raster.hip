@@ -767,7 +767,7 @@ __device__ __forceinline__ void tri_colours(const void *__restrict__ attr, size_
 constexpr int kDirThreads = 256;     // one triangle per lane
 constexpr int kDirectMaxPx = 16384;  // larger bboxes (never int32-path ones) take the tile path
 constexpr int kSpan = 8;             // pixels per work unit (one bbox column segment)
-constexpr int kTileSpan = 4;         // the same in k_tile_raster
+constexpr int kTileSpan = 8;         // the same in k_tile_raster
 constexpr int kLargeTile = 8;
 
 // One staged triangle (64 bytes, read with 16-byte LDS loads).  Edge function
@@ -1190,7 +1190,6 @@ constexpr int kTRThreads = 256;                   // k_tile_raster block
 constexpr int kTRWinPer = 2;                      // chunks each k_tile_raster thread reads per window
 constexpr int kTRWin = kTRThreads * kTRWinPer;    // chunks read per window
 constexpr int kExactPx = 16;                      // k_tile_bin: exact tile mask up to this many bbox pixels
-constexpr int kLanePx = 16;                       // k_tile_raster: one lane rasterises rects up to this size
 
 struct ChunkDesc {
   uint32_t job;      // index in the job table
@@ -1352,8 +1351,7 @@ __device__ __forceinline__ void tile_span(const TileStage &s, int o, uint32_t k,
   const int4 b = s.xy2z[o];
   const uint4 m = s.misc[o];
   const uint32_t w = m.w & 0xffffu, h = (m.w >> 16) & 0x7fffu;
-  // band and column of span k: (k + 0.5) / w is at least 0.5 / w from an
-  // integer and k < 1024, w <= 64, so the float quotient truncates exactly
+  // band and column of span k (the quotient truncates exactly: see k_vs_direct)
   const uint32_t q = (uint32_t)(((float)k + 0.5f) * __builtin_amdgcn_rcpf((float)w));
   const uint32_t sx = k - q * w, sy = q * kTileSpan;
   const int px = (int)(m.z & 0xffffu) + (int)sx, py0 = (int)(m.z >> 16) + (int)sy;
@@ -1502,48 +1500,13 @@ __global__ void __launch_bounds__(kTRThreads) k_tile_raster(const int4 *__restri
           y1 = min(y1, yhi);
           const int w = max(0, x1 - x0), h = max(0, y1 - y0);
           if (w > 0 && h > 0) {
-            const float rcp = __int_as_float(gc[1].z);
-            const float z2 = __int_as_float(gc[1].w), z0mz2 = __int_as_float(gc[2].x), z1mz2 = __int_as_float(gc[2].y);
-            const bool small = small_bbox(p, S);
-            if (small && w * h <= kLanePx) {
-              // a few pixels: this lane tests them itself (raster.pikostage:223-241)
-              const uint32_t half = 1u << (S - 1);
-              const uint32_t cx = ((uint32_t)x0 << S) + half, cy = ((uint32_t)y0 << S) + half;
-              uint32_t er[3], dx[3], dy[3];
-#pragma unroll
-              for (int kk = 0; kk < 3; ++kk) {
-                const uint32_t xa = (uint32_t)p[2 * kk], ya = (uint32_t)p[2 * kk + 1];
-                const uint32_t xb = (uint32_t)p[(2 * kk + 2) % 6], yb = (uint32_t)p[(2 * kk + 3) % 6];
-                er[kk] = (cy - ya) * (xb - xa) - (cx - xa) * (yb - ya);
-                dx[kk] = (0u - (yb - ya)) << S;
-                dy[kk] = (xb - xa) << S;
-              }
-              const unsigned long long low = (unsigned long long)(0xFFFFFFFEu - v_cur);
-              for (int r = 0; r < h; ++r) {
-                uint32_t e0 = er[0], e1 = er[1], e2 = er[2];
-                for (int c = 0; c < w; ++c) {
-                  if ((int)(e0 | e1 | e2) >= 0) {
-                    const float alpha = (float)(int)e1 * rcp, beta = (float)(int)e2 * rcp;  // raster.pikostage:269-270
-                    const float zz = interp(z0mz2, z1mz2, z2, alpha, beta);                // raster.pikostage:272
-                    atomicMin((long long *)&vis[(y0 - ylo + r) * T + (x0 - xlo + c)],
-                              (long long)(((unsigned long long)__float_as_uint(zz) << 32) | low));
-                  }
-                  e0 += dx[0];
-                  e1 += dx[1];
-                  e2 += dx[2];
-                }
-#pragma unroll
-                for (int kk = 0; kk < 3; ++kk) er[kk] += dy[kk];
-              }
-            } else {
-              units = (uint32_t)w * (uint32_t)((h + kTileSpan - 1) / kTileSpan);
-              s.xy01[lane] = make_int4(p[0], p[1], p[2], p[3]);
-              s.xy2z[lane] = make_int4(p[4], p[5], gc[1].w, gc[2].x);
-              s.misc[lane] = make_uint4(__float_as_uint(z1mz2), __float_as_uint(rcp),
-                                        (uint32_t)x0 | ((uint32_t)y0 << 16),
-                                        (uint32_t)w | ((uint32_t)h << 16) | (small ? 0x80000000u : 0u));
-              s.tri[lane] = v_cur;
-            }
+            // staged for the wave's balanced spans (tile_span)
+            units = (uint32_t)w * (uint32_t)((h + kTileSpan - 1) / kTileSpan);
+            s.xy01[lane] = make_int4(p[0], p[1], p[2], p[3]);
+            s.xy2z[lane] = make_int4(p[4], p[5], gc[1].w, gc[2].x);
+            s.misc[lane] = make_uint4((uint32_t)gc[2].y, (uint32_t)gc[1].z, (uint32_t)x0 | ((uint32_t)y0 << 16),
+                                      (uint32_t)w | ((uint32_t)h << 16) | (small_bbox(p, S) ? 0x80000000u : 0u));
+            s.tri[lane] = v_cur;
           }
         }
         v_cur = v_nxt;
