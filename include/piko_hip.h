@@ -72,7 +72,7 @@ enum { PIKO_INPUT_PRETRANSFORMED = 0, PIKO_INPUT_WORLD = 1 };
  *           in 8x8 tiles; a resolve shades the winner.
  *   TILED:  bins are screen tiles (the bin size, at least 16, larger where a
  *           screen would have more than 8192 tiles).  The vertex stage
- *           counts, scans and scatters each chunk of 8192 triangles into
+ *           counts, scans and scatters each chunk of 4096 triangles into
  *           per-tile lists in LDS; one workgroup per tile then resolves every
  *           fragment in LDS and writes each pixel once.  No global atomics.
  *   DIRECT and TILED keep no sorted queues: the bin queues are built on
@@ -261,7 +261,21 @@ typedef struct {
                           displacement*(x^2 - y^2) of the surface point; 0 = none */
   int roots_per_wave;  /* patches whose split trees one wave walks; 0 = 1 */
   int keep_samples;    /* 1: keep per-sample depth for piko_reyes_get_samples */
+  int sampling;        /* PIKO_SAMPLE_*: how covered samples are composited */
+  int tile;            /* tiled sampling's tile edge in pixels: 16 or 32; 0 = 16 */
 } piko_reyes_options;
+
+/* How Reyes composites samples.  Both give the same frame bit for bit.
+ *   GLOBAL: the dice kernel takes a 64-bit atomicMin of (depth, order) per
+ *           covered sample in a W*H*spp buffer; a resolve pass filters it.
+ *   TILED:  the dice kernel keeps the micropolygons and lists each leaf in
+ *           the screen tiles it touches (16 pixels, or 32 where a screen
+ *           would have more than 8192 tiles; larger screens are refused);
+ *           workgroups take the same minimum per tile in LDS and resolve
+ *           its pixels.  A tile with many leaves is split into parts whose
+ *           keys meet in the visibility buffer.
+ *   AUTO:   GLOBAL (the faster of the two on the sphere configuration). */
+enum { PIKO_SAMPLE_AUTO = 0, PIKO_SAMPLE_GLOBAL = 1, PIKO_SAMPLE_TILED = 2 };
 
 typedef struct piko_reyes_pipe piko_reyes_pipe;
 
@@ -304,6 +318,8 @@ typedef struct {
   long long device_bytes;
   int capacity_reruns;       /* frames re-run by the last run call because a
                                 work queue grew */
+  long long tile_pairs;      /* (tile, leaf) list entries of the last frame
+                                (tiled sampling; 0 otherwise) */
 } piko_reyes_stats;
 
 int piko_reyes_get_stats(piko_reyes_pipe *pipe, piko_reyes_stats *st);

@@ -47,13 +47,16 @@ constexpr int kLevels = 21;  // levels 0..20: a patch splits while numSplits < 2
 constexpr int kSplitCapMax = 20;
 constexpr unsigned long long kVisEmpty = ((unsigned long long)(0x3F800000u ^ 0x80000000u) << 32) | 0xFFFFFFFFull;
 constexpr unsigned long long kMaxOrder = 0xFFFFFFFEull;  // quads per frame must stay below this
+constexpr unsigned long long kMaxPairs = 0xFFFFFFFFull;  // (tile, leaf) pairs of tiled sampling
 
 struct ReyesState {
   unsigned long long nleaf;
   unsigned long long nquad;
-  unsigned int overflow;  // bit 0: a split frontier, bit 1: a leaf region, bit 2: quad colours
+  unsigned int overflow;  // bit 0: a split frontier, bit 1: a leaf region, bit 2: quad colours,
+                          // bit 3: the (tile, leaf) lists of tiled sampling
   unsigned int levels;
-  unsigned int pad[2];
+  unsigned int npairs;  // (tile, leaf) pairs of tiled sampling, saturated at 2^32 - 1
+  unsigned int pad;
 };
 
 struct ReyesArgs {
@@ -66,6 +69,10 @@ struct ReyesArgs {
   int front_cap, leaf_cap;        // per split wave: nodes per level buffer, leaves
   unsigned long long qcap;        // quad colour slots
   int keep_samples;
+  // tiled sampling: tiles of 2^tsb pixels, tnx x tny of them; the grid
+  // points of a leaf start at pt_num / 4 times its first order index
+  int tsb, tnx, tny;
+  uint32_t pair_cap, pt_num;
 };
 
 // piko::fminf / fmaxf of the CPU backend are std::min / std::max
@@ -455,6 +462,89 @@ __device__ __forceinline__ bool quad_hit(const float *X, const float *Y, float p
   return r1 || r2;
 }
 
+// One leaf's dice set-up, by the one wave of a workgroup: control points,
+// grid size, the order index of its first micropolygon, then every grid
+// point's screen position, depth and lit colour in s_pt.
+struct LeafGrid {
+  uint32_t qbase;  // order index of the leaf's first micropolygon
+  int gu, gv;      // micropolygons along u and v
+};
+
+template <int MAXP>
+__device__ __forceinline__ LeafGrid dice_leaf(const ReyesArgs &a, const float4 *__restrict__ lcp,
+                                              const uint32_t *__restrict__ lmeta,
+                                              const uint32_t *__restrict__ lqloc, const uint32_t *__restrict__ qoff,
+                                              size_t at, float4 *s_cp, float4 (*s_row)[8], float (*s_pt)[6],
+                                              int lane) {
+  const float *m = a.vp;
+  const int W = a.W, H = a.H;
+  if (lane < 16) s_cp[lane] = lcp[at * 16 + lane];
+  const uint32_t mt = lmeta[at];
+  const uint32_t qbase = qoff[(mt >> 16) * (uint32_t)a.G + (uint32_t)(at / (size_t)a.leaf_cap)] + lqloc[at];
+  int nu, nv, gu, gv;  // dice steps; micropolygons along u and v
+  if (a.dice > 0) {
+    nu = nv = gu = gv = a.dice;
+  } else {
+    nu = (int)(mt & 0xffu);
+    nv = (int)((mt >> 8) & 0xffu);
+    gu = nu + 1;  // k <= nu (dice.pikostage:176-177)
+    gv = nv + 1;
+  }
+  const float du = 1.0f / ((float)nu), dv = 1.0f / ((float)nv);
+  wave_sync();
+  // grid u values: myU = k*du for k < gu, and the last quad's myU + du
+  // (dice.pikostage:180, 233); with power-of-two steps these coincide with
+  // the neighbouring quads' corners bit for bit.
+  for (int e = lane; e < 8 * (gu + 1); e += 64) {  // (grid u, row) pairs
+    const int iu = e >> 3, row = e & 7, r4 = 4 * (row & 3);
+    const float u = iu < gu ? (float)iu * du : (float)(gu - 1) * du + du;
+    float B[4], D[4];
+    bern(u, B, D);
+    s_row[iu][row] = comb(row < 4 ? B : D, s_cp[r4], s_cp[r4 + 1], s_cp[r4 + 2], s_cp[r4 + 3]);
+  }
+  wave_sync();
+  const int np = (gu + 1) * (gv + 1);
+  for (int pt = lane; pt < np; pt += 64) {
+    const int iu = pt / (gv + 1), jv = pt - iu * (gv + 1);
+    const float v = jv < gv ? (float)jv * dv : (float)(gv - 1) * dv + dv;
+    float B[4], D[4];
+    bern(v, B, D);
+    // patchEval (dice.pikostage:111-121), normEval (123-146)
+    float4 P = comb(B, s_row[iu][0], s_row[iu][1], s_row[iu][2], s_row[iu][3]);
+    const float4 A = comb(B, s_row[iu][4], s_row[iu][5], s_row[iu][6], s_row[iu][7]);
+    const float4 Bv = comb(D, s_row[iu][0], s_row[iu][1], s_row[iu][2], s_row[iu][3]);
+    float nx = A.y * Bv.z - A.z * Bv.y, ny = A.z * Bv.x - A.x * Bv.z, nz = A.x * Bv.y - A.y * Bv.x;
+    const float mag = sqrtf(nx * nx + ny * ny + nz * nz);
+    nx /= mag;
+    ny /= mag;
+    nz /= mag;
+    if (a.disp != 0.0f) {
+      const float d = a.disp * (P.x * P.x - P.y * P.y);
+      P.x = P.x + d * nx;
+      P.y = P.y + d * ny;
+      P.z = P.z + d * nz;
+    }
+    // projection (dice.pikostage:209-216)
+    const float sx = ((m[0] * P.x + m[4] * P.y) + m[8] * P.z) + m[12] * P.w;
+    const float sy = ((m[1] * P.x + m[5] * P.y) + m[9] * P.z) + m[13] * P.w;
+    const float sz = ((m[2] * P.x + m[6] * P.y) + m[10] * P.z) + m[14] * P.w;
+    const float sw = ((m[3] * P.x + m[7] * P.y) + m[11] * P.z) + m[15] * P.w;
+    const float onebyw = 1.0f / sw;
+    float *o = s_pt[pt];
+    o[0] = ((sx * onebyw + 1.0f) * (float)W) * 0.5f;
+    o[1] = ((sy * onebyw + 1.0f) * (float)H) * 0.5f;
+    o[2] = (sz * onebyw);
+    const float ix = -nx, iy = -ny, iz = -nz;
+    float dif = ix * 0.10f + iy * 0.35f + iz * 0.85f;
+    dif = dif < 0.0f ? 0.0f : dif;
+    o[3] = sat(0.2f + dif * 0.1773f);
+    o[4] = sat(0.2f + dif * 0.5773f);
+    o[5] = sat(0.2f + dif * 0.973f);
+  }
+  wave_sync();
+  return LeafGrid{qbase, gu, gv};
+}
+
 template <int MAXP, int SPP>
 __global__ __launch_bounds__(64) void k_reyes_dice(ReyesArgs a, const float4 *__restrict__ lcp,
                                                    const uint32_t *__restrict__ lmeta,
@@ -472,74 +562,11 @@ __global__ __launch_bounds__(64) void k_reyes_dice(ReyesArgs a, const float4 *__
   const int lane = threadIdx.x;
   if (st->overflow & 4u) return;  // quad colours would not fit: the frame is re-run with room
   const unsigned long long nleaf = st->nleaf;
-  const float *m = a.vp;
   const int W = a.W, H = a.H;
   for (unsigned long long g = blockIdx.x; g < nleaf; g += gridDim.x) {
-    const size_t at = leaf_at[g];
-    if (lane < 16) s_cp[lane] = lcp[at * 16 + lane];
-    const uint32_t mt = lmeta[at];
-    const uint32_t qbase = qoff[(mt >> 16) * (uint32_t)a.G + (uint32_t)(at / (size_t)a.leaf_cap)] + lqloc[at];
-    int nu, nv, gu, gv;  // dice steps; micropolygons along u and v
-    if (a.dice > 0) {
-      nu = nv = gu = gv = a.dice;
-    } else {
-      nu = (int)(mt & 0xffu);
-      nv = (int)((mt >> 8) & 0xffu);
-      gu = nu + 1;  // k <= nu (dice.pikostage:176-177)
-      gv = nv + 1;
-    }
-    const float du = 1.0f / ((float)nu), dv = 1.0f / ((float)nv);
-    wave_sync();
-    // grid u values: myU = k*du for k < gu, and the last quad's myU + du
-    // (dice.pikostage:180, 233); with power-of-two steps these coincide with
-    // the neighbouring quads' corners bit for bit.
-    for (int e = lane; e < 8 * (gu + 1); e += 64) {  // (grid u, row) pairs
-      const int iu = e >> 3, row = e & 7, r4 = 4 * (row & 3);
-      const float u = iu < gu ? (float)iu * du : (float)(gu - 1) * du + du;
-      float B[4], D[4];
-      bern(u, B, D);
-      s_row[iu][row] = comb(row < 4 ? B : D, s_cp[r4], s_cp[r4 + 1], s_cp[r4 + 2], s_cp[r4 + 3]);
-    }
-    wave_sync();
-    const int np = (gu + 1) * (gv + 1);
-    for (int pt = lane; pt < np; pt += 64) {
-      const int iu = pt / (gv + 1), jv = pt - iu * (gv + 1);
-      const float v = jv < gv ? (float)jv * dv : (float)(gv - 1) * dv + dv;
-      float B[4], D[4];
-      bern(v, B, D);
-      // patchEval (dice.pikostage:111-121), normEval (123-146)
-      float4 P = comb(B, s_row[iu][0], s_row[iu][1], s_row[iu][2], s_row[iu][3]);
-      const float4 A = comb(B, s_row[iu][4], s_row[iu][5], s_row[iu][6], s_row[iu][7]);
-      const float4 Bv = comb(D, s_row[iu][0], s_row[iu][1], s_row[iu][2], s_row[iu][3]);
-      float nx = A.y * Bv.z - A.z * Bv.y, ny = A.z * Bv.x - A.x * Bv.z, nz = A.x * Bv.y - A.y * Bv.x;
-      const float mag = sqrtf(nx * nx + ny * ny + nz * nz);
-      nx /= mag;
-      ny /= mag;
-      nz /= mag;
-      if (a.disp != 0.0f) {
-        const float d = a.disp * (P.x * P.x - P.y * P.y);
-        P.x = P.x + d * nx;
-        P.y = P.y + d * ny;
-        P.z = P.z + d * nz;
-      }
-      // projection (dice.pikostage:209-216)
-      const float sx = ((m[0] * P.x + m[4] * P.y) + m[8] * P.z) + m[12] * P.w;
-      const float sy = ((m[1] * P.x + m[5] * P.y) + m[9] * P.z) + m[13] * P.w;
-      const float sz = ((m[2] * P.x + m[6] * P.y) + m[10] * P.z) + m[14] * P.w;
-      const float sw = ((m[3] * P.x + m[7] * P.y) + m[11] * P.z) + m[15] * P.w;
-      const float onebyw = 1.0f / sw;
-      float *o = s_pt[pt];
-      o[0] = ((sx * onebyw + 1.0f) * (float)W) * 0.5f;
-      o[1] = ((sy * onebyw + 1.0f) * (float)H) * 0.5f;
-      o[2] = (sz * onebyw);
-      const float ix = -nx, iy = -ny, iz = -nz;
-      float dif = ix * 0.10f + iy * 0.35f + iz * 0.85f;
-      dif = dif < 0.0f ? 0.0f : dif;
-      o[3] = sat(0.2f + dif * 0.1773f);
-      o[4] = sat(0.2f + dif * 0.5773f);
-      o[5] = sat(0.2f + dif * 0.973f);
-    }
-    wave_sync();
+    const LeafGrid lg = dice_leaf<MAXP>(a, lcp, lmeta, lqloc, qoff, leaf_at[g], s_cp, s_row, s_pt, lane);
+    const uint32_t qbase = lg.qbase;
+    const int gu = lg.gu, gv = lg.gv;
     const int nq = gu * gv;
     for (int q0 = 0; q0 < nq; q0 += 64) {
       // one lane per micropolygon: flat depth and colour, pixel box
@@ -594,6 +621,451 @@ __global__ __launch_bounds__(64) void k_reyes_dice(ReyesArgs a, const float4 *__
       }
       wave_sync();
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tiled sampling.  The dice kernel above composites every covered sample
+// with a 64-bit atomicMin in a W x H x spp buffer; on this GPU those atomics
+// are performed at the memory side.  Here the dice kernel instead keeps each
+// micropolygon (grid points, flat depth, colour, pixel box) and the leaf's
+// pixel box; the leaves are listed per screen tile, and workgroups take the
+// same minimum of the same keys over the same (micropolygon, sample) pairs in
+// LDS, then resolve the tile's pixels.  A tile with more than kPartLeaves
+// leaves is split into parts run by separate workgroups; their LDS keys meet
+// in the global buffer (one atomicMin per covered sample and part) and the
+// last part to finish resolves the tile.  The minimum does not depend on the
+// order of the leaves or of the parts, so the frame is the same bit for bit.
+
+constexpr int kMaxSampleTiles = 8192;  // tiles per screen (LDS histograms of the binning)
+constexpr int kLeafChunk = 4096;       // leaves per binning block
+constexpr int kBinThreads = 1024;
+constexpr int kTileWaves = 4;          // waves per tile workgroup
+constexpr int kPartLeaves = 64;        // leaves per tile part (at most 64 per wave); 32 and 128 were slower
+static_assert(kPartLeaves <= 64 * kTileWaves, "one batch of leaf records per wave");
+
+// Tiles covered by the pixel box [x0, x1) x [y0, y1) (non-empty).
+struct TileRange {
+  int tx0, ty0, ntx, nty;
+};
+__device__ __forceinline__ TileRange tile_range(uint4 info, int tsb) {
+  const int x0 = (int)(info.z & 0xffffu), y0 = (int)(info.z >> 16);
+  const int x1 = (int)(info.w & 0xffffu), y1 = (int)(info.w >> 16);
+  TileRange r;
+  if (x0 >= x1) {
+    r.tx0 = r.ty0 = r.ntx = r.nty = 0;
+    return r;
+  }
+  r.tx0 = x0 >> tsb;
+  r.ty0 = y0 >> tsb;
+  r.ntx = ((x1 - 1) >> tsb) - r.tx0 + 1;
+  r.nty = ((y1 - 1) >> tsb) - r.ty0 + 1;
+  return r;
+}
+
+// Grid point (iu, jv) of a leaf whose first micropolygon has order index
+// qbase sits at qbase * pt_num / 4 + iu * (gv + 1) + jv.  A leaf of
+// gu x gv micropolygons has (gu + 1)(gv + 1) grid points, at most 9/4 of
+// gu * gv for gu, gv >= 2 and 4 times it for gu = gv = 1 (pt_num 9 or 16),
+// and its successor's first order index is qbase + gu * gv, so the leaves'
+// point runs do not overlap.
+__device__ __forceinline__ size_t point_base(uint32_t qbase, uint32_t pt_num) {
+  return (size_t)(((unsigned long long)qbase * pt_num) >> 2);
+}
+
+template <int MAXP>
+__global__ __launch_bounds__(64) void k_reyes_dice_bin(
+    ReyesArgs a, const float4 *__restrict__ lcp, const uint32_t *__restrict__ lmeta,
+    const uint32_t *__restrict__ lqloc, const uint32_t *__restrict__ leaf_at, const uint32_t *__restrict__ qoff,
+    const ReyesState *__restrict__ st, float *__restrict__ qcol, float2 *__restrict__ pxy, float *__restrict__ qz,
+    ushort4 *__restrict__ qbox, uint4 *__restrict__ linfo) {
+  __shared__ float4 s_cp[16];
+  __shared__ float4 s_row[MAXP][8];
+  __shared__ float s_pt[MAXP * MAXP][6];
+  const int lane = threadIdx.x;
+  if (st->overflow & 4u) return;  // quad colours would not fit: the frame is re-run with room
+  const unsigned long long nleaf = st->nleaf;
+  const int W = a.W, H = a.H;
+  for (unsigned long long g = blockIdx.x; g < nleaf; g += gridDim.x) {
+    const LeafGrid lg = dice_leaf<MAXP>(a, lcp, lmeta, lqloc, qoff, leaf_at[g], s_cp, s_row, s_pt, lane);
+    const uint32_t qbase = lg.qbase;
+    const int gu = lg.gu, gv = lg.gv;
+    const size_t pbase = point_base(qbase, a.pt_num);
+    const int np = (gu + 1) * (gv + 1);
+    for (int pt = lane; pt < np; pt += 64) pxy[pbase + pt] = make_float2(s_pt[pt][0], s_pt[pt][1]);
+    int bx0 = 0x7fffffff, by0 = 0x7fffffff, bx1 = 0, by1 = 0;  // union of the pixel boxes
+    const int nq = gu * gv;
+    for (int qd = lane; qd < nq; qd += 64) {
+      const int k = qd / gv, j = qd - k * gv;
+      const float *c0 = s_pt[k * (gv + 1) + j], *c1 = s_pt[(k + 1) * (gv + 1) + j];
+      const float *c2 = s_pt[(k + 1) * (gv + 1) + j + 1], *c3 = s_pt[k * (gv + 1) + j + 1];
+      const float X[4] = {c0[0], c1[0], c2[0], c3[0]}, Y[4] = {c0[1], c1[1], c2[1], c3[1]};
+      const uint32_t order = qbase + (uint32_t)qd;
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch)  // shade.pikostage:130-138
+        qcol[(size_t)order * 3 + ch] = fmn(((c0[3 + ch] + c1[3 + ch]) + c2[3 + ch]) + c3[3 + ch], 4.0f) * 0.25f;
+      qz[order] = (((c0[2] + c1[2]) + c2[2]) + c3[2]) * 0.25f;  // shade.pikostage:98
+      int x0, y0, x1, y1;
+      quad_pixels(X, Y, W, H, &x0, &y0, &x1, &y1);
+      qbox[order] = make_ushort4((unsigned short)x0, (unsigned short)y0, (unsigned short)x1, (unsigned short)y1);
+      if (x0 < x1) {
+        bx0 = min(bx0, x0);
+        by0 = min(by0, y0);
+        bx1 = max(bx1, x1);
+        by1 = max(by1, y1);
+      }
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+      bx0 = min(bx0, __shfl_xor(bx0, d));
+      by0 = min(by0, __shfl_xor(by0, d));
+      bx1 = max(bx1, __shfl_xor(bx1, d));
+      by1 = max(by1, __shfl_xor(by1, d));
+    }
+    if (bx0 >= bx1) bx0 = by0 = bx1 = by1 = 0;  // no micropolygon covers a pixel
+    if (lane == 0)
+      linfo[g] = make_uint4(qbase, (uint32_t)gu | ((uint32_t)gv << 8), (uint32_t)bx0 | ((uint32_t)by0 << 16),
+                            (uint32_t)bx1 | ((uint32_t)by1 << 16));
+    wave_sync();
+  }
+}
+
+// Tile histogram of one chunk of consecutive leaves in LDS.  Returns the
+// chunk's first leaf and its leaf count (0 past the last leaf).
+__device__ __forceinline__ int chunk_histogram(const ReyesArgs &a, unsigned long long nleaf,
+                                               const uint4 *__restrict__ linfo, uint32_t *s_h,
+                                               unsigned long long *g0) {
+  const int nt = a.tnx * a.tny;
+  *g0 = (unsigned long long)blockIdx.x * kLeafChunk;
+  const int n = *g0 < nleaf ? (int)min((unsigned long long)kLeafChunk, nleaf - *g0) : 0;
+  for (int k = threadIdx.x; k < nt; k += blockDim.x) s_h[k] = 0;
+  __syncthreads();
+  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+    const TileRange r = tile_range(linfo[*g0 + l], a.tsb);
+    for (int ty = r.ty0; ty < r.ty0 + r.nty; ++ty)
+      for (int tx = r.tx0; tx < r.tx0 + r.ntx; ++tx) atomicAdd(&s_h[ty * a.tnx + tx], 1u);
+  }
+  __syncthreads();
+  return n;
+}
+
+// Leaves per tile: one global add per (chunk, tile) with leaves.
+__global__ __launch_bounds__(kBinThreads) void k_reyes_tile_count(ReyesArgs a, const ReyesState *__restrict__ st,
+                                                                  const uint4 *__restrict__ linfo,
+                                                                  uint32_t *__restrict__ tcnt) {
+  __shared__ uint32_t s_h[kMaxSampleTiles];
+  if (st->overflow & 4u) return;
+  unsigned long long g0;
+  if (chunk_histogram(a, st->nleaf, linfo, s_h, &g0) == 0) return;
+  for (int k = threadIdx.x; k < a.tnx * a.tny; k += blockDim.x)
+    if (s_h[k]) atomicAdd(&tcnt[k], s_h[k]);
+}
+
+// Exclusive scans over the tiles (one block) of the leaf counts (list
+// offsets and scatter cursors) and of the parts (at least one per tile, so
+// every pixel is written), and the tile of every part; the counts are
+// cleared for the next frame.
+__global__ __launch_bounds__(kBinThreads) void k_reyes_tile_scan(uint32_t *__restrict__ tcnt,
+                                                                 uint32_t *__restrict__ toff,
+                                                                 uint32_t *__restrict__ tcur,
+                                                                 uint32_t *__restrict__ ioff,
+                                                                 uint32_t *__restrict__ ptile, int nt,
+                                                                 uint32_t pair_cap, ReyesState *__restrict__ st) {
+  __shared__ unsigned long long sh[2][kBinThreads / 64];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int run = (nt + kBinThreads - 1) / kBinThreads;
+  const int w0 = min(t * run, nt), w1 = min(w0 + run, nt);
+  unsigned long long sum = 0, parts = 0;
+  for (int w = w0; w < w1; ++w) {
+    const uint32_t c = tcnt[w];
+    sum += c;
+    parts += c > (uint32_t)kPartLeaves ? (c + kPartLeaves - 1) / kPartLeaves : 1u;
+  }
+  const unsigned long long inc = wave_incl_scan<unsigned long long>(sum, lane);
+  const unsigned long long pinc = wave_incl_scan<unsigned long long>(parts, lane);
+  if (lane == 63) {
+    sh[0][wv] = inc;
+    sh[1][wv] = pinc;
+  }
+  __syncthreads();
+  unsigned long long o = inc - sum, po = pinc - parts, total = 0, ptotal = 0;
+#pragma unroll
+  for (int k = 0; k < kBinThreads / 64; ++k) {
+    if (k < wv) {
+      o += sh[0][k];
+      po += sh[1][k];
+    }
+    total += sh[0][k];
+    ptotal += sh[1][k];
+  }
+  const bool fits = total <= (unsigned long long)pair_cap;
+  for (int w = w0; w < w1; ++w) {
+    const uint32_t c = tcnt[w];
+    const uint32_t np = c > (uint32_t)kPartLeaves ? (c + kPartLeaves - 1) / kPartLeaves : 1u;
+    toff[w] = (uint32_t)o;
+    tcur[w] = (uint32_t)o;
+    ioff[w] = (uint32_t)po;
+    if (fits)  // then the parts fit their table (nt + pair_cap / kPartLeaves entries)
+      for (uint32_t q = 0; q < np; ++q) ptile[po + q] = (uint32_t)w;
+    tcnt[w] = 0;
+    o += c;
+    po += np;
+  }
+  if (t == 0) {
+    toff[nt] = (uint32_t)total;
+    ioff[nt] = fits ? (uint32_t)ptotal : 0u;
+    st->npairs = total > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)total;
+    if (!fits) atomicOr(&st->overflow, 8u);
+  }
+}
+
+// Every leaf into the list of each tile its pixel box touches: one global
+// add per (chunk, tile) reserves the chunk's run of the tile's list, LDS
+// cursors fill it.  The order within a list is immaterial.
+__global__ __launch_bounds__(kBinThreads) void k_reyes_tile_scatter(ReyesArgs a, const ReyesState *__restrict__ st,
+                                                                    const uint4 *__restrict__ linfo,
+                                                                    uint32_t *__restrict__ tcur,
+                                                                    uint32_t *__restrict__ tlist) {
+  __shared__ uint32_t s_h[kMaxSampleTiles];
+  if (st->overflow & 12u) return;
+  unsigned long long g0;
+  const int n = chunk_histogram(a, st->nleaf, linfo, s_h, &g0);
+  if (n == 0) return;
+  for (int k = threadIdx.x; k < a.tnx * a.tny; k += blockDim.x) {
+    const uint32_t c = s_h[k];
+    if (c) s_h[k] = atomicAdd(&tcur[k], c);
+  }
+  __syncthreads();
+  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+    const TileRange r = tile_range(linfo[g0 + l], a.tsb);
+    for (int ty = r.ty0; ty < r.ty0 + r.nty; ++ty)
+      for (int tx = r.tx0; tx < r.tx0 + r.ntx; ++tx)
+        tlist[atomicAdd(&s_h[ty * a.tnx + tx], 1u)] = (uint32_t)(g0 + l);
+  }
+}
+
+// One workgroup per tile part.  Consecutive parts go to different XCDs (the
+// hardware deals workgroups out in turn), which spreads the dense tiles of a
+// screen region over the whole GPU; giving each XCD a contiguous run of
+// tiles instead left some XCDs with the empty part of the screen and took
+// 40% longer.  Each wave takes every kTileWaves-th leaf of the part and
+// reads their records at once.  In each round it loads the pixel boxes of up
+// to kTileBatch chunks of 64 micropolygons together and queues in LDS those
+// that touch the tile (order index, leaf, box clipped to the tile).  Every 64
+// queued micropolygons are rebuilt from the stored grid points and their
+// (micropolygon, pixel) pairs run with the dice kernel's fan tests, keeping
+// the minimum key per sample in LDS.  The tile's pixels are resolved as
+// k_reyes_resolve does, by the only part or by the last of several.
+constexpr int kTileBatch = 4;
+constexpr int kTileQueue = 64 * (kTileBatch + 1);
+
+template <int SPP>
+__global__ __launch_bounds__(64 * kTileWaves) void k_reyes_tile_sample(
+    ReyesArgs a, const ReyesState *__restrict__ st, const uint32_t *__restrict__ toff,
+    const uint32_t *__restrict__ ioff, const uint32_t *__restrict__ ptile, const uint32_t *__restrict__ tlist,
+    const uint4 *__restrict__ linfo, const float2 *__restrict__ pxy, const float *__restrict__ qz,
+    const ushort4 *__restrict__ qbox, const float *__restrict__ qcol, unsigned long long *__restrict__ vis,
+    uint32_t *__restrict__ tdone, float4 *__restrict__ rgba, float *__restrict__ depth, int32_t *__restrict__ sz) {
+  // keys, sample-major: sample s of tile pixel k at s * 2^(2 tsb) + k (neighbouring
+  // pixels in neighbouring banks)
+  extern __shared__ unsigned long long s_vis[];
+  __shared__ float s_qx[kTileWaves][64][4], s_qy[kTileWaves][64][4];
+  __shared__ unsigned long long s_key[kTileWaves][64];
+  __shared__ uint32_t s_box[kTileWaves][64];  // x0 | y0 << 13 | width << 26
+  __shared__ int s_pre[kTileWaves][64];
+  // queue: order index; leaf (in this wave's batch) | clipped box << 6
+  // (x0, y0, x1, y1 relative to the tile, 6 bits each)
+  __shared__ uint32_t s_qord[kTileWaves][kTileQueue], s_qlr[kTileWaves][kTileQueue];
+  __shared__ int s_last;
+  if (st->overflow & 12u) return;  // the frame is re-run with room
+  const int nt = a.tnx * a.tny;
+  const uint32_t item = blockIdx.x;
+  if (item >= ioff[nt]) return;
+  const int t = (int)ptile[item];
+  const int W = a.W, H = a.H, tsb = a.tsb, TS = 1 << tsb;
+  for (int k = threadIdx.x; k < (TS * TS) * SPP; k += blockDim.x) s_vis[k] = kVisEmpty;
+  const uint32_t part = item - ioff[t], nparts = ioff[t + 1] - ioff[t];
+  const int ty = t / a.tnx, tx = t - ty * a.tnx;
+  const int rx0 = tx << tsb, ry0 = ty << tsb, rx1 = min(rx0 + TS, W), ry1 = min(ry0 + TS, H);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t lbeg = toff[t] + part * (uint32_t)kPartLeaves;
+  const uint32_t lend = min(lbeg + (uint32_t)kPartLeaves, toff[t + 1]);
+  // this wave's leaves: lbeg + wv + kTileWaves * m, m < nmine (at most 64)
+  const int nmine = lbeg + (uint32_t)wv < lend ? (int)((lend - lbeg - (uint32_t)wv + kTileWaves - 1) / kTileWaves) : 0;
+  uint4 mine = make_uint4(0u, 0u, 0u, 0u);
+  if (lane < nmine) mine = linfo[tlist[lbeg + (uint32_t)wv + (uint32_t)(kTileWaves * lane)]];
+  __syncthreads();  // s_vis is cleared
+  uint32_t *qord = s_qord[wv], *qlr = s_qlr[wv];
+  int qn = 0;         // queued micropolygons
+  int m = 0, q0 = 0;  // the next chunk: micropolygons q0.. of leaf m
+  while (m < nmine || qn > 0) {
+    ushort4 b[kTileBatch];
+    uint32_t ord[kTileBatch], lf[kTileBatch];
+    bool has[kTileBatch];
+#pragma unroll
+    for (int r = 0; r < kTileBatch; ++r) {
+      has[r] = false;
+      if (m < nmine) {
+        const uint32_t qbase = __shfl(mine.x, m), grid = __shfl(mine.y, m);
+        const int nq = (int)(grid & 0xffu) * (int)((grid >> 8) & 0xffu);
+        const int qd = q0 + lane;
+        if (qd < nq) {
+          ord[r] = qbase + (uint32_t)qd;
+          b[r] = qbox[ord[r]];
+          lf[r] = (uint32_t)m;
+          has[r] = true;
+        }
+        q0 += 64;
+        if (q0 >= nq) {
+          ++m;
+          q0 = 0;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kTileBatch; ++r) {
+      bool hit = false;
+      uint32_t lr = 0;
+      if (has[r]) {
+        const int x0 = max((int)b[r].x, rx0), y0 = max((int)b[r].y, ry0);
+        const int x1 = min((int)b[r].z, rx1), y1 = min((int)b[r].w, ry1);
+        if (x0 < x1 && y0 < y1) {
+          hit = true;
+          lr = lf[r] | ((uint32_t)(x0 - rx0) << 6) | ((uint32_t)(y0 - ry0) << 12) | ((uint32_t)(x1 - rx0) << 18) |
+               ((uint32_t)(y1 - ry0) << 24);
+        }
+      }
+      const unsigned long long mask = __ballot(hit);
+      if (hit) {
+        const int at = qn + __popcll(mask & ((1ull << lane) - 1ull));
+        qord[at] = ord[r];
+        qlr[at] = lr;
+      }
+      qn += __popcll(mask);
+    }
+    wave_sync();
+    while (qn >= 64 || (m >= nmine && qn > 0)) {  // 64 queued at a time; the rest once no leaf is left
+      const int n = min(qn, 64);
+      const uint32_t order = lane < n ? qord[lane] : 0u, lr = lane < n ? qlr[lane] : 0u;
+      const int mi = (int)(lr & 63u);
+      const uint32_t qbase = __shfl(mine.x, mi), grid = __shfl(mine.y, mi);
+      int x0 = 0, y0 = 0, wid = 0, npx = 0;
+      if (lane < n) {
+        const int gv = (int)((grid >> 8) & 0xffu), qd = (int)(order - qbase);
+        const size_t pbase = point_base(qbase, a.pt_num);
+        const int k = qd / gv, j = qd - k * gv;
+        const float2 c0 = pxy[pbase + k * (gv + 1) + j], c1 = pxy[pbase + (k + 1) * (gv + 1) + j];
+        const float2 c2 = pxy[pbase + (k + 1) * (gv + 1) + j + 1], c3 = pxy[pbase + k * (gv + 1) + j + 1];
+        const float z = qz[order];
+        s_qx[wv][lane][0] = c0.x;
+        s_qx[wv][lane][1] = c1.x;
+        s_qx[wv][lane][2] = c2.x;
+        s_qx[wv][lane][3] = c3.x;
+        s_qy[wv][lane][0] = c0.y;
+        s_qy[wv][lane][1] = c1.y;
+        s_qy[wv][lane][2] = c2.y;
+        s_qy[wv][lane][3] = c3.y;
+        s_key[wv][lane] = ((unsigned long long)(__float_as_uint(z) ^ 0x80000000u) << 32) |
+                          (unsigned long long)(0xFFFFFFFEu - order);
+        x0 = rx0 + (int)((lr >> 6) & 63u);
+        y0 = ry0 + (int)((lr >> 12) & 63u);
+        wid = rx0 + (int)((lr >> 18) & 63u) - x0;
+        npx = wid * (ry0 + (int)((lr >> 24) & 63u) - y0);
+      }
+      const int inc = wave_incl_scan<int>(npx, lane);
+      const int total = __shfl(inc, 63);
+      s_box[wv][lane] = (uint32_t)x0 | ((uint32_t)y0 << 13) | ((uint32_t)wid << 26);
+      s_pre[wv][lane] = inc - npx;
+      wave_sync();
+      for (int p = lane; p < total; p += 64) {
+        int q = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+          if (q + step < n && s_pre[wv][q + step] <= p) q += step;
+        const uint32_t bx = s_box[wv][q];
+        const int local = p - s_pre[wv][q], w = (int)(bx >> 26);
+        const int dy = local / w, dx = local - dy * w;
+        const int x = (int)(bx & 0x1fffu) + dx, y = (int)((bx >> 13) & 0x1fffu) + dy;
+        const float X[4] = {s_qx[wv][q][0], s_qx[wv][q][1], s_qx[wv][q][2], s_qx[wv][q][3]};
+        const float Y[4] = {s_qy[wv][q][0], s_qy[wv][q][1], s_qy[wv][q][2], s_qy[wv][q][3]};
+        const unsigned long long key = s_key[wv][q];
+        unsigned long long *dst = s_vis + (((y - ry0) << tsb) + (x - rx0));
+#pragma unroll
+        for (int smp = 0; smp < SPP; ++smp)
+          if (quad_hit(X, Y, (float)x + sample_off(SPP, smp, 0), (float)y + sample_off(SPP, smp, 1)))
+            atomicMin(dst + (smp << (2 * tsb)), key);
+      }
+      // the queue's remainder to its front (each pass reads 64 entries past
+      // the ones it writes)
+      for (int e = lane; e < qn - n; e += 64) {
+        const uint32_t o = qord[e + n], l = qlr[e + n];
+        qord[e] = o;
+        qlr[e] = l;
+      }
+      qn -= n;
+      wave_sync();
+    }
+  }
+  __syncthreads();
+  if (nparts > 1) {
+    // meet the other parts in the global buffer; the last part resolves
+    for (int k = threadIdx.x; k < (TS * TS) * SPP; k += blockDim.x) {
+      const unsigned long long key = s_vis[k];
+      if (key != kVisEmpty) {
+        const int s = k >> (2 * tsb), px = k & (TS * TS - 1);
+        const size_t p = (size_t)(ry0 + (px >> tsb)) * W + (size_t)(rx0 + (px & (TS - 1)));
+        atomicMin(vis + p * SPP + s, key);
+      }
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = atomicAdd(&tdone[t], 1u) == nparts - 1;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    // the merged keys, and the words reset for the next frame
+    for (int k = threadIdx.x; k < (TS * TS) * SPP; k += blockDim.x) {
+      const int s = k >> (2 * tsb), px = k & (TS * TS - 1);
+      const int x = rx0 + (px & (TS - 1)), y = ry0 + (px >> tsb);
+      if (x < rx1 && y < ry1) s_vis[k] = atomicExch(vis + ((size_t)y * W + x) * SPP + s, kVisEmpty);
+    }
+    if (threadIdx.x == 0) atomicExch(&tdone[t], 0u);
+    __syncthreads();
+  }
+  // PikoScreen clear colour 0xff663313 through fromABGR (math.h:323-330)
+  const float clr[3] = {(float)0x13 / 255.0f, (float)0x33 / 255.0f, (float)0x66 / 255.0f};
+  for (int k = threadIdx.x; k < TS * TS; k += blockDim.x) {
+    const int x = rx0 + (k & (TS - 1)), y = ry0 + (k >> tsb);
+    if (x >= rx1 || y >= ry1) continue;
+    const size_t p = (size_t)y * W + x;
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    int32_t zmin = 0;
+#pragma unroll
+    for (int s = 0; s < SPP; ++s) {
+      const unsigned long long key = s_vis[(s << (2 * tsb)) + k];
+      const uint32_t low = (uint32_t)key;
+      float c[3];
+      int32_t zi;
+      if (low == 0xFFFFFFFFu) {
+        c[0] = clr[0];
+        c[1] = clr[1];
+        c[2] = clr[2];
+        zi = 0x3F800000;
+      } else {
+        const unsigned long long order = 0xFFFFFFFEull - low;
+        c[0] = qcol[order * 3 + 0];
+        c[1] = qcol[order * 3 + 1];
+        c[2] = qcol[order * 3 + 2];
+        zi = (int32_t)((uint32_t)(key >> 32) ^ 0x80000000u);
+      }
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) acc[ch] = s == 0 ? c[ch] : acc[ch] + c[ch];
+      zmin = (s == 0 || zi < zmin) ? zi : zmin;
+      if (a.keep_samples) sz[p * SPP + s] = zi;
+    }
+    const float inv = 1.0f / (float)SPP;
+    rgba[p] = make_float4(acc[0] * inv, acc[1] * inv, acc[2] * inv, 1.0f);
+    depth[p] = __int_as_float(zmin);
   }
 }
 
@@ -667,6 +1139,16 @@ struct piko_reyes_pipe {
   float4 *d_rgba = nullptr;
   float *d_depth = nullptr;
   int32_t *d_sz = nullptr;
+  // tiled sampling
+  bool tiled = false;
+  float2 *d_pxy = nullptr;     // grid point screen positions
+  float *d_qz = nullptr;       // flat depth per micropolygon (order index)
+  ushort4 *d_qbox = nullptr;   // pixel box per micropolygon
+  uint4 *d_linfo = nullptr;    // per leaf (emission rank): first order index, grid, pixel box
+  uint32_t *d_tcnt = nullptr, *d_toff = nullptr, *d_tcur = nullptr, *d_tlist = nullptr;
+  uint32_t *d_ioff = nullptr, *d_tdone = nullptr;  // parts per tile (scanned), finished parts
+  uint32_t *d_ptile = nullptr;                      // tile of every part
+  long long tlist_bytes = 0;
   int cus = 256, dice_grid = 0;
   int reruns = 0;  // capacity re-runs of the last run call
   long long device_bytes = 0;
@@ -713,6 +1195,7 @@ void free_leaves(piko_reyes_pipe *p) {
   rfree(p->d_lmeta);
   rfree(p->d_lqloc);
   rfree(p->d_leaf_at);
+  rfree(p->d_linfo);
   p->device_bytes -= p->leaf_bytes;
   p->leaf_bytes = 0;
 }
@@ -723,18 +1206,51 @@ int alloc_leaves(piko_reyes_pipe *p) {
   if (rc == PIKO_OK) rc = ralloc(p, &p->d_lmeta, n, &p->leaf_bytes);
   if (rc == PIKO_OK) rc = ralloc(p, &p->d_lqloc, n, &p->leaf_bytes);
   if (rc == PIKO_OK) rc = ralloc(p, &p->d_leaf_at, n, &p->leaf_bytes);
+  if (rc == PIKO_OK && p->tiled) rc = ralloc(p, &p->d_linfo, n, &p->leaf_bytes);
   return rc;
 }
 
 void free_qcol(piko_reyes_pipe *p) {
   rfree(p->d_qcol);
+  rfree(p->d_pxy);
+  rfree(p->d_qz);
+  rfree(p->d_qbox);
   p->device_bytes -= p->qcol_bytes;
   p->qcol_bytes = 0;
 }
 
-int alloc_qcol(piko_reyes_pipe *p) { return ralloc(p, &p->d_qcol, (long long)p->ra.qcap * 3, &p->qcol_bytes); }
+int alloc_qcol(piko_reyes_pipe *p) {
+  const long long q = (long long)p->ra.qcap;
+  int rc = ralloc(p, &p->d_qcol, q * 3, &p->qcol_bytes);
+  if (rc == PIKO_OK && p->tiled) rc = ralloc(p, &p->d_pxy, ((q * p->ra.pt_num) >> 2) + 1, &p->qcol_bytes);
+  if (rc == PIKO_OK && p->tiled) rc = ralloc(p, &p->d_qz, q, &p->qcol_bytes);
+  if (rc == PIKO_OK && p->tiled) rc = ralloc(p, &p->d_qbox, q, &p->qcol_bytes);
+  return rc;
+}
+
+// Parts of tiled sampling: at most one per tile plus one per kPartLeaves
+// list entries.
+long long max_parts(const ReyesArgs &a) {
+  return (long long)a.tnx * a.tny + ((long long)a.pair_cap + kPartLeaves - 1) / kPartLeaves;
+}
+
+void free_tlist(piko_reyes_pipe *p) {
+  rfree(p->d_tlist);
+  rfree(p->d_ptile);
+  p->device_bytes -= p->tlist_bytes;
+  p->tlist_bytes = 0;
+}
+
+int alloc_tlist(piko_reyes_pipe *p) {
+  int rc = ralloc(p, &p->d_tlist, (long long)p->ra.pair_cap, &p->tlist_bytes);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->d_ptile, max_parts(p->ra), &p->tlist_bytes);
+  return rc;
+}
 
 const char *kReyesKernels[] = {"k_reyes_split", "k_reyes_levels+k_reyes_rank", "k_reyes_dice", "k_reyes_resolve"};
+const char *kReyesTiledKernels[] = {"k_reyes_split", "k_reyes_levels+k_reyes_rank",
+                                    "k_reyes_dice_bin+k_reyes_tile_count+k_reyes_tile_scan+k_reyes_tile_scatter",
+                                    "k_reyes_tile_sample"};
 
 int enqueue_frame(piko_reyes_pipe *p, hipEvent_t *ev) {
   const ReyesArgs &a = p->ra;
@@ -748,10 +1264,39 @@ int enqueue_frame(piko_reyes_pipe *p, hipEvent_t *ev) {
                                                         a.G, a.leaf_cap, a.qcap, p->d_off, p->d_qoff, p->d_leaf_at,
                                                         p->d_st);
   if (ev) (void)hipEventRecord(ev[1], p->stream);
+  const bool small = a.dice > 0 && a.dice <= 8;
+  if (p->tiled) {
+    const int nt = a.tnx * a.tny;
+#define PIKO_DICE_BIN(MAXP)                                                                                   \
+  k_reyes_dice_bin<MAXP><<<p->dice_grid, 64, 0, p->stream>>>(a, p->d_lcp, p->d_lmeta, p->d_lqloc, p->d_leaf_at, \
+                                                             p->d_qoff, p->d_st, p->d_qcol, p->d_pxy, p->d_qz,   \
+                                                             p->d_qbox, p->d_linfo)
+    if (small) PIKO_DICE_BIN(9);
+    else PIKO_DICE_BIN(34);
+#undef PIKO_DICE_BIN
+    const unsigned chunks = (unsigned)cdiv((long long)a.G * a.leaf_cap, kLeafChunk);
+    k_reyes_tile_count<<<chunks, kBinThreads, 0, p->stream>>>(a, p->d_st, p->d_linfo, p->d_tcnt);
+    k_reyes_tile_scan<<<1, kBinThreads, 0, p->stream>>>(p->d_tcnt, p->d_toff, p->d_tcur, p->d_ioff, p->d_ptile, nt,
+                                                       a.pair_cap, p->d_st);
+    k_reyes_tile_scatter<<<chunks, kBinThreads, 0, p->stream>>>(a, p->d_st, p->d_linfo, p->d_tcur, p->d_tlist);
+    if (ev) (void)hipEventRecord(ev[2], p->stream);
+    const size_t lds = ((size_t)1 << (2 * a.tsb)) * (size_t)a.spp * 8;
+    const unsigned grid = (unsigned)max_parts(a);
+#define PIKO_TILE_SAMPLE(SPP)                                                                                    \
+  k_reyes_tile_sample<SPP><<<grid, 64 * kTileWaves, lds, p->stream>>>(a, p->d_st, p->d_toff, p->d_ioff, p->d_ptile, \
+                                                                      p->d_tlist,                                   \
+                                                                      p->d_linfo, p->d_pxy, p->d_qz, p->d_qbox,    \
+                                                                      p->d_qcol, p->d_vis, p->d_tdone, p->d_rgba,  \
+                                                                      p->d_depth, p->d_sz)
+    if (a.spp == 4) PIKO_TILE_SAMPLE(4);
+    else PIKO_TILE_SAMPLE(1);
+#undef PIKO_TILE_SAMPLE
+    if (ev) (void)hipEventRecord(ev[3], p->stream);
+    return hipGetLastError() == hipSuccess ? PIKO_OK : PIKO_ERR_HIP;
+  }
 #define PIKO_DICE(MAXP, SPP)                                                                                  \
   k_reyes_dice<MAXP, SPP><<<p->dice_grid, 64, 0, p->stream>>>(a, p->d_lcp, p->d_lmeta, p->d_lqloc, p->d_leaf_at, \
                                                               p->d_qoff, p->d_st, p->d_qcol, p->d_vis)
-  const bool small = a.dice > 0 && a.dice <= 8;
   if (a.spp == 4) {
     if (small) PIKO_DICE(9, 4);
     else PIKO_DICE(34, 4);
@@ -796,6 +1341,13 @@ int check_capacity(piko_reyes_pipe *p, int *rerun) {
     p->ra.qcap = std::min<unsigned long long>(s.nquad + s.nquad / 8 + 4096, kMaxOrder);
     if ((rc = alloc_qcol(p)) != PIKO_OK) return rc;
   }
+  if ((s.overflow & 8u) && !(s.overflow & 7u)) {  // the pair count is exact only for a complete dice
+    const unsigned long long want = (unsigned long long)s.npairs + s.npairs / 4 + 4096;
+    if (s.npairs == 0xFFFFFFFFu || want > kMaxPairs) return PIKO_ERR_CAPACITY;
+    free_tlist(p);
+    p->ra.pair_cap = (uint32_t)want;
+    if ((rc = alloc_tlist(p)) != PIKO_OK) return rc;
+  }
   *rerun = 1;
   return PIKO_OK;
 }
@@ -817,6 +1369,15 @@ int make_pipe(piko_reyes_pipe **out, const piko_constant_state *cs, const float 
   if (!valid_dice(opt->dice) || (opt->spp != 1 && opt->spp != 4)) return PIKO_ERR_INVALID;
   if (!(opt->displacement == opt->displacement) || opt->roots_per_wave < 0 || opt->roots_per_wave > 256)
     return PIKO_ERR_INVALID;
+  if (opt->sampling < PIKO_SAMPLE_AUTO || opt->sampling > PIKO_SAMPLE_TILED) return PIKO_ERR_INVALID;
+  if (opt->tile != 0 && opt->tile != 16 && opt->tile != 32) return PIKO_ERR_INVALID;
+  // tiled sampling bins into at most kMaxSampleTiles tiles: 16 pixels (or
+  // the tile asked for), 32 where a screen would have more; a TILED request
+  // for a larger screen is refused
+  int tsb = opt->tile == 32 ? 5 : 4;
+  if (cdiv(W, 1 << tsb) * cdiv(H, 1 << tsb) > kMaxSampleTiles) tsb = 5;
+  const bool tiles_fit = cdiv(W, 1 << tsb) * cdiv(H, 1 << tsb) <= kMaxSampleTiles;
+  if (opt->sampling == PIKO_SAMPLE_TILED && !tiles_fit) return PIKO_ERR_INVALID;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PIKO_ERR_NO_DEVICE;
   piko_reyes_pipe *p = new (std::nothrow) piko_reyes_pipe();
@@ -843,6 +1404,12 @@ int make_pipe(piko_reyes_pipe **out, const piko_constant_state *cs, const float 
   a.qcap = std::min<unsigned long long>((unsigned long long)a.G * a.leaf_cap * per_leaf,
                                         std::max<unsigned long long>(1ull << 24, (unsigned long long)count * 64));
   a.keep_samples = opt->keep_samples ? 1 : 0;
+  a.tsb = tsb;
+  a.tnx = (int)cdiv(W, 1 << tsb);
+  a.tny = (int)cdiv(H, 1 << tsb);
+  p->tiled = opt->sampling == PIKO_SAMPLE_TILED;
+  a.pt_num = a.dice == 1 ? 16u : 9u;
+  a.pair_cap = (uint32_t)std::min<long long>(std::max<long long>(4096, (long long)count * 64), 1ll << 30);
   int rc = PIKO_OK;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, 0) == hipSuccess && prop.multiProcessorCount > 0) p->cus = prop.multiProcessorCount;
@@ -874,14 +1441,26 @@ int make_pipe(piko_reyes_pipe **out, const piko_constant_state *cs, const float 
   if (rc == PIKO_OK) rc = ralloc(p, &p->d_lvltot, 2 * kLevels);
   if (rc == PIKO_OK) rc = ralloc(p, &p->d_wstat, a.G);
   if (rc == PIKO_OK) rc = ralloc(p, &p->d_st, 1);
-  if (rc == PIKO_OK) rc = ralloc(p, &p->d_vis, ns);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->d_vis, ns);  // tiled: where split tiles meet
+  if (rc == PIKO_OK && p->tiled) {
+    const long long nt = (long long)a.tnx * a.tny;
+    rc = ralloc(p, &p->d_tcnt, nt);
+    if (rc == PIKO_OK) rc = ralloc(p, &p->d_toff, nt + 1);
+    if (rc == PIKO_OK) rc = ralloc(p, &p->d_tcur, nt);
+    if (rc == PIKO_OK) rc = ralloc(p, &p->d_ioff, nt + 1);
+    if (rc == PIKO_OK) rc = ralloc(p, &p->d_tdone, nt);
+    if (rc == PIKO_OK) rc = alloc_tlist(p);
+    if (rc == PIKO_OK && (hipMemsetAsync(p->d_tcnt, 0, (size_t)nt * 4, p->stream) != hipSuccess ||
+                          hipMemsetAsync(p->d_tdone, 0, (size_t)nt * 4, p->stream) != hipSuccess))
+      rc = PIKO_ERR_HIP;
+  }
   if (rc == PIKO_OK) rc = ralloc(p, &p->d_rgba, npix);
   if (rc == PIKO_OK) rc = ralloc(p, &p->d_depth, npix);
   if (rc == PIKO_OK) rc = ralloc(p, &p->d_sz, a.keep_samples ? ns : 1);
-  if (rc == PIKO_OK) {
+  if (rc == PIKO_OK)
     k_fill_u64<<<(unsigned)std::min<long long>(cdiv(ns, 256), 4096), 256, 0, p->stream>>>(p->d_vis, ns, kVisEmpty);
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(p->stream) != hipSuccess) rc = PIKO_ERR_HIP;
-  }
+  if (rc == PIKO_OK && (hipGetLastError() != hipSuccess || hipStreamSynchronize(p->stream) != hipSuccess))
+    rc = PIKO_ERR_HIP;
   if (rc != PIKO_OK) {
     piko_reyes_destroy(p);
     return rc;
@@ -957,7 +1536,7 @@ int piko_reyes_profile_frames(piko_reyes_pipe *p, int nframes, float *kernel_ms,
   const int n = std::min(4, cap);
   for (int k = 0; k < n; ++k) {
     kernel_ms[k] = (float)(acc[k] / nframes);
-    if (names) names[k] = kReyesKernels[k];
+    if (names) names[k] = p->tiled ? kReyesTiledKernels[k] : kReyesKernels[k];
   }
   *nk = n;
   if (frame_ms) *frame_ms = (float)(facc / nframes);
@@ -1028,6 +1607,7 @@ int piko_reyes_get_stats(piko_reyes_pipe *p, piko_reyes_stats *st) {
   st->levels = (int)s.levels;
   st->device_bytes = p->device_bytes;
   st->capacity_reruns = p->reruns;
+  st->tile_pairs = p->tiled ? (long long)s.npairs : 0;
   return PIKO_OK;
 }
 
@@ -1046,6 +1626,12 @@ int piko_reyes_destroy(piko_reyes_pipe *p) {
   rfree(p->d_wstat);
   rfree(p->d_st);
   rfree(p->d_vis);
+  rfree(p->d_tcnt);
+  rfree(p->d_toff);
+  rfree(p->d_tcur);
+  rfree(p->d_ioff);
+  rfree(p->d_tdone);
+  free_tlist(p);
   rfree(p->d_rgba);
   rfree(p->d_depth);
   rfree(p->d_sz);

@@ -27,6 +27,7 @@ _lib = None
 PIKO_INPUT_PRETRANSFORMED = 0
 PIKO_INPUT_WORLD = 1
 RASTER_AUTO, RASTER_BINNED, RASTER_DIRECT, RASTER_TILED = 0, 1, 2, 3   # PIKO_RASTER_* (include/piko_hip.h)
+SAMPLE_AUTO, SAMPLE_GLOBAL, SAMPLE_TILED = 0, 1, 2  # PIKO_SAMPLE_* (include/piko_hip.h)
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "no HIP device",
           -4: "queue capacity exceeded", -5: "device out of memory", -6: "admission refused"}
@@ -97,13 +98,14 @@ class ReyesOptions(ctypes.Structure):
     """piko_reyes_options (include/piko_hip.h)."""
     _fields_ = [("threshold", ctypes.c_float), ("max_splits", ctypes.c_int), ("dice", ctypes.c_int),
                 ("spp", ctypes.c_int), ("displacement", ctypes.c_float), ("roots_per_wave", ctypes.c_int),
-                ("keep_samples", ctypes.c_int)]
+                ("keep_samples", ctypes.c_int), ("sampling", ctypes.c_int), ("tile", ctypes.c_int)]
 
 
 class ReyesStats(ctypes.Structure):
     _fields_ = [("patches", ctypes.c_longlong), ("leaves", ctypes.c_longlong),
                 ("micropolygons", ctypes.c_longlong), ("levels", ctypes.c_int),
-                ("device_bytes", ctypes.c_longlong), ("capacity_reruns", ctypes.c_int)]
+                ("device_bytes", ctypes.c_longlong), ("capacity_reruns", ctypes.c_int),
+                ("tile_pairs", ctypes.c_longlong)]
 
 
 def lib_path():
@@ -367,13 +369,16 @@ class ReyesPipe:
     def __init__(self):
         self._h = None
 
-    def allocate(self, scene, roots_per_wave=0, keep_samples=False, max_splits=20):
+    def allocate(self, scene, roots_per_wave=0, keep_samples=False, max_splits=20, sampling=SAMPLE_AUTO, tile=0):
+        """`sampling`: SAMPLE_GLOBAL (per-sample atomics in a visibility
+        buffer), SAMPLE_TILED (per-tile LDS compositing) or SAMPLE_AUTO
+        (global); `tile`: the tiled path's tile edge, 16 or 32 (0 = 16)."""
         if self._h is not None:
             raise RuntimeError("pipe already allocated")
         self.W, self.H, self.spp = scene.W, scene.H, scene.spp
         self.cs = ConstantState.for_scene(scene)
         opt = ReyesOptions(float(scene.threshold), int(max_splits), int(scene.dice), int(scene.spp),
-                           float(scene.disp), int(roots_per_wave), 1 if keep_samples else 0)
+                           float(scene.disp), int(roots_per_wave), 1 if keep_samples else 0, int(sampling), int(tile))
         cp = np.ascontiguousarray(scene.cp, np.float32)
         h = ctypes.c_void_p()
         _check(lib().piko_reyes_allocate(ctypes.byref(h), ctypes.byref(self.cs), _p(cp), int(cp.shape[0]),

@@ -1,7 +1,10 @@
 """GPU parity: the HIP Reyes pipeline (split -> dice -> shade -> resolve)
 against the oracle through the C-ABI, bit for bit: split leaves (control
 points and the reference's nu/nv), every sample's depth, the resolved FP32
-RGBA and the per-pixel depth."""
+RGBA and the per-pixel depth.  Every frame is checked with both sample
+compositing paths: global (one 64-bit atomicMin per covered sample in a
+visibility buffer) and tiled (per-tile LDS visibility, 16- and 32-pixel
+tiles)."""
 import sys
 
 import numpy as np
@@ -21,6 +24,9 @@ def sphere(nlat, nlon, W, H, **kw):
     return scenes.sphere_patches(seed=3, nlat=nlat, nlon=nlon, W=W, H=H, **kw)
 
 
+SAMPLINGS = (("global", 0), ("tiled", 16), ("tiled", 32))
+
+
 def gpu_reyes(scene, **alloc):
     import piko_amd
     with piko_amd.ReyesPipe().allocate(scene, keep_samples=True, **alloc) as pipe:
@@ -29,24 +35,29 @@ def gpu_reyes(scene, **alloc):
         return pipe.color(), pipe.depth(), pipe.samples(), pipe.leaves(), pipe.stats()
 
 
-def assert_reyes_equal(scene, oracle_mod, check_leaves=True, **alloc):
-    log("case P=%d W=%d H=%d thr=%g dice=%d spp=%d disp=%g %s" % (
-        scene.n, scene.W, scene.H, scene.threshold, scene.dice, scene.spp, scene.disp, alloc))
-    rgba, depth, z, (lcp, lnu, lnv), st = gpu_reyes(scene, **alloc)
+def assert_reyes_equal(scene, oracle_mod, check_leaves=True, samplings=SAMPLINGS, **alloc):
+    import piko_amd
     orgba, oz, nup = oracle_mod.reyes_frame(scene)
     if check_leaves:
         ocp, onu, onv = oracle_mod.reyes_split(scene.cp, scene.W, scene.H, scene.view_proj, scene.threshold)
-        assert lcp.shape == ocp.shape, "leaf count %d vs oracle %d" % (lcp.shape[0], ocp.shape[0])
-        assert np.array_equal(lcp.view(np.int32), ocp.view(np.int32)), "leaf control points differ"
-        assert np.array_equal(lnu, onu) and np.array_equal(lnv, onv), "leaf nu/nv differ"
-    assert st["micropolygons"] == nup, "micropolygons %d vs oracle %d" % (st["micropolygons"], nup)
-    bad = np.argwhere(z != oz)
-    assert bad.size == 0, "sample depth differs at %d samples, first %s" % (len(bad), bad[0])
-    bad = np.argwhere(np.any(rgba.view(np.int32) != orgba.view(np.int32), axis=-1))
-    assert bad.size == 0, "colour differs at %d pixels, first %s: gpu %s oracle %s" % (
-        len(bad), bad[0], rgba[tuple(bad[0])], orgba[tuple(bad[0])])
-    assert np.array_equal(depth.view(np.int32), oz.min(axis=2)), "pixel depth differs"
-    assert np.isfinite(rgba).all()
+    for mode, tile in samplings:
+        log("case P=%d W=%d H=%d thr=%g dice=%d spp=%d disp=%g %s sampling=%s tile=%d" % (
+            scene.n, scene.W, scene.H, scene.threshold, scene.dice, scene.spp, scene.disp, alloc, mode, tile))
+        sampling = piko_amd.SAMPLE_TILED if mode == "tiled" else piko_amd.SAMPLE_GLOBAL
+        rgba, depth, z, (lcp, lnu, lnv), st = gpu_reyes(scene, sampling=sampling, tile=tile, **alloc)
+        if check_leaves:
+            assert lcp.shape == ocp.shape, "leaf count %d vs oracle %d" % (lcp.shape[0], ocp.shape[0])
+            assert np.array_equal(lcp.view(np.int32), ocp.view(np.int32)), "leaf control points differ"
+            assert np.array_equal(lnu, onu) and np.array_equal(lnv, onv), "leaf nu/nv differ"
+        assert st["micropolygons"] == nup, "micropolygons %d vs oracle %d" % (st["micropolygons"], nup)
+        bad = np.argwhere(z != oz)
+        assert bad.size == 0, "%s: sample depth differs at %d samples, first %s" % (mode, len(bad), bad[0])
+        bad = np.argwhere(np.any(rgba.view(np.int32) != orgba.view(np.int32), axis=-1))
+        assert bad.size == 0, "%s: colour differs at %d pixels, first %s: gpu %s oracle %s" % (
+            mode, len(bad), bad[0], rgba[tuple(bad[0])], orgba[tuple(bad[0])])
+        assert np.array_equal(depth.view(np.int32), oz.min(axis=2)), "%s: pixel depth differs" % mode
+        assert np.isfinite(rgba).all()
+        assert (st["tile_pairs"] > 0) == (mode == "tiled" and nup > 0)
     return rgba, st
 
 
@@ -86,11 +97,14 @@ def test_roots_per_wave_invariance(oracle_mod, rpw):
         assert st["capacity_reruns"] >= 1
 
 
-def test_frames_repeat_and_camera_change(oracle_mod):
+@pytest.mark.parametrize("mode", ["global", "tiled"])
+def test_frames_repeat_and_camera_change(oracle_mod, mode):
     import piko_amd
     a = sphere(8, 8, 256, 256)
     b = sphere(8, 8, 256, 256, eye=(1.5, -0.5, 2.0))
-    with piko_amd.ReyesPipe().allocate(a, keep_samples=True) as pipe:
+    sampling = piko_amd.SAMPLE_TILED if mode == "tiled" else piko_amd.SAMPLE_GLOBAL
+    log("repeat and camera change, sampling=%s" % mode)
+    with piko_amd.ReyesPipe().allocate(a, keep_samples=True, sampling=sampling) as pipe:
         pipe.prepare()
         pipe.run_frames(3)
         c3 = pipe.color()
@@ -103,6 +117,23 @@ def test_frames_repeat_and_camera_change(oracle_mod):
         assert np.array_equal(pipe.samples(), oz)
 
 
+def test_tiled_lists_grow(oracle_mod):
+    # 8 patches seen from close by: the first (tile, leaf) list capacity
+    # (4096 entries) is too small, so the lists grow and the frame is
+    # re-run, with the same image
+    sc = sphere(2, 4, 480, 270, eye=(0.0, 0.2, 1.6))
+    _, st = assert_reyes_equal(sc, oracle_mod, samplings=(("tiled", 16),), roots_per_wave=8)
+    log("pairs %d leaves %d reruns %d" % (st["tile_pairs"], st["leaves"], st["capacity_reruns"]))
+    assert st["tile_pairs"] > 4096 and st["capacity_reruns"] >= 1
+
+
+def test_tiled_tile_grows_with_screen(oracle_mod):
+    # 2560x1440 in 16-pixel tiles would be 14,400 tiles, more than the 8,192
+    # the binning histograms hold: the tiled path uses 32-pixel tiles there
+    sc = sphere(4, 8, 2560, 1440, spp=1)
+    assert_reyes_equal(sc, oracle_mod, check_leaves=False, samplings=(("tiled", 16),))
+
+
 def test_invalid_options():
     import piko_amd
     sc = sphere(2, 2, 64, 64)
@@ -112,3 +143,10 @@ def test_invalid_options():
             piko_amd.ReyesPipe().allocate(s2)
     with pytest.raises(piko_amd.PikoError):
         piko_amd.ReyesPipe().allocate(sc, max_splits=21)
+    for bad in (dict(sampling=3), dict(sampling=-1), dict(tile=8), dict(tile=64)):
+        with pytest.raises(piko_amd.PikoError):
+            piko_amd.ReyesPipe().allocate(sc, **bad)
+    # more than 8,192 tiles even at 32 pixels: refused for tiled sampling
+    big = sphere(2, 2, 4096, 4096)
+    with pytest.raises(piko_amd.PikoError):
+        piko_amd.ReyesPipe().allocate(big, sampling=piko_amd.SAMPLE_TILED)
