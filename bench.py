@@ -65,6 +65,33 @@ def cpu_baseline(scene, T, S, budget_s=12.0):
                       "single-threaded oracle, host %s" % (len(times), T, med, cpu_model())}
 
 
+def cpu_reference_schedule(scene, T, S, budget_s=6.0):
+    """The oracle on the reference CPU backend's schedule: the raster bins
+    block-partitioned over host threads (CPUBackend.cpp:534-557, which
+    hard-codes 64 threads), at 64 threads and at this host's CPU share
+    (OMP_NUM_THREADS, else the core count).  The vertex stage and bin walk
+    stay serial, as kernel0 is in the reference."""
+    from oracle import oracle
+    oracle.build()
+    share = int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1)
+    runs = []
+    for threads in sorted({64, share}, reverse=True):
+        times = []
+        t_all = time.perf_counter()
+        while True:
+            t0 = time.perf_counter()
+            oracle.frame(scene, S, T, threads=threads)
+            times.append(time.perf_counter() - t0)
+            if time.perf_counter() - t_all > budget_s or len(times) >= 10:
+                break
+        times.sort()
+        med = times[len(times) // 2]
+        runs.append({"threads": threads, "value": round(scene.n / med / 1e6, 4), "unit": "Mtri/s",
+                     "frames": len(times), "median_s": round(med, 4)})
+    return {"kind": "port", "bin_size": T, "runs": runs, "host": cpu_model(),
+            "sample": "whole frames of the same scene; raster bins over host threads"}
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload` from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE measurements (profiles/traffic.json,
@@ -407,6 +434,7 @@ def main():
     if not args.no_cpu:
         log("timing the CPU oracle on a bounded sample")
         out["cpu_baseline"] = cpu_baseline(scene, T, S)
+        out["cpu_reference_schedule"] = cpu_reference_schedule(scene, T, S)
     if args.workload in ("batch", "all"):
         out["batch"] = bench_batch(log)
     if args.workload in ("serve", "all"):
