@@ -273,14 +273,16 @@ def bench_serve_saturated(log, tris_per_scene, nreq=512):
     return out
 
 
-def bench_reyes(log, steps, warmup, cpu=True):
+def bench_reyes(log, steps, warmup, cpu=True, sampling="auto", tile=0):
     """The Reyes configuration: 4,096 bicubic patches tiling a noisy sphere,
-    split below 16 px, 8x8 dice, displacement, 4 samples, 1280x720 FP32."""
+    split below 16 px, 8x8 dice, displacement, 4 samples, 1280x720 FP32.
+    `sampling`: auto, global or tiled (piko_reyes_options.sampling)."""
     import piko_amd
     from piko_amd import scenes
     sc = scenes.sphere_patches()
     alg = scenes.algorithmic_bytes(sc)
-    with piko_amd.ReyesPipe().allocate(sc) as pipe:
+    smode = {"auto": piko_amd.SAMPLE_AUTO, "global": piko_amd.SAMPLE_GLOBAL, "tiled": piko_amd.SAMPLE_TILED}[sampling]
+    with piko_amd.ReyesPipe().allocate(sc, sampling=smode, tile=tile) as pipe:
         pipe.prepare()
         pipe.run_frames(max(warmup, 1))
         t0 = time.perf_counter()
@@ -291,6 +293,7 @@ def bench_reyes(log, steps, warmup, cpu=True):
     dom = max(kernels, key=lambda k: k[1])
     out = {"workload": "reyes-sphere-4096-720p: 64x64 bicubic patches on a unit sphere, radial noise +-0.05 "
                        "(seed 3), split < 16 px, dice 8x8, displacement, 4 spp, 1280x720 FP32 RGBA",
+           "sampling": "tiled" if st["tile_pairs"] > 0 else "global", "tile_pairs": st["tile_pairs"],
            "frames_per_s": round(1.0 / dt, 2), "ms_per_frame": round(dt * 1e3, 4),
            "event_frame_ms": round(frame_ms, 4), "patches": st["patches"], "leaves": st["leaves"],
            "micropolygons": st["micropolygons"], "split_levels": st["levels"],
@@ -326,6 +329,9 @@ def main():
     ap.add_argument("--modes", default="tiled,direct,binned", help="raster modes to sweep")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="all", choices=["raster", "batch", "serve", "reyes", "all"])
+    ap.add_argument("--reyes-sampling", default="auto", choices=["auto", "global", "tiled"],
+                    help="how the Reyes frame composites samples (auto = global)")
+    ap.add_argument("--reyes-tile", type=int, default=0, help="tiled sampling's tile edge: 16 or 32 (0 = 16)")
     ap.add_argument("--serve-tris", type=int, default=300_000, help="triangles per resident scene when serving")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -441,7 +447,8 @@ def main():
         out["serve"] = bench_serve(log, args.serve_tris, args.serve_requests)
         out["serve_saturated"] = bench_serve_saturated(log, args.serve_tris)
     if args.workload in ("reyes", "all"):
-        out["reyes"] = bench_reyes(log, args.steps, args.warmup, cpu=not args.no_cpu)
+        out["reyes"] = bench_reyes(log, args.steps, args.warmup, cpu=not args.no_cpu, sampling=args.reyes_sampling,
+                                   tile=args.reyes_tile)
     print(json.dumps(out), flush=True)
 
 
