@@ -1061,7 +1061,11 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
   }
 }
 
-// One wave per 8x8 tile of a large triangle, over all waves of the grid.
+// One wave per 8x8 tile of a large triangle.  Each wave of the grid takes a
+// contiguous run of the frame's tiles and walks it in order, so it looks its
+// triangle up (and, in a batch, the triangle's job) once per run and again
+// only where the run enters the next triangle's tiles.  Tiles handed out one
+// at a time over the grid repeated both searches for every tile.
 template <bool MULTI>
 __global__ void __launch_bounds__(256) k_large_raster(const int4 *__restrict__ geo,
                                                       const uint32_t *__restrict__ large,
@@ -1075,30 +1079,53 @@ __global__ void __launch_bounds__(256) k_large_raster(const int4 *__restrict__ g
   const int nl = (int)nl32;
   const uint32_t total = (uint32_t)(acc >> 32);
   const int lane = threadIdx.x & 63;
-  const uint32_t nw = gridDim.x * (blockDim.x / 64);
-  for (uint32_t g = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); g < total; g += nw) {
-    int lo = 0, hi = nl - 1;  // largest k with tile_off[k] <= g
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (tile_off[mid] <= g) lo = mid;
-      else hi = mid - 1;
+  const unsigned long long nw = (unsigned long long)gridDim.x * (blockDim.x / 64);
+  const unsigned long long w = (unsigned long long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const unsigned long long per = ((unsigned long long)total + nw - 1) / nw;
+  const unsigned long long gbeg = w * per, gend = min(gbeg + per, (unsigned long long)total);
+  if (gbeg >= gend) return;
+  int k = 0, hi = nl - 1;  // largest k with tile_off[k] <= gbeg
+  while (k < hi) {
+    const int mid = (k + hi + 1) >> 1;
+    if (tile_off[mid] <= gbeg) k = mid;
+    else hi = mid - 1;
+  }
+  uint32_t kbeg = 0, kend = 0, tri = 0, tw = 0;
+  int p[6], x0 = 0, y0 = 0, x1 = 0, y1 = 0, S = 0, W = 0;
+  bool sm = false;
+  float rcp = 0.0f, z0mz2 = 0.0f, z1mz2 = 0.0f, z2 = 0.0f;
+  unsigned long long fbb = 0;
+  for (unsigned long long g = gbeg; g < gend; ++g) {
+    if (g == gbeg || g >= kend) {
+      // the triangle whose tiles hold g: the ranges [tile_off[k], tile_off[k + 1])
+      // are consecutive and end at total, so this stops at k < nl with g inside
+      while (k + 1 < nl && tile_off[k + 1] <= g) ++k;
+      kbeg = tile_off[k];
+      kend = k + 1 < nl ? tile_off[k + 1] : total;
+      tri = large[k];
+      const FrameArgs *ja = &fa;
+      if (MULTI) ja = &jobs[job_search(jobs, njobs, tri, false)];
+      S = ja->S;
+      W = ja->W;
+      fbb = MULTI ? ja->fb_base : 0ull;
+      const int4 a = geo[3 * (size_t)tri], b = geo[3 * (size_t)tri + 1], c = geo[3 * (size_t)tri + 2];
+      p[0] = a.x; p[1] = a.y; p[2] = a.z; p[3] = a.w; p[4] = b.x; p[5] = b.y;
+      clip_rect(p, S, W, ja->H, x0, y0, x1, y1);
+      tw = (uint32_t)(x1 - x0 + kLargeTile - 1) / kLargeTile;
+      sm = small_bbox(p, S);
+      rcp = __int_as_float(b.z);
+      z2 = __int_as_float(b.w);
+      z0mz2 = __int_as_float(c.x);
+      z1mz2 = __int_as_float(c.y);
     }
-    const uint32_t tri = large[lo];
-    const FrameArgs &ja = MULTI ? jobs[job_search(jobs, njobs, tri, false)] : fa;
-    const int4 a = geo[3 * (size_t)tri], b = geo[3 * (size_t)tri + 1], c = geo[3 * (size_t)tri + 2];
-    const int p[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
-    int x0, y0, x1, y1;
-    clip_rect(p, ja.S, ja.W, ja.H, x0, y0, x1, y1);
-    const uint32_t tw = (uint32_t)(x1 - x0 + kLargeTile - 1) / kLargeTile;
-    const uint32_t local = g - tile_off[lo];
+    const uint32_t local = (uint32_t)(g - kbeg);
     const int ty = (int)(local / tw), tx = (int)(local - (uint32_t)ty * tw);
     const int px = x0 + tx * kLargeTile + (lane & (kLargeTile - 1)), py = y0 + ty * kLargeTile + lane / kLargeTile;
     if (px >= x1 || py >= y1) continue;
     float alpha, beta;
-    if (covered_at(p, small_bbox(p, ja.S), px, py, ja.S, __int_as_float(b.z), alpha, beta)) {
-      const float z = interp(__int_as_float(c.x), __int_as_float(c.y), __int_as_float(b.w), alpha, beta);
-      const unsigned long long fbb = MULTI ? ja.fb_base : 0ull;
-      atomicMin((long long *)&vis[fbb + (size_t)py * ja.W + px], (long long)vis_key(z, tri));
+    if (covered_at(p, sm, px, py, S, rcp, alpha, beta)) {
+      const float z = interp(z0mz2, z1mz2, z2, alpha, beta);
+      atomicMin((long long *)&vis[fbb + (size_t)py * W + px], (long long)vis_key(z, tri));
     }
   }
 }
