@@ -168,6 +168,27 @@ def test_direct_large_after_frames_without(oracle_mod):
             assert np.array_equal(c, oc) and np.array_equal(d.view(np.int32), od.view(np.int32)), rep
 
 
+def test_direct_frames_across_camera_changes(oracle_mod):
+    # runs of back-to-back direct frames with the camera changed between
+    # runs, large triangles appearing and disappearing (drawn inline after a
+    # run without any, then by the large-triangle launch): each run ends on
+    # the oracle's image
+    import piko_amd
+    far = scenes.heightfield(seed=6, grid=80, W=400, H=240, eye=(0.0, 2.5, 3.0), target=(0.0, 0.0, 0.0))
+    near = scenes.heightfield(seed=6, grid=80, W=400, H=240, eye=(0.0, 0.12, 0.25), target=(0.0, 0.0, -0.3))
+    ofar, odfar, _ = oracle_mod.frame(far, 8, 16)
+    onear, odnear, _ = oracle_mod.frame(near, 8, 16)
+    with piko_amd.RasterPipe().allocate(far, 16, 8, raster_mode=piko_amd.RASTER_DIRECT) as pipe:
+        pipe.prepare()
+        for step, (sc, oc, od, n) in enumerate([(far, ofar, odfar, 4), (near, onear, odnear, 2), (near, onear, odnear, 5),
+                                                (far, ofar, odfar, 3), (near, onear, odnear, 1)]):
+            log("run %d: %d frames" % (step, n))
+            pipe.prepare(sc)
+            pipe.run_frames(n)
+            c, d = pipe.pikoScreen.getData(), pipe.get_depth()
+            assert np.array_equal(c, oc) and np.array_equal(d.view(np.int32), od.view(np.int32)), step
+
+
 def test_tiled_large_lists(oracle_mod):
     # every triangle spans more than 2x2 tiles of 16 pixels: all of them go to
     # the chunks' large lists and are clipped to each tile they touch
