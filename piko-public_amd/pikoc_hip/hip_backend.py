@@ -71,6 +71,12 @@ class HIPBackend:
                 self.dice = int(_macro(ps, "PIKO_REYES_DICE", 0, int))
                 self.spp = int(_macro(ps, "PIKO_REYES_SPP", 1, int))
                 self.disp = float(_macro(ps, "PIKO_REYES_DISPLACEMENT", 0.0))
+                # sample compositing: 0 = auto, 1 = global atomics, 2 = tiled
+                # (piko_reyes_options.sampling); the tiled path's tile edge
+                self.sampling = int(_macro(ps, "PIKO_REYES_SAMPLING", 0, int))
+                self.tile = int(_macro(ps, "PIKO_REYES_TILE", 0, int))
+                if self.sampling not in (0, 1, 2) or self.tile not in (0, 16, 32):
+                    raise BackendError("PIKO_REYES_SAMPLING must be 0, 1 or 2 and PIKO_REYES_TILE 0, 16 or 32")
                 self.f_cp = _field(ps.input_fields, ("cvec4f",), 16, 0)
                 if not self.f_cp:
                     raise BackendError("input %s needs cvec4f CP[16]" % ps.input_type)
@@ -180,6 +186,8 @@ class HIPBackend:
             out.write("  opt.threshold = %rf;\n  opt.max_splits = %d;\n  opt.dice = %d;\n  opt.spp = %d;\n"
                       "  opt.displacement = %rf;\n" % (self.threshold, self.max_splits, self.dice, self.spp,
                                                      self.disp))
+            if self.sampling or self.tile:
+                out.write("  opt.sampling = %d;\n  opt.tile = %d;\n" % (self.sampling, self.tile))
             out.write("  PIKO_CHECK(piko_reyes_allocate(&piko_hip_pipe_, &constState, cp.data(), count, &opt));\n")
         out.write("  pikoScreen.bind(piko_hip_pipe_, &piko_fetch_%s,\n"
                   "                  (size_t)constState.screenSizeX * (size_t)constState.screenSizeY);\n}\n\n"

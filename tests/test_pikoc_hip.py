@@ -155,6 +155,18 @@ def test_emitted_reyes_host_code(tmp_path):
     assert be.kind == "reyes" and be.threshold == 16.0 and be.dice == 8 and be.spp == 4
     h = (tmp_path / "__pikoCompiledPipe.h").read_text()
     assert "piko_reyes_allocate(&piko_hip_pipe_" in h and "opt.dice = 8;" in h and "input[i].CP[k].w" in h
+    assert "opt.sampling" not in h  # unset: the zero-filled options pick auto
+
+
+def test_emitted_reyes_sampling_settings(tmp_path):
+    be = compile_pipe(os.path.join(SAMPLES, "reyes", "reyesPipe.h"), str(tmp_path), [INC],
+                      defines={"PIKO_REYES_SAMPLING": "2", "PIKO_REYES_TILE": "32"}, log=None)
+    assert be.sampling == 2 and be.tile == 32
+    h = (tmp_path / "__pikoCompiledPipe.h").read_text()
+    assert "opt.sampling = 2;" in h and "opt.tile = 32;" in h
+    with pytest.raises(BackendError):
+        compile_pipe(os.path.join(SAMPLES, "reyes", "reyesPipe.h"), str(tmp_path), [INC],
+                     defines={"PIKO_REYES_TILE": "8"}, log=None)
 
 
 def test_cli_exit_codes(tmp_path):
