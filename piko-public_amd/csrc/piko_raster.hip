@@ -882,13 +882,7 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
   const int i = blockIdx.x * kDirThreads + t;
   if (MULTI) stage_jobs(js, jobs, njobs, (unsigned long long)blockIdx.x * kDirThreads, false);
   uint32_t units = 0, tiles = 0;
-  bool big = false;
-  // a large triangle's setup, kept for the wave that rasterises it inline
-  int lp[6] = {0, 0, 0, 0, 0, 0};
-  int lrect = 0, lwh = 0;
-  float lrcp = 0.0f, lz[3] = {0.0f, 0.0f, 0.0f};
-  unsigned long long lfb = 0;
-  int lW = 0;
+  bool big = false;  // listed large triangle; drawn inline from its setup record
   if (i < n) {
     // vertex positions only: the winner's colours are computed in k_resolve
     const FrameArgs &ja = MULTI ? staged_job(js, jobs, njobs, (unsigned long long)i, false) : fa;
@@ -911,15 +905,6 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
       if (listed) {
         big = npx > 0;
         tiles = (uint32_t)((w + kLargeTile - 1) / kLargeTile) * (uint32_t)((h + kLargeTile - 1) / kLargeTile);
-#pragma unroll
-        for (int k = 0; k < 6; ++k) lp[k] = p[k];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) lz[k] = z[k];
-        lrcp = rcp;
-        lrect = x0 | (y0 << 16);
-        lwh = w | (h << 16);
-        lfb = MULTI ? ja.fb_base : 0ull;
-        lW = ja.W;
       } else if (npx > 0) {
         units = (uint32_t)w * ((uint32_t)(h + kSpan - 1) / kSpan);
         // edge functions at the centre of pixel (x0, y0) and their steps;
@@ -968,19 +953,24 @@ __global__ void __launch_bounds__(kDirThreads) k_vs_direct(const float *__restri
       if (!inline_large && (unsigned long long)btile + tsum > 0xFFFFFFFFull) atomicOr(&fs->overflow, 4u);
     }
     if (inline_large) {
+      // the wave's own setup records of its listed triangles (full 48-byte
+      // records, written above): their stores before these loads
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
       for (unsigned long long m = bm; m; m &= m - 1) {
         const int L = __ffsll((long long)m) - 1;
-        int xy[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) xy[k] = __shfl(lp[k], L, 64);
-        const float rcp = __shfl(lrcp, L, 64);
-        const float z2 = __shfl(lz[2], L, 64);
-        const float z0mz2 = __shfl(lz[0], L, 64) - z2, z1mz2 = __shfl(lz[1], L, 64) - z2;
-        const int rect = __shfl(lrect, L, 64), wh = __shfl(lwh, L, 64);
-        const int W = __shfl(lW, L, 64);
-        const unsigned long long fbb = __shfl(lfb, L, 64);
         const uint32_t tri = (uint32_t)(blockIdx.x * kDirThreads + (t & ~63) + L);
-        const int x0 = rect & 0xffff, y0 = rect >> 16, w = wh & 0xffff, h = wh >> 16;
+        const FrameArgs &jl = MULTI ? staged_job(js, jobs, njobs, (unsigned long long)tri, false) : fa;
+        const int4 ga = geo[3 * (size_t)tri], gb = geo[3 * (size_t)tri + 1], gc = geo[3 * (size_t)tri + 2];
+        const int xy[6] = {ga.x, ga.y, ga.z, ga.w, gb.x, gb.y};
+        const float rcp = __int_as_float(gb.z), z2 = __int_as_float(gb.w);
+        const float z0mz2 = __int_as_float(gc.x), z1mz2 = __int_as_float(gc.y);
+        const int W = jl.W;
+        const unsigned long long fbb = MULTI ? jl.fb_base : 0ull;
+        int x0, y0, x1, y1;
+        clip_rect(xy, jl.S, jl.W, jl.H, x0, y0, x1, y1);
+        const int w = x1 - x0, h = y1 - y0;
         const bool sm = small_bbox(xy, S);
         const int tw = (w + kLargeTile - 1) / kLargeTile, th = (h + kLargeTile - 1) / kLargeTile;
         for (int g = 0; g < tw * th; ++g) {
