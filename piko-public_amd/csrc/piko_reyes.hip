@@ -627,9 +627,9 @@ __global__ __launch_bounds__(64) void k_reyes_dice(ReyesArgs a, const float4 *__
 // ---------------------------------------------------------------------------
 // Tiled sampling.  The dice kernel above composites every covered sample
 // with a 64-bit atomicMin in a W x H x spp buffer; on this GPU those atomics
-// are performed at the memory side.  Here the dice kernel instead keeps each
-// micropolygon (grid points, flat depth, colour, pixel box) and the leaf's
-// pixel box; the leaves are listed per screen tile, and workgroups take the
+// are performed at the memory side.  Here k_reyes_dice_bin instead keeps
+// each micropolygon (grid points, flat depth, colour, pixel box) and the
+// leaf's pixel box; the leaves are listed per screen tile, and workgroups take the
 // same minimum of the same keys over the same (micropolygon, sample) pairs in
 // LDS, then resolve the tile's pixels.  A tile with more than kPartLeaves
 // leaves is split into parts run by separate workgroups; their LDS keys meet
@@ -644,7 +644,8 @@ constexpr int kTileWaves = 4;          // waves per tile workgroup
 constexpr int kPartLeaves = 64;        // leaves per tile part (at most 64 per wave); 32 and 128 were slower
 static_assert(kPartLeaves <= 64 * kTileWaves, "one batch of leaf records per wave");
 
-// Tiles covered by the pixel box [x0, x1) x [y0, y1) (non-empty).
+// Tiles covered by a leaf record's pixel box [x0, x1) x [y0, y1) (none when
+// the box is empty).
 struct TileRange {
   int tx0, ty0, ntx, nty;
 };

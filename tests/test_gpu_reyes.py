@@ -134,6 +134,37 @@ def test_tiled_tile_grows_with_screen(oracle_mod):
     assert_reyes_equal(sc, oracle_mod, check_leaves=False, samplings=(("tiled", 16),))
 
 
+@pytest.mark.parametrize("mode", ["global", "tiled"])
+def test_no_patches(mode):
+    # an empty patch list: every pixel is the clear colour (fromABGR of
+    # 0xff663313) at depth 1.0, on both sampling paths
+    import piko_amd
+    sc = sphere(2, 2, 96, 64)
+    sc.cp = sc.cp[:0]
+    sampling = piko_amd.SAMPLE_TILED if mode == "tiled" else piko_amd.SAMPLE_GLOBAL
+    with piko_amd.ReyesPipe().allocate(sc, sampling=sampling) as pipe:
+        pipe.prepare()
+        pipe.run_frames(2)
+        rgba, depth, st = pipe.color(), pipe.depth(), pipe.stats()
+    clear = np.array([0x13 / 255.0, 0x33 / 255.0, 0x66 / 255.0, 1.0], np.float32)
+    assert st["leaves"] == 0 and st["micropolygons"] == 0
+    assert np.array_equal(rgba, np.broadcast_to(clear, rgba.shape)) and np.all(depth == 1.0)
+
+
+def test_frames_without_kept_samples(oracle_mod):
+    # keep_samples off (the benchmark's setting): the resolved frame is the
+    # oracle's on both sampling paths
+    import piko_amd
+    sc = sphere(8, 8, 320, 180)
+    orgba, oz, _ = oracle_mod.reyes_frame(sc)
+    for sampling in (piko_amd.SAMPLE_GLOBAL, piko_amd.SAMPLE_TILED):
+        with piko_amd.ReyesPipe().allocate(sc, sampling=sampling) as pipe:
+            pipe.prepare()
+            pipe.run_frames(3)
+            assert np.array_equal(pipe.color().view(np.int32), orgba.view(np.int32)), sampling
+            assert np.array_equal(pipe.depth().view(np.int32), oz.min(axis=2)), sampling
+
+
 def test_invalid_options():
     import piko_amd
     sc = sphere(2, 2, 64, 64)
