@@ -95,8 +95,16 @@ int piko_server_create(piko_server **out, const float *pos, const void *attr, lo
   bo.input_kind = s->kind;
   bo.subpixel_bits = s->S;
   bo.max_jobs = s->max_jobs;
-  bo.max_tris = max_scene;
-  bo.max_pixels = 1920 * 1080;
+  // each pipe is sized up front for the largest batch admission lets through
+  // (max_jobs requests of the largest scene at 1920x1080, within the batch
+  // budget), so serving does not grow a pipe's buffers: a free synchronises
+  // the whole device, stalling every pipe's stream.  Larger batches still
+  // grow it.
+  const long long per_tri = 64 + 4 + 4 * 16, per_px = 16;  // as footprint()
+  bo.max_tris = std::max<long long>(max_scene, std::min<long long>((long long)s->max_jobs * max_scene,
+                                                                     s->budget / per_tri));
+  bo.max_pixels = std::max<long long>(1920ll * 1080, std::min<long long>((long long)s->max_jobs * 1920 * 1080,
+                                                                          s->budget / per_px));
   bo.pair_capacity = 0;
   // a request is one screen of one resident scene: the direct path, which
   // measured the lower latency on such frames; PIKO_RASTER_MODE overrides
