@@ -224,7 +224,11 @@ typedef struct {
 typedef struct piko_server piko_server;
 
 /* Uploads (on_device == 0) or adopts the scenes' geometry once; scene k is
- * triangles [scene_base[k], scene_base[k] + scene_count[k]). */
+ * triangles [scene_base[k], scene_base[k] + scene_count[k]).  Each of the
+ * `pipes` batch pipes is allocated here for the largest batch admission lets
+ * through (max_jobs requests of the largest scene at 1920x1080, at most
+ * batch_budget_bytes), so serving itself does not allocate; a batch beyond
+ * that (larger screens) grows its pipe. */
 int piko_server_create(piko_server **server, const float *pos, const void *attr, long long tris, int on_device,
                        const long long *scene_base, const int *scene_count, int nscenes,
                        const piko_server_options *opt);
