@@ -1414,7 +1414,11 @@ int make_pipe(piko_reyes_pipe **out, const piko_constant_state *cs, const float 
   int rc = PIKO_OK;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, 0) == hipSuccess && prop.multiProcessorCount > 0) p->cus = prop.multiProcessorCount;
-  p->dice_grid = p->cus * 16;
+  // one-wave dice workgroups, 128 per CU: on the sphere frame that is about
+  // one leaf each, and the hardware hands the next workgroup to whichever
+  // SIMD frees first.  16 per CU, each walking eight leaves in turn, took 15%
+  // longer.
+  p->dice_grid = p->cus * 128;
   if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) rc = PIKO_ERR_HIP;
   if (rc == PIKO_OK) {
     if (on_device) {
