@@ -446,7 +446,8 @@ def main():
     if not args.no_cpu:
         log("timing the CPU oracle on a bounded sample")
         out["cpu_baseline"] = cpu_baseline(scene, T, S)
-        out["cpu_reference_schedule"] = cpu_reference_schedule(scene, T, S)
+        # the same oracle on the reference CPU backend's threaded schedule
+        out["cpu_baseline"]["reference_schedule"] = cpu_reference_schedule(scene, T, S)
     if args.workload in ("batch", "all"):
         out["batch"] = bench_batch(log)
     if args.workload in ("serve", "all"):
