@@ -1671,6 +1671,7 @@ int dalloc(piko_raster_pipe *p, T **ptr, long long elems, long long *acct = null
   const size_t bytes = (size_t)std::max<long long>(elems, 1) * sizeof(T);
   if (hipMalloc((void **)ptr, bytes) != hipSuccess) {
     *ptr = nullptr;
+    (void)hipGetLastError();  // reported as PIKO_ERR_OOM: not left for a later launch check to find
     return PIKO_ERR_OOM;
   }
   p->device_bytes += (long long)bytes;
@@ -1778,6 +1779,7 @@ int alloc_chunks(piko_raster_pipe *p, long long nch, long long ntab) {
   if ((rc = dalloc(p, &p->d_tpairs, p->cap_chunks * (long long)kChunkCap, &p->chunk_bytes)) != PIKO_OK) return rc;
   if ((rc = dalloc(p, &p->d_chunk_keep, p->cap_chunks, &p->chunk_bytes)) != PIKO_OK) return rc;
   if (hipHostMalloc((void **)&p->h_chunks, (size_t)p->cap_chunks * sizeof(ChunkDesc)) != hipSuccess) {
+    (void)hipGetLastError();
     p->h_chunks = nullptr;
     return PIKO_ERR_OOM;
   }
@@ -2493,8 +2495,10 @@ int piko_batch_create(piko_raster_pipe **out, const float *pos, const void *attr
   int rc = create_common(p, opt->input_kind, opt->subpixel_bits, opt->max_tris, opt->max_pixels, bins,
                          opt->pair_capacity, opt->raster_mode);
   if (rc == PIKO_OK) rc = dalloc(p, &p->d_jobs, (long long)p->cap_jobs);
-  if (rc == PIKO_OK && hipHostMalloc((void **)&p->h_jobs, (size_t)p->cap_jobs * sizeof(FrameArgs)) != hipSuccess)
+  if (rc == PIKO_OK && hipHostMalloc((void **)&p->h_jobs, (size_t)p->cap_jobs * sizeof(FrameArgs)) != hipSuccess) {
+    (void)hipGetLastError();  // reported as PIKO_ERR_OOM
     rc = PIKO_ERR_OOM;
+  }
   if (rc == PIKO_OK) {
     if (on_device) {
       p->d_pos = pos;

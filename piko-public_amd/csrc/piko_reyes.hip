@@ -1168,6 +1168,7 @@ int ralloc(piko_reyes_pipe *p, T **ptr, long long elems, long long *acct = nullp
   const size_t bytes = (size_t)std::max<long long>(elems, 1) * sizeof(T);
   if (hipMalloc((void **)ptr, bytes) != hipSuccess) {
     *ptr = nullptr;
+    (void)hipGetLastError();  // reported as PIKO_ERR_OOM: not left for a later launch check to find
     return PIKO_ERR_OOM;
   }
   p->device_bytes += (long long)bytes;

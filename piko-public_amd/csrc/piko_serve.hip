@@ -112,6 +112,14 @@ int piko_server_create(piko_server **out, const float *pos, const void *attr, lo
   for (int k = 0; rc == PIKO_OK && k < opt->pipes; ++k) {
     piko_raster_pipe *p = nullptr;
     rc = piko_batch_create(&p, s->d_pos, s->d_attr, tris, 1, &bo);
+    if (rc == PIKO_ERR_OOM && (bo.max_tris > max_scene || bo.max_pixels > 1920ll * 1080)) {
+      (void)hipGetLastError();
+      // no room to size every pipe up front: this pipe and the rest start at
+      // one request and grow with their batches
+      bo.max_tris = max_scene;
+      bo.max_pixels = 1920ll * 1080;
+      rc = piko_batch_create(&p, s->d_pos, s->d_attr, tris, 1, &bo);
+    }
     if (rc == PIKO_OK) s->pipes.push_back(p);
   }
   if (rc != PIKO_OK) {

@@ -52,3 +52,19 @@ def test_admission_refuses_what_never_fits():
         with pytest.raises(piko_amd.PikoError) as e:
             s.replay(reqs)
         assert e.value.status == -6
+
+
+def test_pipes_fall_back_when_up_front_sizing_does_not_fit(oracle_mod):
+    # a request cap and batch budget far beyond device memory: the pipes cannot
+    # be sized for the largest admissible batch up front (1.4e11 pixels), so
+    # they start at one request and grow with their batches; the server still
+    # starts and every frame is the oracle's
+    pos, nrm, base, count = serve.resident_scenes(4, 3000, seed=4)
+    reqs, meta = serve.poisson_trace(12, 50000.0, 4, seed=6, resolutions=SMALL_RES)
+    with serve.Server(pos, nrm, base, count, pipes=2, max_jobs=1 << 16, budget_bytes=1 << 44) as s:
+        done, st, cols = s.replay(reqs, keep=True)
+        log("replay: %s" % st)
+    for i in range(3):
+        sc, view = meta[i]
+        oc, od, _ = oracle_mod.frame(view_scene(pos, nrm, base, count, sc, view), 8, 8)
+        assert np.array_equal(cols[i], oc), i
