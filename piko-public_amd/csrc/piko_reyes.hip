@@ -26,7 +26,8 @@
 //                   of (depth, reversed emission order), so the nearest micropolygon
 //                   wins and on equal depth the later one -- the outcome of
 //                   the reference's atomicMin loop + (zBuffer == z) test run
-//                   in emission order (shade.pikostage:226-241).
+//                   in emission order (shade.pikostage:226-241).  A hit that
+//                   already loses to the sample's current key takes no atomic.
 //   k_reyes_resolve one thread per pixel: box filter of the samples' colours
 //                   to FP32 RGBA, nearest depth, and the visibility buffer
 //                   reset to empty for the next frame.
@@ -614,10 +615,27 @@ __global__ __launch_bounds__(64) void k_reyes_dice(ReyesArgs a, const float4 *__
         const float Y[4] = {s_qy[q][0], s_qy[q][1], s_qy[q][2], s_qy[q][3]};
         const unsigned long long key = s_key[q];
         unsigned long long *dst = vis + ((size_t)y * W + x) * SPP;
+        bool hit[SPP];
+        bool any = false;
 #pragma unroll
-        for (int smp = 0; smp < SPP; ++smp)
-          if (quad_hit(X, Y, (float)x + sample_off(SPP, smp, 0), (float)y + sample_off(SPP, smp, 1)))
-            atomicMin(dst + smp, key);
+        for (int smp = 0; smp < SPP; ++smp) {
+          hit[smp] = quad_hit(X, Y, (float)x + sample_off(SPP, smp, 0), (float)y + sample_off(SPP, smp, 1));
+          any |= hit[smp];
+        }
+        if (any) {
+          // A hit whose key cannot beat the sample's current key is dropped
+          // before its atomic.  The keys only decrease within the frame and
+          // were reset by a kernel earlier on this stream, so whatever value
+          // this load returns, however stale, is at least the sample's
+          // minimum so far: a dropped key would not have changed it.  On the
+          // 4,096-patch sphere (back faces drawn too) dice ran 30% faster.
+          unsigned long long cur[SPP];
+#pragma unroll
+          for (int smp = 0; smp < SPP; ++smp) cur[smp] = dst[smp];
+#pragma unroll
+          for (int smp = 0; smp < SPP; ++smp)
+            if (hit[smp] && key < cur[smp]) atomicMin(dst + smp, key);
+        }
       }
       wave_sync();
     }
