@@ -325,17 +325,44 @@ __global__ __launch_bounds__(kRankThreads) void k_reyes_rank(
   __shared__ uint32_t s_or, s_lv;
   const int L = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
   if (lvltot[2 * L] == 0 && L != kLevels - 1) return;  // no leaves at this level
-  unsigned long long base_l = 0, base_q = 0;
-  for (int l = 0; l < L; ++l) {
-    base_l += lvltot[2 * l];
-    base_q += lvltot[2 * l + 1];
-  }
+  // leaves and quads of the levels before this one, one level per lane: a
+  // loop over them waited for each level's load in turn, and the last
+  // level's block (which always runs) waited twenty times
+  static_assert(kLevels <= 64, "one lane per level");
+  const unsigned long long base_l = wave_sum_u64(lane < L ? lvltot[2 * lane] : 0ull);
+  const unsigned long long base_q = wave_sum_u64(lane < L ? lvltot[2 * lane + 1] : 0ull);
   const int run = (G + kRankThreads - 1) / kRankThreads;
   const int w0 = min(t * run, G), w1 = min(w0 + run, G);
+  // A run of up to kRunRegs waves is read into registers with every load in
+  // flight at once; a loop over the run waited for each wave's loads in turn.
+  // Longer runs (more than kRunRegs * kRankThreads split waves) loop.
+  constexpr int kRunRegs = 4;
+  const bool regs = run <= kRunRegs;
+  uint32_t rc[kRunRegs] = {}, rq[kRunRegs] = {}, rp[kRunRegs] = {};
   unsigned long long sl = 0, sq = 0;
-  for (int w = w0; w < w1; ++w) {
-    sl += cnt[L * G + w];
-    sq += qcnt[L * G + w];
+  if (lvltot[2 * L] != 0) {
+    if (regs) {
+      if (w0 < w1) {
+#pragma unroll
+        for (int i = 0; i < kRunRegs; ++i) {
+          const int w = min(w0 + i, w1 - 1);  // in the run: no branch around the loads
+          rc[i] = cnt[L * G + w];
+          rq[i] = qcnt[L * G + w];
+          rp[i] = lvlpre[L * G + w];
+        }
+#pragma unroll
+        for (int i = 0; i < kRunRegs; ++i)
+          if (w0 + i < w1) {
+            sl += rc[i];
+            sq += rq[i];
+          }
+      }
+    } else {
+      for (int w = w0; w < w1; ++w) {
+        sl += cnt[L * G + w];
+        sq += qcnt[L * G + w];
+      }
+    }
   }
   // exclusive block scan of the per-thread run totals
   const unsigned long long il = wave_incl_scan<unsigned long long>(sl, lane);
@@ -352,14 +379,33 @@ __global__ __launch_bounds__(kRankThreads) void k_reyes_rank(
       o += sh[0][k];
       oq += sh[1][k];
     }
-  for (int w = w0; w < w1; ++w) {
-    const uint32_t c = cnt[L * G + w];
-    off[L * G + w] = (uint32_t)o;
-    qoff[L * G + w] = (uint32_t)oq;  // exact whenever the frame's quads fit (status bit 2 clear)
-    const uint32_t slot0 = (uint32_t)w * (uint32_t)leaf_cap + lvlpre[L * G + w];
-    for (uint32_t k = 0; k < c; ++k) leaf_at[o + k] = slot0 + k;
-    o += c;
-    oq += qcnt[L * G + w];
+  // (the last level's block runs for the frame totals below; with no leaves
+  // at that level it has no ranks to write)
+  if (lvltot[2 * L] != 0) {
+    if (regs) {
+#pragma unroll
+      for (int i = 0; i < kRunRegs; ++i) {
+        const int w = w0 + i;
+        if (w < w1) {
+          off[L * G + w] = (uint32_t)o;
+          qoff[L * G + w] = (uint32_t)oq;  // exact whenever the frame's quads fit (status bit 2 clear)
+          const uint32_t slot0 = (uint32_t)w * (uint32_t)leaf_cap + rp[i];
+          for (uint32_t k = 0; k < rc[i]; ++k) leaf_at[o + k] = slot0 + k;
+          o += rc[i];
+          oq += rq[i];
+        }
+      }
+    } else {
+      for (int w = w0; w < w1; ++w) {
+        const uint32_t c = cnt[L * G + w];
+        off[L * G + w] = (uint32_t)o;
+        qoff[L * G + w] = (uint32_t)oq;
+        const uint32_t slot0 = (uint32_t)w * (uint32_t)leaf_cap + lvlpre[L * G + w];
+        for (uint32_t k = 0; k < c; ++k) leaf_at[o + k] = slot0 + k;
+        o += c;
+        oq += qcnt[L * G + w];
+      }
+    }
   }
   if (L != kLevels - 1) return;
   if (t == 0) {
@@ -368,9 +414,21 @@ __global__ __launch_bounds__(kRankThreads) void k_reyes_rank(
   }
   __syncthreads();
   uint32_t ovf = 0, lv = 0;
-  for (int w = t; w < G; w += kRankThreads) {
-    ovf |= wstat[w] & 3u;
-    lv = max(lv, wstat[w] >> 8);
+  if (regs) {
+    uint32_t ws[kRunRegs];
+#pragma unroll
+    for (int i = 0; i < kRunRegs; ++i) ws[i] = wstat[min(t + i * kRankThreads, G - 1)];
+#pragma unroll
+    for (int i = 0; i < kRunRegs; ++i)
+      if (t + i * kRankThreads < G) {
+        ovf |= ws[i] & 3u;
+        lv = max(lv, ws[i] >> 8);
+      }
+  } else {
+    for (int w = t; w < G; w += kRankThreads) {
+      ovf |= wstat[w] & 3u;
+      lv = max(lv, wstat[w] >> 8);
+    }
   }
   atomicOr(&s_or, ovf);
   atomicMax(&s_lv, lv);

@@ -97,6 +97,15 @@ def test_roots_per_wave_invariance(oracle_mod, rpw):
         assert st["capacity_reruns"] >= 1
 
 
+def test_more_split_waves_than_rank_registers(oracle_mod):
+    # 4,608 patches, one per split wave: the rank kernel's threads each own a
+    # run of 5 waves, past the 4 it keeps in registers, so the loop form runs
+    sc = sphere(72, 64, 320, 240)
+    _, st = assert_reyes_equal(sc, oracle_mod, check_leaves=False, samplings=(("global", 0),), roots_per_wave=1)
+    log("4608 patches: leaves %d micropolygons %d" % (st["leaves"], st["micropolygons"]))
+    assert sc.n == 4608 and st["leaves"] > 0
+
+
 @pytest.mark.parametrize("mode", ["global", "tiled"])
 def test_frames_repeat_and_camera_change(oracle_mod, mode):
     import piko_amd
