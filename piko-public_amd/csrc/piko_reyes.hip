@@ -617,7 +617,7 @@ __global__ __launch_bounds__(64) void k_reyes_dice(ReyesArgs a, const float4 *__
   __shared__ float s_pt[MAXP * MAXP][6];
   __shared__ float s_qx[64][4], s_qy[64][4];
   __shared__ unsigned long long s_key[64];
-  __shared__ int s_x0[64], s_y0[64], s_wid[64], s_pre[64];
+  __shared__ int s_x0[64], s_y0[64], s_wid[64], s_pre[64], s_own[64];
   const int lane = threadIdx.x;
   if (st->overflow & 4u) return;  // quad colours would not fit: the frame is re-run with room
   const unsigned long long nleaf = st->nleaf;
@@ -660,14 +660,34 @@ __global__ __launch_bounds__(64) void k_reyes_dice(ReyesArgs a, const float4 *__
       s_wid[lane] = wid;
       s_pre[lane] = inc - npx;
       wave_sync();
-      const int nqc = min(64, nq - q0);
-      for (int t = lane; t < total; t += 64) {
-        int q = 0;
-#pragma unroll
-        for (int step = 32; step > 0; step >>= 1)
-          if (q + step < nqc && s_pre[q + step] <= t) q += step;
+      // Pair slot t belongs to the last quad whose first pair is at or before
+      // t; quads with pairs start at distinct slots.  Per chunk of 64 slots
+      // each such quad marks its start, the ballot of the marks gives every
+      // slot the nearest mark at or below it, and a slot with none keeps the
+      // previous chunk's last owner (a binary search over the starts took six
+      // dependent LDS reads per slot).  The quotient local / w uses the float
+      // reciprocal, exact for local < 2^21 as shown for the raster's span
+      // index; a chunk with more pairs divides.
+      const int pre = inc - npx;
+      const bool fquot = total < (1 << 21);
+      int carry = 0;
+      for (int t0 = 0; t0 < total; t0 += 64) {
+        s_own[lane] = -1;
+        wave_sync();
+        const int rel = pre - t0;
+        if (npx > 0 && rel >= 0 && rel < 64) s_own[rel] = lane;
+        wave_sync();
+        const int mark = s_own[lane];
+        const unsigned long long m = __ballot(mark >= 0) & ((2ull << lane) - 1ull);
+        const int got = __shfl(mark, m ? 63 - __clzll((long long)m) : 0);
+        const int q = m ? got : carry;
+        carry = __shfl(q, 63);
+        wave_sync();
+        const int t = t0 + lane;
+        if (t >= total) break;  // the last chunk
         const int local = t - s_pre[q], w = s_wid[q];
-        const int dy = local / w, dx = local - dy * w;
+        const int dy = fquot ? (int)(((float)local + 0.5f) * __builtin_amdgcn_rcpf((float)w)) : local / w;
+        const int dx = local - dy * w;
         const int x = s_x0[q] + dx, y = s_y0[q] + dy;
         const float X[4] = {s_qx[q][0], s_qx[q][1], s_qx[q][2], s_qx[q][3]};
         const float Y[4] = {s_qy[q][0], s_qy[q][1], s_qy[q][2], s_qy[q][3]};
