@@ -1222,6 +1222,13 @@ struct piko_reyes_pipe {
   piko_reyes_options opt{};
   ReyesArgs ra{};
   hipStream_t stream = nullptr;
+  // Global sampling in run_frames: each frame's resolve runs on rstream
+  // beside the next frame's split and ranks (neither touches what the other
+  // reads or writes); the next dice waits for it (e_res), as it clears the
+  // visibility buffer that dice fills.
+  hipStream_t rstream = nullptr;
+  hipEvent_t e_dice = nullptr, e_res = nullptr;
+  bool res_pending = false;
   const float4 *d_cp = nullptr;
   bool owns_input = false;
   float4 *d_front = nullptr, *d_lcp = nullptr;
@@ -1350,7 +1357,8 @@ const char *kReyesTiledKernels[] = {"k_reyes_split", "k_reyes_levels+k_reyes_ran
                                     "k_reyes_dice_bin+k_reyes_tile_count+k_reyes_tile_scan+k_reyes_tile_scatter",
                                     "k_reyes_tile_sample"};
 
-int enqueue_frame(piko_reyes_pipe *p, hipEvent_t *ev) {
+// overlap: global sampling's resolve on rstream (see piko_reyes_pipe)
+int enqueue_frame(piko_reyes_pipe *p, hipEvent_t *ev, bool overlap = false) {
   const ReyesArgs &a = p->ra;
   const int NL = kLevels * a.G;
   (void)NL;
@@ -1392,6 +1400,10 @@ int enqueue_frame(piko_reyes_pipe *p, hipEvent_t *ev) {
     if (ev) (void)hipEventRecord(ev[3], p->stream);
     return hipGetLastError() == hipSuccess ? PIKO_OK : PIKO_ERR_HIP;
   }
+  if (p->res_pending) {
+    HIPOK(hipStreamWaitEvent(p->stream, p->e_res, 0));
+    p->res_pending = false;
+  }
 #define PIKO_DICE(MAXP, SPP)                                                                                  \
   k_reyes_dice<MAXP, SPP><<<p->dice_grid, 64, 0, p->stream>>>(a, p->d_lcp, p->d_lmeta, p->d_lqloc, p->d_leaf_at, \
                                                               p->d_qoff, p->d_st, p->d_qcol, p->d_vis)
@@ -1404,8 +1416,18 @@ int enqueue_frame(piko_reyes_pipe *p, hipEvent_t *ev) {
   }
 #undef PIKO_DICE
   if (ev) (void)hipEventRecord(ev[2], p->stream);
-  k_reyes_resolve<<<(unsigned)cdiv((long long)a.W * a.H, 256), 256, 0, p->stream>>>(a, p->d_vis, p->d_qcol,
-                                                                                    p->d_rgba, p->d_depth, p->d_sz);
+  hipStream_t rs = p->stream;
+  if (overlap) {
+    HIPOK(hipEventRecord(p->e_dice, p->stream));
+    HIPOK(hipStreamWaitEvent(p->rstream, p->e_dice, 0));
+    rs = p->rstream;
+  }
+  k_reyes_resolve<<<(unsigned)cdiv((long long)a.W * a.H, 256), 256, 0, rs>>>(a, p->d_vis, p->d_qcol, p->d_rgba,
+                                                                             p->d_depth, p->d_sz);
+  if (overlap) {
+    HIPOK(hipEventRecord(p->e_res, p->rstream));
+    p->res_pending = true;
+  }
   if (ev) (void)hipEventRecord(ev[3], p->stream);
   return hipGetLastError() == hipSuccess ? PIKO_OK : PIKO_ERR_HIP;
 }
@@ -1516,7 +1538,11 @@ int make_pipe(piko_reyes_pipe **out, const piko_constant_state *cs, const float 
   // SIMD frees first.  16 per CU, each walking eight leaves in turn, took 15%
   // longer.
   p->dice_grid = p->cus * 128;
-  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) rc = PIKO_ERR_HIP;
+  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&p->rstream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&p->e_dice, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&p->e_res, hipEventDisableTiming) != hipSuccess)
+    rc = PIKO_ERR_HIP;
   if (rc == PIKO_OK) {
     if (on_device) {
       p->d_cp = (const float4 *)patches;
@@ -1595,10 +1621,20 @@ int piko_reyes_prepare(piko_reyes_pipe *p, const piko_constant_state *cs) {
 int piko_reyes_run_frames(piko_reyes_pipe *p, int nframes) {
   if (!p || nframes < 0) return PIKO_ERR_INVALID;
   p->reruns = 0;
+  // with more than one frame, each resolve overlaps the next frame's split
+  const bool overlap = !p->tiled && nframes > 1;
   for (int attempt = 0; attempt < 12; ++attempt) {
     for (int f = 0; f < nframes; ++f) {
-      const int rc = enqueue_frame(p, nullptr);
-      if (rc != PIKO_OK) return rc;
+      const int rc = enqueue_frame(p, nullptr, overlap);
+      if (rc != PIKO_OK) {
+        if (p->res_pending) (void)hipStreamWaitEvent(p->stream, p->e_res, 0);
+        p->res_pending = false;
+        return rc;
+      }
+    }
+    if (p->res_pending) {  // the caller's later work on the pipe's stream follows the last resolve
+      HIPOK(hipStreamWaitEvent(p->stream, p->e_res, 0));
+      p->res_pending = false;
     }
     int rerun = 0;
     const int rc = check_capacity(p, &rerun);
@@ -1715,6 +1751,7 @@ int piko_reyes_get_stats(piko_reyes_pipe *p, piko_reyes_stats *st) {
 
 int piko_reyes_destroy(piko_reyes_pipe *p) {
   if (!p) return PIKO_ERR_INVALID;
+  if (p->rstream) (void)hipStreamSynchronize(p->rstream);
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   free_front(p);
   free_leaves(p);
@@ -1741,6 +1778,9 @@ int piko_reyes_destroy(piko_reyes_pipe *p) {
     float4 *d = const_cast<float4 *>(p->d_cp);
     rfree(d);
   }
+  if (p->e_dice) (void)hipEventDestroy(p->e_dice);
+  if (p->e_res) (void)hipEventDestroy(p->e_res);
+  if (p->rstream) (void)hipStreamDestroy(p->rstream);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
   return PIKO_OK;

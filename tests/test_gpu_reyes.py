@@ -2,8 +2,8 @@
 against the oracle through the C-ABI, bit for bit: split leaves (control
 points and the reference's nu/nv), every sample's depth, the resolved FP32
 RGBA and the per-pixel depth.  Every frame is checked with both sample
-compositing paths: global (one 64-bit atomicMin per covered sample in a
-visibility buffer) and tiled (per-tile LDS visibility, 16- and 32-pixel
+compositing paths: global (a 64-bit atomicMin per covered sample in a
+visibility buffer, unless the sample's key already wins) and tiled (per-tile LDS visibility, 16- and 32-pixel
 tiles)."""
 import sys
 
@@ -123,6 +123,11 @@ def test_frames_repeat_and_camera_change(oracle_mod, mode):
         pipe.run_single()
         ob, oz, _ = oracle_mod.reyes_frame(b)
         assert np.array_equal(pipe.color().view(np.int32), ob.view(np.int32)), "new camera differs"
+        assert np.array_equal(pipe.samples(), oz)
+        # several frames in one call: on the global path each resolve runs on
+        # a second stream beside the next frame's split
+        pipe.run_frames(5)
+        assert np.array_equal(pipe.color().view(np.int32), ob.view(np.int32)), "frames in one call differ"
         assert np.array_equal(pipe.samples(), oz)
 
 
