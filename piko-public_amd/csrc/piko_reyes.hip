@@ -1222,13 +1222,23 @@ struct piko_reyes_pipe {
   piko_reyes_options opt{};
   ReyesArgs ra{};
   hipStream_t stream = nullptr;
-  // Global sampling in run_frames: each frame's resolve runs on rstream
-  // beside the next frame's split and ranks (neither touches what the other
-  // reads or writes); the next dice waits for it (e_res), as it clears the
-  // visibility buffer that dice fills.
+  // Global sampling in run_frames: dice and resolve run on rstream, split and
+  // ranks on stream, so a frame's split and ranks run beside the previous
+  // frame's dice and resolve.  Consecutive frames alternate between two
+  // slots of what the split and ranks write and dice reads (the d_* pointers
+  // below are the current slot, a_* the other one); a split waits for the
+  // dice that last read its slot (e_slot), a dice for its frame's ranks
+  // (e_rank).  Dice and resolve keep frame order on rstream, which covers the
+  // visibility buffer and the quad colours.
   hipStream_t rstream = nullptr;
-  hipEvent_t e_dice = nullptr, e_res = nullptr;
-  bool res_pending = false;
+  hipEvent_t e_rank = nullptr, e_res = nullptr, e_slot[2] = {nullptr, nullptr};
+  bool res_pending = false;  // stream has not yet waited for rstream's last resolve
+  int slot = 0;
+  bool alt_ready = false;
+  float4 *a_lcp = nullptr;
+  uint32_t *a_lmeta = nullptr, *a_lqloc = nullptr, *a_leaf_at = nullptr, *a_off = nullptr, *a_qoff = nullptr;
+  ReyesState *a_st = nullptr;
+  long long alt_bytes = 0;
   const float4 *d_cp = nullptr;
   bool owns_input = false;
   float4 *d_front = nullptr, *d_lcp = nullptr;
@@ -1295,7 +1305,50 @@ int alloc_front(piko_reyes_pipe *p) {
   return ralloc(p, &p->d_front, (long long)p->ra.G * 2 * p->ra.front_cap * 16, &p->front_bytes);
 }
 
+// The second slot of the split and rank outputs (see piko_reyes_pipe).
+void free_alt(piko_reyes_pipe *p) {
+  rfree(p->a_lcp);
+  rfree(p->a_lmeta);
+  rfree(p->a_lqloc);
+  rfree(p->a_leaf_at);
+  rfree(p->a_off);
+  rfree(p->a_qoff);
+  rfree(p->a_st);
+  p->device_bytes -= p->alt_bytes;
+  p->alt_bytes = 0;
+  p->alt_ready = false;
+}
+
+int alloc_alt(piko_reyes_pipe *p) {
+  const long long n = (long long)p->ra.G * p->ra.leaf_cap, NL = (long long)kLevels * p->ra.G;
+  int rc = ralloc(p, &p->a_lcp, n * 16, &p->alt_bytes);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->a_lmeta, n, &p->alt_bytes);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->a_lqloc, n, &p->alt_bytes);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->a_leaf_at, n, &p->alt_bytes);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->a_off, NL, &p->alt_bytes);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->a_qoff, NL, &p->alt_bytes);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->a_st, 1, &p->alt_bytes);
+  if (rc != PIKO_OK) {
+    free_alt(p);
+    return rc;
+  }
+  p->alt_ready = true;
+  return PIKO_OK;
+}
+
+void swap_slot(piko_reyes_pipe *p) {
+  std::swap(p->d_lcp, p->a_lcp);
+  std::swap(p->d_lmeta, p->a_lmeta);
+  std::swap(p->d_lqloc, p->a_lqloc);
+  std::swap(p->d_leaf_at, p->a_leaf_at);
+  std::swap(p->d_off, p->a_off);
+  std::swap(p->d_qoff, p->a_qoff);
+  std::swap(p->d_st, p->a_st);
+  p->slot ^= 1;
+}
+
 void free_leaves(piko_reyes_pipe *p) {
+  free_alt(p);  // sized by the leaf capacity too; made again when next needed
   rfree(p->d_lcp);
   rfree(p->d_lmeta);
   rfree(p->d_lqloc);
@@ -1357,11 +1410,13 @@ const char *kReyesTiledKernels[] = {"k_reyes_split", "k_reyes_levels+k_reyes_ran
                                     "k_reyes_dice_bin+k_reyes_tile_count+k_reyes_tile_scan+k_reyes_tile_scatter",
                                     "k_reyes_tile_sample"};
 
-// overlap: global sampling's resolve on rstream (see piko_reyes_pipe)
+// overlap: global sampling's dice and resolve on rstream, beside the next
+// frame's split and ranks (see piko_reyes_pipe)
 int enqueue_frame(piko_reyes_pipe *p, hipEvent_t *ev, bool overlap = false) {
   const ReyesArgs &a = p->ra;
   const int NL = kLevels * a.G;
   (void)NL;
+  if (overlap) HIPOK(hipStreamWaitEvent(p->stream, p->e_slot[p->slot], 0));  // the last dice to read this slot
   k_reyes_split<<<a.G, 64, 0, p->stream>>>(p->d_cp, a, p->d_front, p->d_lcp, p->d_lmeta, p->d_lqloc, p->d_cnt,
                                            p->d_qcnt, p->d_lvlpre, p->d_wstat);
   if (ev) (void)hipEventRecord(ev[0], p->stream);
@@ -1400,13 +1455,18 @@ int enqueue_frame(piko_reyes_pipe *p, hipEvent_t *ev, bool overlap = false) {
     if (ev) (void)hipEventRecord(ev[3], p->stream);
     return hipGetLastError() == hipSuccess ? PIKO_OK : PIKO_ERR_HIP;
   }
-  if (p->res_pending) {
+  hipStream_t ds = p->stream;
+  if (overlap) {
+    HIPOK(hipEventRecord(p->e_rank, p->stream));
+    HIPOK(hipStreamWaitEvent(p->rstream, p->e_rank, 0));
+    ds = p->rstream;
+  } else if (p->res_pending) {
     HIPOK(hipStreamWaitEvent(p->stream, p->e_res, 0));
     p->res_pending = false;
   }
-#define PIKO_DICE(MAXP, SPP)                                                                                  \
-  k_reyes_dice<MAXP, SPP><<<p->dice_grid, 64, 0, p->stream>>>(a, p->d_lcp, p->d_lmeta, p->d_lqloc, p->d_leaf_at, \
-                                                              p->d_qoff, p->d_st, p->d_qcol, p->d_vis)
+#define PIKO_DICE(MAXP, SPP)                                                                           \
+  k_reyes_dice<MAXP, SPP><<<p->dice_grid, 64, 0, ds>>>(a, p->d_lcp, p->d_lmeta, p->d_lqloc, p->d_leaf_at, \
+                                                       p->d_qoff, p->d_st, p->d_qcol, p->d_vis)
   if (a.spp == 4) {
     if (small) PIKO_DICE(9, 4);
     else PIKO_DICE(34, 4);
@@ -1416,13 +1476,8 @@ int enqueue_frame(piko_reyes_pipe *p, hipEvent_t *ev, bool overlap = false) {
   }
 #undef PIKO_DICE
   if (ev) (void)hipEventRecord(ev[2], p->stream);
-  hipStream_t rs = p->stream;
-  if (overlap) {
-    HIPOK(hipEventRecord(p->e_dice, p->stream));
-    HIPOK(hipStreamWaitEvent(p->rstream, p->e_dice, 0));
-    rs = p->rstream;
-  }
-  k_reyes_resolve<<<(unsigned)cdiv((long long)a.W * a.H, 256), 256, 0, rs>>>(a, p->d_vis, p->d_qcol, p->d_rgba,
+  if (overlap) HIPOK(hipEventRecord(p->e_slot[p->slot], p->rstream));
+  k_reyes_resolve<<<(unsigned)cdiv((long long)a.W * a.H, 256), 256, 0, ds>>>(a, p->d_vis, p->d_qcol, p->d_rgba,
                                                                              p->d_depth, p->d_sz);
   if (overlap) {
     HIPOK(hipEventRecord(p->e_res, p->rstream));
@@ -1540,8 +1595,10 @@ int make_pipe(piko_reyes_pipe **out, const piko_constant_state *cs, const float 
   p->dice_grid = p->cus * 128;
   if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&p->rstream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&p->e_dice, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&p->e_res, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&p->e_rank, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&p->e_res, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&p->e_slot[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&p->e_slot[1], hipEventDisableTiming) != hipSuccess)
     rc = PIKO_ERR_HIP;
   if (rc == PIKO_OK) {
     if (on_device) {
@@ -1621,10 +1678,14 @@ int piko_reyes_prepare(piko_reyes_pipe *p, const piko_constant_state *cs) {
 int piko_reyes_run_frames(piko_reyes_pipe *p, int nframes) {
   if (!p || nframes < 0) return PIKO_ERR_INVALID;
   p->reruns = 0;
-  // with more than one frame, each resolve overlaps the next frame's split
-  const bool overlap = !p->tiled && nframes > 1;
+  // with more than one frame, each frame's dice and resolve overlap the next
+  // frame's split and ranks, the frames alternating between two slots
+  bool overlap = !p->tiled && nframes > 1;
+  if (overlap && !p->alt_ready && alloc_alt(p) != PIKO_OK) overlap = false;  // no room: one frame at a time
   for (int attempt = 0; attempt < 12; ++attempt) {
+    if (overlap && !p->alt_ready && alloc_alt(p) != PIKO_OK) overlap = false;  // regrown leaves freed it
     for (int f = 0; f < nframes; ++f) {
+      if (overlap && f > 0) swap_slot(p);
       const int rc = enqueue_frame(p, nullptr, overlap);
       if (rc != PIKO_OK) {
         if (p->res_pending) (void)hipStreamWaitEvent(p->stream, p->e_res, 0);
@@ -1778,8 +1839,10 @@ int piko_reyes_destroy(piko_reyes_pipe *p) {
     float4 *d = const_cast<float4 *>(p->d_cp);
     rfree(d);
   }
-  if (p->e_dice) (void)hipEventDestroy(p->e_dice);
+  if (p->e_rank) (void)hipEventDestroy(p->e_rank);
   if (p->e_res) (void)hipEventDestroy(p->e_res);
+  for (hipEvent_t e : p->e_slot)
+    if (e) (void)hipEventDestroy(e);
   if (p->rstream) (void)hipStreamDestroy(p->rstream);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;

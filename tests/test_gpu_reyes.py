@@ -97,6 +97,31 @@ def test_roots_per_wave_invariance(oracle_mod, rpw):
         assert st["capacity_reruns"] >= 1
 
 
+def test_frames_in_one_call_regrow(oracle_mod):
+    # several frames in one call run each frame's split beside the previous
+    # frame's dice, alternating two slots of leaves and ranks; this close-up
+    # view overflows the first leaf regions, so the slots are freed, grown
+    # and the frames re-run
+    import piko_amd
+    sc = sphere(8, 16, 400, 300, eye=(0.0, 0.2, 2.2))
+    orgba, oz, _ = oracle_mod.reyes_frame(sc)
+    ocp, onu, onv = oracle_mod.reyes_split(sc.cp, sc.W, sc.H, sc.view_proj, sc.threshold)
+    log("frames in one call with regrowth: P=%d" % sc.n)
+    with piko_amd.ReyesPipe().allocate(sc, keep_samples=True, sampling=piko_amd.SAMPLE_GLOBAL,
+                                       roots_per_wave=1) as pipe:
+        pipe.prepare()
+        pipe.run_frames(4)
+        st = pipe.stats()
+        assert st["capacity_reruns"] >= 1
+        assert np.array_equal(pipe.color().view(np.int32), orgba.view(np.int32)), "colour differs"
+        assert np.array_equal(pipe.samples(), oz), "sample depth differs"
+        lcp, lnu, lnv = pipe.leaves()
+        assert np.array_equal(lcp.view(np.int32), ocp.view(np.int32)), "leaves differ"
+        assert np.array_equal(lnu, onu) and np.array_equal(lnv, onv)
+        pipe.run_frames(3)
+        assert np.array_equal(pipe.color().view(np.int32), orgba.view(np.int32)), "colour differs on later frames"
+
+
 def test_more_split_waves_than_rank_registers(oracle_mod):
     # 4,608 patches, one per split wave: the rank kernel's threads each own a
     # run of 5 waves, past the 4 it keeps in registers, so the loop form runs
