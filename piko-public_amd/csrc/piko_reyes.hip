@@ -1222,22 +1222,26 @@ struct piko_reyes_pipe {
   piko_reyes_options opt{};
   ReyesArgs ra{};
   hipStream_t stream = nullptr;
-  // Global sampling in run_frames: dice and resolve run on rstream, split and
-  // ranks on stream, so a frame's split and ranks run beside the previous
-  // frame's dice and resolve.  Consecutive frames alternate between two
-  // slots of what the split and ranks write and dice reads (the d_* pointers
-  // below are the current slot, a_* the other one); a split waits for the
-  // dice that last read its slot (e_slot), a dice for its frame's ranks
-  // (e_rank).  Dice and resolve keep frame order on rstream, which covers the
-  // visibility buffer and the quad colours.
-  hipStream_t rstream = nullptr;
-  hipEvent_t e_rank = nullptr, e_res = nullptr, e_slot[2] = {nullptr, nullptr};
-  bool res_pending = false;  // stream has not yet waited for rstream's last resolve
+  // Global sampling in run_frames: split and ranks run on stream, dice on
+  // rstream, resolve on vstream, so a frame's split and ranks and the
+  // previous frame's resolve run beside a frame's dice.  Consecutive frames
+  // alternate between two slots of what split and ranks write and dice
+  // reads, and of the visibility buffer and quad colours that dice writes
+  // and resolve reads and clears (the d_* pointers below are the current
+  // slot, a_* the other one).  A split waits for the dice that last read its
+  // slot (e_slot), a dice for its frame's ranks (e_rank) and for the resolve
+  // that last cleared its slot (e_res), a resolve for its frame's dice.
+  hipStream_t rstream = nullptr, vstream = nullptr;
+  hipEvent_t e_rank = nullptr, e_slot[2] = {nullptr, nullptr}, e_res[2] = {nullptr, nullptr};
+  hipEvent_t e_last = nullptr;  // the last resolve enqueued on vstream
+  bool res_pending = false;     // stream has not yet waited for e_last
   int slot = 0;
   bool alt_ready = false;
   float4 *a_lcp = nullptr;
   uint32_t *a_lmeta = nullptr, *a_lqloc = nullptr, *a_leaf_at = nullptr, *a_off = nullptr, *a_qoff = nullptr;
   ReyesState *a_st = nullptr;
+  unsigned long long *a_vis = nullptr;
+  float *a_qcol = nullptr;
   long long alt_bytes = 0;
   const float4 *d_cp = nullptr;
   bool owns_input = false;
@@ -1314,6 +1318,8 @@ void free_alt(piko_reyes_pipe *p) {
   rfree(p->a_off);
   rfree(p->a_qoff);
   rfree(p->a_st);
+  rfree(p->a_vis);
+  rfree(p->a_qcol);
   p->device_bytes -= p->alt_bytes;
   p->alt_bytes = 0;
   p->alt_ready = false;
@@ -1328,6 +1334,15 @@ int alloc_alt(piko_reyes_pipe *p) {
   if (rc == PIKO_OK) rc = ralloc(p, &p->a_off, NL, &p->alt_bytes);
   if (rc == PIKO_OK) rc = ralloc(p, &p->a_qoff, NL, &p->alt_bytes);
   if (rc == PIKO_OK) rc = ralloc(p, &p->a_st, 1, &p->alt_bytes);
+  const long long ns = (long long)p->W * p->H * p->ra.spp;
+  if (rc == PIKO_OK) rc = ralloc(p, &p->a_vis, ns, &p->alt_bytes);
+  if (rc == PIKO_OK) rc = ralloc(p, &p->a_qcol, (long long)p->ra.qcap * 3, &p->alt_bytes);
+  if (rc == PIKO_OK) {
+    // empty keys before any frame uses the slot: the fill runs on stream,
+    // ahead of every split, so ahead of every dice and resolve
+    k_fill_u64<<<(unsigned)std::min<long long>(cdiv(ns, 256), 4096), 256, 0, p->stream>>>(p->a_vis, ns, kVisEmpty);
+    if (hipGetLastError() != hipSuccess) rc = PIKO_ERR_HIP;
+  }
   if (rc != PIKO_OK) {
     free_alt(p);
     return rc;
@@ -1344,6 +1359,8 @@ void swap_slot(piko_reyes_pipe *p) {
   std::swap(p->d_off, p->a_off);
   std::swap(p->d_qoff, p->a_qoff);
   std::swap(p->d_st, p->a_st);
+  std::swap(p->d_vis, p->a_vis);
+  std::swap(p->d_qcol, p->a_qcol);
   p->slot ^= 1;
 }
 
@@ -1369,6 +1386,7 @@ int alloc_leaves(piko_reyes_pipe *p) {
 }
 
 void free_qcol(piko_reyes_pipe *p) {
+  free_alt(p);  // its quad colours are sized by the same capacity
   rfree(p->d_qcol);
   rfree(p->d_pxy);
   rfree(p->d_qz);
@@ -1410,8 +1428,8 @@ const char *kReyesTiledKernels[] = {"k_reyes_split", "k_reyes_levels+k_reyes_ran
                                     "k_reyes_dice_bin+k_reyes_tile_count+k_reyes_tile_scan+k_reyes_tile_scatter",
                                     "k_reyes_tile_sample"};
 
-// overlap: global sampling's dice and resolve on rstream, beside the next
-// frame's split and ranks (see piko_reyes_pipe)
+// overlap: global sampling's split and ranks, dice and resolve on three
+// streams, consecutive frames in alternate slots (see piko_reyes_pipe)
 int enqueue_frame(piko_reyes_pipe *p, hipEvent_t *ev, bool overlap = false) {
   const ReyesArgs &a = p->ra;
   const int NL = kLevels * a.G;
@@ -1455,13 +1473,15 @@ int enqueue_frame(piko_reyes_pipe *p, hipEvent_t *ev, bool overlap = false) {
     if (ev) (void)hipEventRecord(ev[3], p->stream);
     return hipGetLastError() == hipSuccess ? PIKO_OK : PIKO_ERR_HIP;
   }
-  hipStream_t ds = p->stream;
+  hipStream_t ds = p->stream, vs = p->stream;
   if (overlap) {
     HIPOK(hipEventRecord(p->e_rank, p->stream));
     HIPOK(hipStreamWaitEvent(p->rstream, p->e_rank, 0));
+    HIPOK(hipStreamWaitEvent(p->rstream, p->e_res[p->slot], 0));  // this slot's buffer cleared
     ds = p->rstream;
+    vs = p->vstream;
   } else if (p->res_pending) {
-    HIPOK(hipStreamWaitEvent(p->stream, p->e_res, 0));
+    HIPOK(hipStreamWaitEvent(p->stream, p->e_last, 0));
     p->res_pending = false;
   }
 #define PIKO_DICE(MAXP, SPP)                                                                           \
@@ -1476,11 +1496,15 @@ int enqueue_frame(piko_reyes_pipe *p, hipEvent_t *ev, bool overlap = false) {
   }
 #undef PIKO_DICE
   if (ev) (void)hipEventRecord(ev[2], p->stream);
-  if (overlap) HIPOK(hipEventRecord(p->e_slot[p->slot], p->rstream));
-  k_reyes_resolve<<<(unsigned)cdiv((long long)a.W * a.H, 256), 256, 0, ds>>>(a, p->d_vis, p->d_qcol, p->d_rgba,
+  if (overlap) {
+    HIPOK(hipEventRecord(p->e_slot[p->slot], p->rstream));
+    HIPOK(hipStreamWaitEvent(p->vstream, p->e_slot[p->slot], 0));
+  }
+  k_reyes_resolve<<<(unsigned)cdiv((long long)a.W * a.H, 256), 256, 0, vs>>>(a, p->d_vis, p->d_qcol, p->d_rgba,
                                                                              p->d_depth, p->d_sz);
   if (overlap) {
-    HIPOK(hipEventRecord(p->e_res, p->rstream));
+    HIPOK(hipEventRecord(p->e_res[p->slot], p->vstream));
+    p->e_last = p->e_res[p->slot];
     p->res_pending = true;
   }
   if (ev) (void)hipEventRecord(ev[3], p->stream);
@@ -1595,10 +1619,12 @@ int make_pipe(piko_reyes_pipe **out, const piko_constant_state *cs, const float 
   p->dice_grid = p->cus * 128;
   if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&p->rstream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&p->vstream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&p->e_rank, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&p->e_res, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&p->e_slot[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&p->e_slot[1], hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&p->e_slot[1], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&p->e_res[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&p->e_res[1], hipEventDisableTiming) != hipSuccess)
     rc = PIKO_ERR_HIP;
   if (rc == PIKO_OK) {
     if (on_device) {
@@ -1688,13 +1714,13 @@ int piko_reyes_run_frames(piko_reyes_pipe *p, int nframes) {
       if (overlap && f > 0) swap_slot(p);
       const int rc = enqueue_frame(p, nullptr, overlap);
       if (rc != PIKO_OK) {
-        if (p->res_pending) (void)hipStreamWaitEvent(p->stream, p->e_res, 0);
+        if (p->res_pending) (void)hipStreamWaitEvent(p->stream, p->e_last, 0);
         p->res_pending = false;
         return rc;
       }
     }
     if (p->res_pending) {  // the caller's later work on the pipe's stream follows the last resolve
-      HIPOK(hipStreamWaitEvent(p->stream, p->e_res, 0));
+      HIPOK(hipStreamWaitEvent(p->stream, p->e_last, 0));
       p->res_pending = false;
     }
     int rerun = 0;
@@ -1813,6 +1839,7 @@ int piko_reyes_get_stats(piko_reyes_pipe *p, piko_reyes_stats *st) {
 int piko_reyes_destroy(piko_reyes_pipe *p) {
   if (!p) return PIKO_ERR_INVALID;
   if (p->rstream) (void)hipStreamSynchronize(p->rstream);
+  if (p->vstream) (void)hipStreamSynchronize(p->vstream);
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   free_front(p);
   free_leaves(p);
@@ -1840,9 +1867,11 @@ int piko_reyes_destroy(piko_reyes_pipe *p) {
     rfree(d);
   }
   if (p->e_rank) (void)hipEventDestroy(p->e_rank);
-  if (p->e_res) (void)hipEventDestroy(p->e_res);
   for (hipEvent_t e : p->e_slot)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : p->e_res)
+    if (e) (void)hipEventDestroy(e);
+  if (p->vstream) (void)hipStreamDestroy(p->vstream);
   if (p->rstream) (void)hipStreamDestroy(p->rstream);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
