@@ -970,6 +970,7 @@ __global__ __launch_bounds__(64 * kTileWaves) void k_reyes_tile_sample(
   __shared__ unsigned long long s_key[kTileWaves][64];
   __shared__ uint32_t s_box[kTileWaves][64];  // x0 | y0 << 13 | width << 26
   __shared__ int s_pre[kTileWaves][64];
+  __shared__ int s_own[kTileWaves][64];  // pair owners: start marks per chunk of 64 pairs
   // queue: order index; leaf (in this wave's batch) | clipped box << 6
   // (x0, y0, x1, y1 relative to the tile, 6 bits each)
   __shared__ uint32_t s_qord[kTileWaves][kTileQueue], s_qlr[kTileWaves][kTileQueue];
@@ -1074,14 +1075,28 @@ __global__ __launch_bounds__(64 * kTileWaves) void k_reyes_tile_sample(
       s_box[wv][lane] = (uint32_t)x0 | ((uint32_t)y0 << 13) | ((uint32_t)wid << 26);
       s_pre[wv][lane] = inc - npx;
       wave_sync();
-      for (int p = lane; p < total; p += 64) {
-        int q = 0;
-#pragma unroll
-        for (int step = 32; step > 0; step >>= 1)
-          if (q + step < n && s_pre[wv][q + step] <= p) q += step;
+      // pair owners and rows as in k_reyes_dice
+      const int pre = inc - npx;
+      const bool fquot = total < (1 << 21);
+      int carry = 0;
+      for (int p0 = 0; p0 < total; p0 += 64) {
+        s_own[wv][lane] = -1;
+        wave_sync();
+        const int rel = pre - p0;
+        if (npx > 0 && rel >= 0 && rel < 64) s_own[wv][rel] = lane;
+        wave_sync();
+        const int mark = s_own[wv][lane];
+        const unsigned long long mk = __ballot(mark >= 0) & ((2ull << lane) - 1ull);
+        const int got = __shfl(mark, mk ? 63 - __clzll((long long)mk) : 0);
+        const int q = mk ? got : carry;
+        carry = __shfl(q, 63);
+        wave_sync();
+        const int p = p0 + lane;
+        if (p >= total) break;  // the last chunk
         const uint32_t bx = s_box[wv][q];
         const int local = p - s_pre[wv][q], w = (int)(bx >> 26);
-        const int dy = local / w, dx = local - dy * w;
+        const int dy = fquot ? (int)(((float)local + 0.5f) * __builtin_amdgcn_rcpf((float)w)) : local / w;
+        const int dx = local - dy * w;
         const int x = (int)(bx & 0x1fffu) + dx, y = (int)((bx >> 13) & 0x1fffu) + dy;
         const float X[4] = {s_qx[wv][q][0], s_qx[wv][q][1], s_qx[wv][q][2], s_qx[wv][q][3]};
         const float Y[4] = {s_qy[wv][q][0], s_qy[wv][q][1], s_qy[wv][q][2], s_qy[wv][q][3]};
