@@ -25,11 +25,15 @@ int piko_scene_tris_uniform(uint64_t seed, int n, int W, int H, float area_media
                             float z_lo, float z_hi, float *pos, uint32_t *col);
 
 /* World-space height field over [-1,1]^2 (x,z), y = amplitude * value noise
- * with `octaves` octaves; grid x grid quads, two triangles each, row-major.
+ * with `octaves` octaves (lattice frequency base_freq per unit for the first
+ * octave, doubling per octave; piko_scene_heightfield uses 4); grid x grid
+ * quads, two triangles each, row-major.
  * pos/nrm: [2*grid*grid*9] triangle soup of positions and unit normals. */
 long long piko_scene_heightfield_count(int grid);
 int piko_scene_heightfield(uint64_t seed, int grid, float amplitude, int octaves, float *pos,
                            float *nrm);
+int piko_scene_heightfield_freq(uint64_t seed, int grid, float amplitude, int octaves, float base_freq,
+                                float *pos, float *nrm);
 
 /* World-space triangle soup (the uniform-soup workload placed in a cube so a
  * camera can view it): centroids uniform in [-extent, extent]^3, uniform

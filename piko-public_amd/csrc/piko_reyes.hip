@@ -702,11 +702,18 @@ __global__ __launch_bounds__(64) void k_reyes_dice(ReyesArgs a, const float4 *__
         }
         if (any) {
           // A hit whose key cannot beat the sample's current key is dropped
-          // before its atomic.  The keys only decrease within the frame and
-          // were reset by a kernel earlier on this stream, so whatever value
-          // this load returns, however stale, is at least the sample's
-          // minimum so far: a dropped key would not have changed it.  On the
-          // 4,096-patch sphere (back faces drawn too) dice ran 30% faster.
+          // before its atomic.  The keys only decrease within the frame, and
+          // the reset that starts the frame is the previous resolve of this
+          // slot's visibility buffer: in run_frames that resolve runs on
+          // another stream, and this dice is ordered after it only by the
+          // wait on the slot's resolve event (e_res[slot]) made before the
+          // launch.  With that wait in place whatever value this load
+          // returns, however stale, is at least the sample's minimum so far,
+          // so a dropped key would not have changed it; without it a stale
+          // key of the previous frame could drop a winning hit.  The
+          // multi-frame parity tests (frames in one call with regrowth,
+          // repeated frames across camera changes) cover the ordering.  On
+          // the 4,096-patch sphere (back faces drawn too) dice ran 30% faster.
           unsigned long long cur[SPP];
 #pragma unroll
           for (int smp = 0; smp < SPP; ++smp) cur[smp] = dst[smp];
@@ -1592,6 +1599,13 @@ int make_pipe(piko_reyes_pipe **out, const piko_constant_state *cs, const float 
   if (cdiv(W, 1 << tsb) * cdiv(H, 1 << tsb) > kMaxSampleTiles) tsb = 5;
   const bool tiles_fit = cdiv(W, 1 << tsb) * cdiv(H, 1 << tsb) <= kMaxSampleTiles;
   if (opt->sampling == PIKO_SAMPLE_TILED && !tiles_fit) return PIKO_ERR_INVALID;
+  // global sampling scans the (micropolygon, pixel) pairs of a wave's 64
+  // quads in int32: a quad's clipped box holds at most W*H pixels, so the
+  // scan cannot overflow while 64*W*H < 2^31; larger screens take tiled
+  // sampling under AUTO and are refused for GLOBAL
+  const bool global_fits = 64ll * W * H < (1ll << 31);
+  if (opt->sampling == PIKO_SAMPLE_GLOBAL && !global_fits) return PIKO_ERR_INVALID;
+  if (opt->sampling == PIKO_SAMPLE_AUTO && !global_fits && !tiles_fit) return PIKO_ERR_INVALID;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PIKO_ERR_NO_DEVICE;
   piko_reyes_pipe *p = new (std::nothrow) piko_reyes_pipe();
@@ -1621,7 +1635,7 @@ int make_pipe(piko_reyes_pipe **out, const piko_constant_state *cs, const float 
   a.tsb = tsb;
   a.tnx = (int)cdiv(W, 1 << tsb);
   a.tny = (int)cdiv(H, 1 << tsb);
-  p->tiled = opt->sampling == PIKO_SAMPLE_TILED;
+  p->tiled = opt->sampling == PIKO_SAMPLE_TILED || !global_fits;
   a.pt_num = a.dice == 1 ? 16u : 9u;
   a.pair_cap = (uint32_t)std::min<long long>(std::max<long long>(4096, (long long)count * 64), 1ll << 30);
   int rc = PIKO_OK;

@@ -55,8 +55,8 @@ inline double lattice(uint64_t seed, int octave, int64_t ix, int64_t iz) {
 
 inline double fade(double t) { return t * t * t * (t * (t * 6.0 - 15.0) + 10.0); }
 
-double value_noise(uint64_t seed, int octaves, double x, double z) {
-  double sum = 0.0, amp = 1.0, norm = 0.0, freq = 4.0;
+double value_noise(uint64_t seed, int octaves, double base_freq, double x, double z) {
+  double sum = 0.0, amp = 1.0, norm = 0.0, freq = base_freq;
   for (int o = 0; o < octaves; ++o) {
     double fx = x * freq, fz = z * freq;
     double flx = std::floor(fx), flz = std::floor(fz);
@@ -107,7 +107,12 @@ long long piko_scene_heightfield_count(int grid) { return 2ll * grid * grid; }
 
 int piko_scene_heightfield(uint64_t seed, int grid, float amplitude, int octaves, float *pos,
                            float *nrm) {
-  if (grid <= 0 || !pos || !nrm) return -1;
+  return piko_scene_heightfield_freq(seed, grid, amplitude, octaves, 4.0f, pos, nrm);
+}
+
+int piko_scene_heightfield_freq(uint64_t seed, int grid, float amplitude, int octaves, float base_freq,
+                                float *pos, float *nrm) {
+  if (grid <= 0 || !pos || !nrm || !(base_freq > 0.0f)) return -1;
   const int nv = grid + 1;
   const double step = 2.0 / grid;
   // heights on an (nv+2)^2 lattice with a one-vertex apron for central differences
@@ -116,7 +121,7 @@ int piko_scene_heightfield(uint64_t seed, int grid, float amplitude, int octaves
   for (int j = 0; j < na; ++j)
     for (int i = 0; i < na; ++i) {
       const double x = -1.0 + (i - 1) * step, z = -1.0 + (j - 1) * step;
-      h[(size_t)j * na + i] = (double)amplitude * value_noise(seed, octaves, x, z);
+      h[(size_t)j * na + i] = (double)amplitude * value_noise(seed, octaves, base_freq, x, z);
     }
   float *vp = new float[(size_t)nv * nv * 3];
   float *vn = new float[(size_t)nv * nv * 3];

@@ -28,6 +28,11 @@ class StageSummary:
         self.tile_split_size = 0
         self.max_out_prims = 0
         self.process_policy = "Empty"
+        # EndStage wait: the stage whose end this stage waits for (pikoc's
+        # schedule endStageName / endStagePtr, PikoSummary.hpp:198-199, set
+        # from the name in PipeSummary's link pass, PikoSummary.cpp:609-611)
+        self.end_stage_name = ""
+        self.end_stage = None
         self.next_by_port = {}         # out port -> [StageSummary]
         self.prev = []
         self.next = []
@@ -82,6 +87,10 @@ class PipeSummary:
         (sources first, then breadth-first along the ports)."""
         for s in self.stages:
             s.loop = s in s.next
+            # PikoSummary.cpp:609-611: an EndStage wait names the stage it waits for
+            s.end_stage = None
+            if s.wait_policy == "EndStage" and s.end_stage_name:
+                s.end_stage = self.stage(s.end_stage_name)
         self.has_loop = any(s.loop for s in self.stages)
         self.drain = [s for s in self.stages if not [n for n in s.next if n is not s]]
         for s in self.drain:
@@ -111,12 +120,16 @@ class PipeSummary:
     @staticmethod
     def can_fuse(s1, s2, done):
         """PipeSummary::canFuse (PikoSummary.cpp:686-755) for the one
-        schedule this backend plans."""
+        schedule this backend plans.  `done`: the stages whose kernels have
+        been planned before s1's."""
         if s1.type == s2.type:
             return False
         if len([p for p in s2.prev if p is not s2]) > 1:
             return False
         if len(s1.next) > 1:
+            return False
+        # an unresolved EndStage dependency blocks fusion (PikoSummary.cpp:707-709)
+        if s2.end_stage is not None and not any(d is s2.end_stage for d in done):
             return False
         same_core = ((s2.sched_policy == s1.sched_policy and s2.assign_policy == "InBin"
                       and s1.sched_policy == "DirectMap")
@@ -135,7 +148,7 @@ class PipeSummary:
         k = 1
         groups = []
         for s in self.order:
-            if groups and self.can_fuse(groups[-1][-1], s, [g[-1] for g in groups[:-1]]):
+            if groups and self.can_fuse(groups[-1][-1], s, [d for g in groups[:-1] for d in g]):
                 groups[-1].append(s)
             else:
                 groups.append([s])

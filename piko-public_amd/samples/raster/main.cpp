@@ -67,8 +67,8 @@ int main(int argc, char **argv) {
   const int W = 1920, H = 1080, grid = 1000;
   const int n = (int)piko_scene_heightfield_count(grid);
   std::vector<float> pos((size_t)n * 9), nrm((size_t)n * 9);
-  if (piko_scene_heightfield(2, grid, 0.2f, 6, pos.data(), nrm.data()) != 0) return 1;
-  const float eye[3] = {0.0f, 0.3f, 1.0f}, target[3] = {0.0f, -0.3f, 0.0f}, up[3] = {0.0f, 1.0f, 0.0f};
+  if (piko_scene_heightfield_freq(2, grid, 0.2f, 6, 16.0f, pos.data(), nrm.data()) != 0) return 1;
+  const float eye[3] = {0.0f, 0.12f, 0.7f}, target[3] = {0.0f, -0.3f, 0.0f}, up[3] = {0.0f, 1.0f, 0.0f};
   if (piko_camera_view_proj(eye, target, up, 60.0f, (float)W / (float)H, 0.02f, 4.0f, cs.viewProjMatrix) != 0)
     return 1;
 #endif

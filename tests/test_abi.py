@@ -73,3 +73,31 @@ def test_scene_generators_deterministic():
     px = np.array([s[2] * s[3] for s in spec])
     assert ns.min() >= 1000 and ns.max() <= 500000
     assert px.min() >= 0.95 * 256 ** 2 and px.max() <= 1.05 * 2048 ** 2
+
+
+def test_reyes_global_sampling_screen_bound_without_device():
+    """Global sampling scans a wave's (micropolygon, pixel) pairs in int32,
+    so it is refused where 64*W*H reaches 2^31; AUTO falls to tiled sampling
+    there, and a screen neither path can hold is refused."""
+    import sys
+    sys.path.insert(0, os.path.join(REPO, "piko-public_amd"))
+    import piko_amd
+    L = piko_amd.lib()
+    L.piko_reyes_allocate.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(piko_amd.ConstantState),
+                                      ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(piko_amd.ReyesOptions)]
+    h = ctypes.c_void_p()
+
+    def alloc(W, H, sampling):
+        cs = piko_amd.ConstantState()
+        cs.screenSizeX, cs.screenSizeY = W, H
+        opt = piko_amd.ReyesOptions(16.0, 20, 8, 4, 0.0, 0, 0, sampling, 0)
+        return L.piko_reyes_allocate(ctypes.byref(h), ctypes.byref(cs), None, 0, ctypes.byref(opt))
+
+    # 8192 x 4096: 64*W*H = 2^31 exactly -> global refused; 32-px tiles: 256 x 128 > 8192 tiles
+    assert alloc(8192, 4096, piko_amd.SAMPLE_GLOBAL) == -1
+    assert alloc(8192, 4096, piko_amd.SAMPLE_AUTO) == -1
+    # without a device every admissible request stops at the device check (-3)
+    ok = alloc(4096, 4095, piko_amd.SAMPLE_GLOBAL)
+    assert ok in (-3, 0)
+    if ok == 0:
+        L.piko_reyes_destroy(h)

@@ -83,6 +83,32 @@ def test_can_fuse_rules():
     assert not f(stage("a", sched="All"), stage("b", sched="All", wait="EndStage"), [])
 
 
+def test_can_fuse_end_stage_dependency():
+    # PikoSummary.cpp:707-709: a stage waiting for the end of another stage
+    # fuses only once that stage's kernel is done
+    ps = PipeSummary("P", "p.h")
+    for s in (stage("src", T=16), stage("a"), stage("b", wait="EndStage")):
+        ps.add_stage(s)
+    ps.stage("b").end_stage_name = "src"
+    ps.connect("src", "a", 0, 0)
+    ps.connect("a", "b", 0, 0)
+    ps.process_links()
+    src, a, b = ps.stage("src"), ps.stage("a"), ps.stage("b")
+    assert b.end_stage is src
+    assert not PipeSummary.can_fuse(a, b, [])       # src not done: no fusion
+    assert PipeSummary.can_fuse(a, b, [src])        # resolved dependency
+    # src (other bins) has its own kernel before a's, so a and b fuse; a
+    # dependency on a stage planned later keeps them apart
+    assert [p for _, p in ps.kernel_plan()][2] == ["a.process", "emit[0] -> b.assignBin", "b.process"]
+    ps.stage("b").end_stage_name = "b2"
+    late = stage("b2")
+    ps.add_stage(late)
+    ps.connect("b", "b2", 0, 0)
+    ps.process_links()
+    names = [p for _, p in ps.kernel_plan()]
+    assert ["a.process", "emit[0] -> b.assignBin"] in names and any(p[0] == "b.process" for p in names)
+
+
 def test_fused_plan(tmp_path):
     # two InBin stages under LOAD_BALANCE with equal bins fuse into one kernel
     (tmp_path / "p.h").write_text("""
