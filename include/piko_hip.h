@@ -75,11 +75,28 @@ enum { PIKO_INPUT_PRETRANSFORMED = 0, PIKO_INPUT_WORLD = 1 };
  *           counts, scans and scatters each chunk of 4096 triangles into
  *           per-tile lists in LDS; one workgroup per tile then resolves every
  *           fragment in LDS and writes each pixel once.  No global atomics.
- *   DIRECT and TILED keep no sorted queues: the bin queues are built on
- *   demand when asked for (piko_raster_get_bins).
+ *   QUEUE:  one persistent launch per frame pulls its work from a device
+ *           queue: bin items (vertex stage and trivial-reject test of a
+ *           chunk of 1024 triangles, an LDS histogram over the screen tiles,
+ *           one global add per touched tile reserving the chunk's run in
+ *           that tile's queue), then part items (a tile's queue cut into
+ *           parts of at most part_pairs entries, densest tile first; each
+ *           part resolves its fragments in LDS, the parts of a split tile
+ *           merge through scratch slots).  Tiles are the bin size, at least
+ *           16, larger where a screen would have more than 4096 tiles.  Tile
+ *           queues are sized from the counts of the first frame and grow
+ *           (the frame re-runs) when a later frame needs more.
+ *   DIRECT, TILED and QUEUE keep no sorted queues: the bin queues are built
+ *   on demand when asked for (piko_raster_get_bins).
  *   AUTO:   DIRECT for a plain pipe, TILED for a batch.  The environment
- *           variable PIKO_RASTER_MODE=binned|direct|tiled overrides AUTO. */
-enum { PIKO_RASTER_AUTO = 0, PIKO_RASTER_BINNED = 1, PIKO_RASTER_DIRECT = 2, PIKO_RASTER_TILED = 3 };
+ *           variable PIKO_RASTER_MODE=binned|direct|tiled|queue overrides AUTO. */
+enum {
+  PIKO_RASTER_AUTO = 0,
+  PIKO_RASTER_BINNED = 1,
+  PIKO_RASTER_DIRECT = 2,
+  PIKO_RASTER_TILED = 3,
+  PIKO_RASTER_QUEUE = 4
+};
 
 typedef struct {
   int bin_size;           /* raster bin edge T in pixels: 8, 16, 32 or 64 */
@@ -87,8 +104,10 @@ typedef struct {
   int input_kind;         /* PIKO_INPUT_* */
   long long pair_capacity;/* (triangle, bin) queue capacity; 0 = automatic,
                              grows on demand between frames */
-  int waves_per_cu;       /* persistent raster waves per CU; 0 = default (20) */
+  int waves_per_cu;       /* persistent raster waves per CU; 0 = default (20;
+                             QUEUE: 16, four 256-thread workgroups) */
   int raster_mode;        /* PIKO_RASTER_*; 0 = AUTO */
+  int part_pairs;         /* QUEUE: queue entries per part item; 0 = 1024 */
 } piko_raster_options;
 
 typedef struct piko_raster_pipe piko_raster_pipe;
@@ -180,6 +199,8 @@ typedef struct {
   long long max_pixels;    /* initial per-batch pixel capacity (grows) */
   long long pair_capacity; /* 0 = automatic */
   int raster_mode;         /* PIKO_RASTER_* for every batch; 0 = AUTO */
+  int part_pairs;          /* QUEUE: queue entries per part item; 0 = 1024 */
+  int waves_per_cu;        /* QUEUE: persistent waves per CU; 0 = 16 */
 } piko_batch_options;
 
 /* Creates a batch pipe over `input_tris` resident triangles (pos/attr as for

@@ -81,7 +81,8 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh, ui
 struct FrameState {
   uint32_t total;        // (triangle, bin) pairs of this frame (binned path)
   uint32_t overflow;     // sticky: bit 0 pairs exceeded the queues, bit 1 the large-triangle list,
-                         // bit 2 the large triangles' 8x8 tiles exceeded 2^32 (cannot grow)
+                         // bit 2 the large triangles' 8x8 tiles exceeded 2^32 (cannot grow),
+                         // bit 3 a tile queue of the queue path
   uint32_t large_peak;   // most large triangles of one frame since the host last looked
   uint32_t tiles_peak;   // most 8x8 tiles of large triangles of one frame since then
   unsigned long long large_acc;  // direct path, this frame: large triangles listed (low
@@ -1573,6 +1574,8 @@ __global__ void __launch_bounds__(kTRThreads) k_tile_raster(const int4 *__restri
   }
 }
 
+#include "raster_queue.h"
+
 inline int ilog2i(int v) {
   int l = 0;
   while ((1 << l) < v) ++l;
@@ -1653,6 +1656,33 @@ struct piko_raster_pipe {
   int raster_waves = 0;       // persistent raster grid (default)
   int cus = 256;              // compute units of the device
   int raster_persistent = 1;  // 1: persistent pipelined waves; 0: one block per bin
+  // queue path: bin items (chunks), per-tile queue ranges and counters, part
+  // items, the pair buffer and the split tiles' scratch slots
+  QChunk *d_qchunks = nullptr;
+  QChunk *h_qchunks = nullptr;          // pinned staging for the chunk table
+  long long cap_qchunks = 0;
+  int q_nchunks = 0;
+  uint32_t *d_qbase = nullptr, *d_qcap = nullptr, *d_qfill = nullptr, *d_qneed = nullptr;
+  long long cap_qtiles = 0;
+  int q_ntiles = 0;
+  std::vector<uint32_t> q_cap;          // host copy of the tiles' queue capacities
+  bool q_caps_known = false;            // q_cap holds counts of a frame of the current jobs
+  QItem *d_qitems = nullptr;
+  long long cap_qitems = 0;
+  int q_nitems = 0;
+  uint32_t *d_qsplits = nullptr;        // first part's item of each split tile
+  long long cap_qsplits = 0;
+  int q_nsplits = 0;
+  uint32_t *d_qpairs = nullptr;
+  long long cap_qpairs = 0;
+  unsigned long long *d_qscr = nullptr;
+  long long cap_qscr = 0;               // scratch words
+  QState *d_qs = nullptr;
+  int q_QT = 32;                        // LDS tile edge of the launch (largest job tile)
+  int q_part_pairs = 1024;
+  int q_wpc = 16;                       // persistent waves per CU
+  FrameArgs fa_q{};                     // plain pipe job in queue tile numbering
+  long long q_bytes = 0;
   long long large_tiles_hint = -1;  // 8x8 tiles of large triangles last seen (-1: unknown)
   bool last_inline = false;         // the last direct frame drew large triangles in the vertex stage
   long long device_bytes = 0;
@@ -1836,6 +1866,178 @@ int setup_chunks(piko_raster_pipe *p, FrameArgs *jobs, int njobs, long long tota
   return PIKO_OK;
 }
 
+// ---- queue path (host) ---------------------------------------------------
+
+// Tile edge of the queue path for a job asked to bin at T: at least 16, and
+// large enough that one job has at most kQMaxTiles tiles (64 always is for
+// screens up to 4095 x 4095).
+int queue_T(int T, int W, int H) {
+  int tt = std::max(T, 16);
+  while (tt < 64 && cdiv(W, tt) * cdiv(H, tt) > kQMaxTiles) tt *= 2;
+  return tt;
+}
+
+template <typename T>
+void q_free(piko_raster_pipe *p, T *&ptr, long long &cap) {
+  if (ptr) {
+    const long long b = std::max<long long>(cap, 1) * (long long)sizeof(T);
+    p->device_bytes -= b;
+    p->q_bytes -= b;
+  }
+  dfree(ptr);
+  cap = 0;
+}
+
+// Grows a queue-path buffer to at least `need` elements (contents dropped).
+template <typename T>
+int q_grow(piko_raster_pipe *p, T *&ptr, long long &cap, long long need) {
+  if (ptr && need <= cap) return PIKO_OK;
+  q_free(p, ptr, cap);
+  const long long n = std::max<long long>(need + need / 4, 1);
+  int rc = dalloc(p, &ptr, n, &p->q_bytes);
+  if (rc != PIKO_OK) return rc;
+  cap = n;
+  return PIKO_OK;
+}
+
+void free_queue(piko_raster_pipe *p) {
+  q_free(p, p->d_qchunks, p->cap_qchunks);
+  if (p->h_qchunks) (void)hipHostFree(p->h_qchunks);
+  p->h_qchunks = nullptr;
+  long long tc = p->cap_qtiles;
+  q_free(p, p->d_qbase, tc);
+  tc = p->cap_qtiles;
+  q_free(p, p->d_qcap, tc);
+  tc = p->cap_qtiles;
+  q_free(p, p->d_qfill, tc);
+  tc = p->cap_qtiles;
+  q_free(p, p->d_qneed, tc);
+  p->cap_qtiles = 0;
+  q_free(p, p->d_qitems, p->cap_qitems);
+  q_free(p, p->d_qsplits, p->cap_qsplits);
+  q_free(p, p->d_qpairs, p->cap_qpairs);
+  q_free(p, p->d_qscr, p->cap_qscr);
+  long long one = 1;
+  q_free(p, p->d_qs, one);
+}
+
+// Part items and queue ranges from the tiles' capacities: ranges in tile
+// order, items densest tile first, split tiles with their scratch slots.
+int q_build(piko_raster_pipe *p) {
+  const int n = p->q_ntiles;
+  std::vector<uint32_t> base((size_t)n);
+  unsigned long long total = 0;
+  for (int t = 0; t < n; ++t) {
+    base[t] = (uint32_t)total;
+    total += p->q_cap[t];
+  }
+  if (total >= (unsigned long long)kQNoSlot) return PIKO_ERR_CAPACITY;
+  const long long P = p->q_part_pairs;
+  std::vector<int> order((size_t)n);
+  for (int t = 0; t < n; ++t) order[t] = t;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return p->q_cap[a] > p->q_cap[b]; });
+  std::vector<QItem> items;
+  std::vector<uint32_t> splits;
+  items.reserve((size_t)n + (size_t)(total / (unsigned long long)P) + 1);
+  long long slots = 0;
+  for (int t : order) {
+    const uint32_t np = (uint32_t)std::max<long long>(1, cdiv((long long)p->q_cap[t], P));
+    const uint32_t slot = np > 1 ? (uint32_t)slots : 0u;
+    if (np > 1) {
+      slots += np;
+      splits.push_back((uint32_t)items.size());
+    }
+    for (uint32_t k = 0; k < np; ++k) items.push_back(QItem{(uint32_t)t, k, np, slot, base[t], p->q_cap[t], 0u, 0u});
+  }
+  if (items.size() > 0x7FFFFFFFull || slots > 0xFFFFFFFFll) return PIKO_ERR_CAPACITY;
+  int rc;
+  HIPOK(hipStreamSynchronize(p->stream));  // buffers may still be in use
+  if ((rc = q_grow(p, p->d_qpairs, p->cap_qpairs, (long long)total)) != PIKO_OK) return rc;
+  if ((rc = q_grow(p, p->d_qitems, p->cap_qitems, (long long)items.size())) != PIKO_OK) return rc;
+  if ((rc = q_grow(p, p->d_qscr, p->cap_qscr, slots * p->q_QT * p->q_QT)) != PIKO_OK) return rc;
+  if ((rc = q_grow(p, p->d_qsplits, p->cap_qsplits, (long long)splits.size())) != PIKO_OK) return rc;
+  if (n > 0) {
+    HIPOK(hipMemcpyAsync(p->d_qbase, base.data(), (size_t)n * 4, hipMemcpyHostToDevice, p->stream));
+    HIPOK(hipMemcpyAsync(p->d_qcap, p->q_cap.data(), (size_t)n * 4, hipMemcpyHostToDevice, p->stream));
+  }
+  if (!items.empty())
+    HIPOK(hipMemcpyAsync(p->d_qitems, items.data(), items.size() * sizeof(QItem), hipMemcpyHostToDevice,
+                         p->stream));
+  if (!splits.empty())
+    HIPOK(hipMemcpyAsync(p->d_qsplits, splits.data(), splits.size() * 4, hipMemcpyHostToDevice, p->stream));
+  p->q_nitems = (int)items.size();
+  p->q_nsplits = (int)splits.size();
+  HIPOK(hipStreamSynchronize(p->stream));  // the host vectors go out of scope
+  return PIKO_OK;
+}
+
+// Chunks and tiles of the jobs (device order, tile numbering from bin_base);
+// capacities are kept when the jobs are the ones already set up.
+int q_setup(piko_raster_pipe *p, const FrameArgs *jobs, int njobs, long long total_tris) {
+  long long nch = 0, ntiles = 0;
+  int QT = 32;
+  for (int j = 0; j < njobs; ++j) {
+    const long long end = j + 1 < njobs ? (long long)jobs[j + 1].vbase : total_tris;
+    nch += cdiv(end - (long long)jobs[j].vbase, kQChunk);
+    ntiles = std::max<long long>(ntiles, (long long)jobs[j].bin_base + jobs[j].nbins);
+    if ((1 << jobs[j].lgT) > QT) QT = 1 << jobs[j].lgT;
+  }
+  if (nch > 0x7FFFFFFFll || ntiles > 0x7FFFFFFFll) return PIKO_ERR_CAPACITY;
+  HIPOK(hipStreamSynchronize(p->stream));  // staging and counters may still be in use
+  int rc;
+  if (nch > p->cap_qchunks || !p->h_qchunks) {
+    q_free(p, p->d_qchunks, p->cap_qchunks);
+    if (p->h_qchunks) (void)hipHostFree(p->h_qchunks);
+    p->h_qchunks = nullptr;
+    long long cap = 0;
+    if ((rc = q_grow(p, p->d_qchunks, cap, nch)) != PIKO_OK) return rc;
+    p->cap_qchunks = cap;
+    if (hipHostMalloc((void **)&p->h_qchunks, (size_t)cap * sizeof(QChunk)) != hipSuccess) {
+      (void)hipGetLastError();
+      p->h_qchunks = nullptr;
+      return PIKO_ERR_OOM;
+    }
+  }
+  long long k = 0;
+  for (int j = 0; j < njobs; ++j) {
+    const long long end = j + 1 < njobs ? (long long)jobs[j + 1].vbase : total_tris;
+    for (long long t0 = jobs[j].vbase; t0 < end; t0 += kQChunk, ++k)
+      p->h_qchunks[k] = QChunk{(uint32_t)j, (uint32_t)t0, (uint32_t)std::min<long long>(kQChunk, end - t0), 0u};
+  }
+  p->q_nchunks = (int)nch;
+  if (nch > 0)
+    HIPOK(hipMemcpyAsync(p->d_qchunks, p->h_qchunks, (size_t)nch * sizeof(QChunk), hipMemcpyHostToDevice,
+                         p->stream));
+  const bool same_tiles = p->q_caps_known && ntiles == p->q_ntiles && QT == p->q_QT;
+  if (!p->d_qs) {
+    long long one = 0;
+    if ((rc = q_grow(p, p->d_qs, one, 1)) != PIKO_OK) return rc;
+    HIPOK(hipMemsetAsync(p->d_qs, 0, sizeof(QState), p->stream));
+  }
+  if (ntiles > p->cap_qtiles) {
+    long long tc = p->cap_qtiles;
+    q_free(p, p->d_qbase, tc);
+    tc = p->cap_qtiles;
+    q_free(p, p->d_qcap, tc);
+    tc = p->cap_qtiles;
+    q_free(p, p->d_qfill, tc);
+    tc = p->cap_qtiles;
+    q_free(p, p->d_qneed, tc);
+    const long long cap = std::max<long long>(ntiles, 1);
+    for (uint32_t **a : {&p->d_qbase, &p->d_qcap, &p->d_qfill, &p->d_qneed})
+      if ((rc = dalloc(p, a, cap, &p->q_bytes)) != PIKO_OK) return rc;
+    p->cap_qtiles = cap;
+    HIPOK(hipMemsetAsync(p->d_qfill, 0, (size_t)cap * 4, p->stream));
+  }
+  p->q_QT = QT;
+  if (!same_tiles) {
+    p->q_ntiles = (int)ntiles;
+    p->q_cap.assign((size_t)ntiles, 0u);
+    p->q_caps_known = false;
+  }
+  return q_build(p);
+}
+
 void free_bins(piko_raster_pipe *p) {
   dfree(p->d_bstart);
   dfree(p->d_bend);
@@ -1869,6 +2071,7 @@ const char *kernel_name(const piko_raster_pipe *p, int k) {
     return k < 3 ? direct[k] : "";
   }
   if (p->last_mode == PIKO_RASTER_TILED) return k == 0 ? "k_tile_bin" : "k_tile_raster";
+  if (p->last_mode == PIKO_RASTER_QUEUE) return k == 0 ? "k_qbin" : (k == 1 ? "k_qraster" : "k_qmerge");
   static const char *head[] = {"k_vs_count", "k_scan_tiles", "k_emit_pairs"};
   static const char *pass[] = {"k_sort_count", "k_sort_rows", "k_sort_scatter"};
   if (k < 3) return head[k];
@@ -2060,12 +2263,97 @@ void enqueue_tiled(piko_raster_pipe *p, Marks &mark) {
   }
 }
 
+// The queue frame's launches: the bin launch, then (raster) the persistent
+// raster launch over the part items and the merge of the split tiles.
+void launch_queue(piko_raster_pipe *p, bool raster, Marks *mark) {
+  QArgs qa{p->d_pos,    p->d_attr,  p->d_geo,    p->d_qpairs,   p->d_qbase, p->d_qcap,
+           p->d_qfill,  p->d_qneed, p->d_qitems, p->d_qsplits,  p->d_qchunks, p->d_qscr,
+           p->d_jobs,   p->d_qs,    p->d_fs,     p->d_color,    p->d_depth,
+           (int)p->jobs.size(), p->q_nchunks, p->q_nitems, p->q_part_pairs, p->q_nsplits};
+  const FrameArgs &fa = p->fa_q;
+  const unsigned bgrid = (unsigned)std::max(p->q_nchunks, 1);
+#define PIKO_QB_LAUNCH(KIND)                                                           \
+  do {                                                                                 \
+    if (p->multi) k_qbin<KIND, true><<<bgrid, kQThreads, 0, p->stream>>>(qa, fa);     \
+    else k_qbin<KIND, false><<<bgrid, kQThreads, 0, p->stream>>>(qa, fa);             \
+  } while (0)
+  if (p->kind == PIKO_INPUT_PRETRANSFORMED) PIKO_QB_LAUNCH(PIKO_INPUT_PRETRANSFORMED);
+  else PIKO_QB_LAUNCH(PIKO_INPUT_WORLD);
+#undef PIKO_QB_LAUNCH
+  if (mark) (*mark)();
+  if (!raster) return;
+  const long long per_cu = std::max(1, p->q_wpc / (kQThreads / 64));
+  const int rgrid = (int)std::max<long long>(1, std::min<long long>(p->q_nitems, (long long)p->cus * per_cu));
+#define PIKO_QR_LAUNCH(QT, KIND)                                                                    \
+  do {                                                                                              \
+    if (p->multi) k_qraster<QT, KIND, true><<<rgrid, kQThreads, 0, p->stream>>>(qa, fa);           \
+    else k_qraster<QT, KIND, false><<<rgrid, kQThreads, 0, p->stream>>>(qa, fa);                   \
+    if (mark) (*mark)();                                                                            \
+    if (p->q_nsplits > 0) {                                                                         \
+      if (p->multi) k_qmerge<QT, KIND, true><<<p->q_nsplits, kQThreads, 0, p->stream>>>(qa, fa);   \
+      else k_qmerge<QT, KIND, false><<<p->q_nsplits, kQThreads, 0, p->stream>>>(qa, fa);           \
+      if (mark) (*mark)();                                                                          \
+    }                                                                                               \
+  } while (0)
+  if (p->q_QT > 32) {
+    if (p->kind == PIKO_INPUT_PRETRANSFORMED) PIKO_QR_LAUNCH(64, PIKO_INPUT_PRETRANSFORMED);
+    else PIKO_QR_LAUNCH(64, PIKO_INPUT_WORLD);
+  } else {
+    if (p->kind == PIKO_INPUT_PRETRANSFORMED) PIKO_QR_LAUNCH(32, PIKO_INPUT_PRETRANSFORMED);
+    else PIKO_QR_LAUNCH(32, PIKO_INPUT_WORLD);
+  }
+#undef PIKO_QR_LAUNCH
+}
+
+// First frame of a job set on the queue path: the bin items alone count each
+// tile's entries (every run is refused, the ranges being empty), and the
+// ranges are sized from those counts with some room for camera changes.
+int q_count(piko_raster_pipe *p) {
+  launch_queue(p, false, nullptr);
+  HIPOK(hipGetLastError());
+  const int n = p->q_ntiles;
+  std::vector<uint32_t> cnt((size_t)std::max(n, 1), 0u);
+  if (n > 0) HIPOK(hipMemcpyAsync(cnt.data(), p->d_qfill, (size_t)n * 4, hipMemcpyDeviceToHost, p->stream));
+  HIPOK(hipStreamSynchronize(p->stream));
+  if (n > 0) HIPOK(hipMemsetAsync(p->d_qfill, 0, (size_t)n * 4, p->stream));
+  HIPOK(hipMemsetAsync(&p->d_fs->overflow, 0, 4, p->stream));
+  for (int t = 0; t < n; ++t) p->q_cap[t] = cnt[t] ? cnt[t] + cnt[t] / 8 + 32 : 0u;
+  p->q_caps_known = true;
+  return q_build(p);
+}
+
+// Queue frame: one persistent launch.
+int enqueue_queue(piko_raster_pipe *p, Marks &mark) {
+  if (!p->q_caps_known) {
+    int rc = q_count(p);
+    if (rc != PIKO_OK) return rc;
+  }
+  launch_queue(p, true, &mark);
+  return PIKO_OK;
+}
+
+// Grows the tile queues that overflowed in the last queue frame, from the
+// counts its parts recorded.
+int q_regrow(piko_raster_pipe *p) {
+  const int n = p->q_ntiles;
+  std::vector<uint32_t> need((size_t)std::max(n, 1), 0u);
+  if (n > 0) HIPOK(hipMemcpyAsync(need.data(), p->d_qneed, (size_t)n * 4, hipMemcpyDeviceToHost, p->stream));
+  HIPOK(hipStreamSynchronize(p->stream));
+  for (int t = 0; t < n; ++t)
+    if (need[t] > p->q_cap[t]) p->q_cap[t] = need[t] + need[t] / 8 + 32;
+  return q_build(p);
+}
+
 // Enqueues one frame of the current job set.  ev (optional) receives an event
 // after each kernel.
 int enqueue_frame(piko_raster_pipe *p, hipEvent_t *ev, int *nev) {
   Marks mark{ev, p->stream};
   p->last_mode = p->mode;
-  if (p->mode == PIKO_RASTER_TILED) {
+  if (p->mode == PIKO_RASTER_QUEUE) {
+    const int rc = enqueue_queue(p, mark);
+    if (rc != PIKO_OK) return rc;
+    p->bins_current = false;
+  } else if (p->mode == PIKO_RASTER_TILED) {
     enqueue_tiled(p, mark);
     p->bins_current = false;
   } else if (p->mode == PIKO_RASTER_DIRECT) {
@@ -2098,6 +2386,8 @@ int check_capacity(piko_raster_pipe *p, int *rerun) {
   if (rc != PIKO_OK) return rc;
   *rerun = 0;
   if (fs.overflow & 4u) return PIKO_ERR_CAPACITY;
+  const bool queue_short = (fs.overflow & 8u) != 0;
+  if (queue_short && (rc = q_regrow(p)) != PIKO_OK) return rc;
   const bool pairs_short = (fs.overflow & 1u) || (p->bins_current && (long long)fs.total > p->cap);
   const bool large_short = (fs.overflow & 2u) != 0;
   if (pairs_short) {
@@ -2121,7 +2411,7 @@ int check_capacity(piko_raster_pipe *p, int *rerun) {
     HIPOK(hipMemcpyAsync(&p->d_fs->large_peak, &zero[1], 8, hipMemcpyHostToDevice, p->stream));
     HIPOK(hipStreamSynchronize(p->stream));
   }
-  *rerun = pairs_short || large_short;
+  *rerun = pairs_short || large_short || queue_short;
   return PIKO_OK;
 }
 
@@ -2155,7 +2445,8 @@ int validate_opts(const piko_constant_state *cs, int count, const piko_raster_op
   if (opt->subpixel_bits < 1 || opt->subpixel_bits > 8) return PIKO_ERR_INVALID;
   if (opt->input_kind != PIKO_INPUT_PRETRANSFORMED && opt->input_kind != PIKO_INPUT_WORLD)
     return PIKO_ERR_INVALID;
-  if (opt->raster_mode < PIKO_RASTER_AUTO || opt->raster_mode > PIKO_RASTER_TILED) return PIKO_ERR_INVALID;
+  if (opt->raster_mode < PIKO_RASTER_AUTO || opt->raster_mode > PIKO_RASTER_QUEUE) return PIKO_ERR_INVALID;
+  if (opt->part_pairs < 0 || opt->part_pairs > (1 << 20) || opt->waves_per_cu < 0) return PIKO_ERR_INVALID;
   return PIKO_OK;
 }
 
@@ -2190,6 +2481,7 @@ int create_common(piko_raster_pipe *p, int kind, int S, long long tris, long lon
     mode = p->multi ? PIKO_RASTER_TILED : PIKO_RASTER_DIRECT;
     if (m && std::strcmp(m, "binned") == 0) mode = PIKO_RASTER_BINNED;
     if (m && std::strcmp(m, "direct") == 0) mode = PIKO_RASTER_DIRECT;
+    if (m && std::strcmp(m, "queue") == 0) mode = PIKO_RASTER_QUEUE;
   }
   p->mode = p->last_mode = mode;
   {
@@ -2261,6 +2553,12 @@ int plain_pipe(piko_raster_pipe **out, const piko_constant_state *cs, const floa
   rc = create_common(p, opt->input_kind, opt->subpixel_bits, count, p->npix, p->nbins, opt->pair_capacity,
                      opt->raster_mode);
   p->classes.assign(1, RasterClass{p->T, 0, p->nbins, opt->waves_per_cu > 0 ? opt->waves_per_cu * p->cus : 0});
+  if (rc == PIKO_OK && p->mode == PIKO_RASTER_QUEUE) {
+    p->fa_q = make_args(cs, opt->subpixel_bits, queue_T(p->T, p->W, p->H));
+    p->q_part_pairs = opt->part_pairs > 0 ? opt->part_pairs : 1024;
+    p->q_wpc = opt->waves_per_cu > 0 ? opt->waves_per_cu : 16;
+    rc = q_setup(p, &p->fa_q, 1, count);
+  }
   if (rc == PIKO_OK && p->mode == PIKO_RASTER_TILED) {
     const int tt = tiled_T(p->T, p->W, p->H);
     p->fa_t = make_args(cs, opt->subpixel_bits, tt);
@@ -2308,6 +2606,7 @@ int piko_raster_destroy(piko_raster_pipe *p) {
   free_pix(p);
   free_bins(p);
   free_chunks(p);
+  free_queue(p);
   dfree(p->d_rowtot);
   dfree(p->d_fs);
   dfree(p->d_jobs);
@@ -2336,6 +2635,7 @@ int piko_raster_prepare(piko_raster_pipe *p, const piko_constant_state *cs) {
   if (cs->screenSizeX != p->W || cs->screenSizeY != p->H) return PIKO_ERR_INVALID;
   p->fa = make_args(cs, p->S, p->T);
   p->jobs.assign(1, p->fa);
+  if (p->mode == PIKO_RASTER_QUEUE) p->fa_q = make_args(cs, p->S, 1 << p->fa_q.lgT);
   if (p->mode == PIKO_RASTER_TILED) {
     FrameArgs ft = make_args(cs, p->S, p->tclass.T);
     ft.chunk_base = p->fa_t.chunk_base;
@@ -2484,13 +2784,16 @@ int piko_batch_create(piko_raster_pipe **out, const float *pos, const void *attr
   if (opt->subpixel_bits < 1 || opt->subpixel_bits > 8) return PIKO_ERR_INVALID;
   if (opt->input_kind != PIKO_INPUT_PRETRANSFORMED && opt->input_kind != PIKO_INPUT_WORLD) return PIKO_ERR_INVALID;
   if (opt->max_jobs <= 0 || opt->max_tris < 0 || opt->max_pixels <= 0) return PIKO_ERR_INVALID;
-  if (opt->raster_mode < PIKO_RASTER_AUTO || opt->raster_mode > PIKO_RASTER_TILED) return PIKO_ERR_INVALID;
+  if (opt->raster_mode < PIKO_RASTER_AUTO || opt->raster_mode > PIKO_RASTER_QUEUE) return PIKO_ERR_INVALID;
+  if (opt->part_pairs < 0 || opt->part_pairs > (1 << 20) || opt->waves_per_cu < 0) return PIKO_ERR_INVALID;
   if (input_tris > 0 && (!pos || !attr)) return PIKO_ERR_INVALID;
   piko_raster_pipe *p = new (std::nothrow) piko_raster_pipe();
   if (!p) return PIKO_ERR_OOM;
   p->multi = true;
   p->input_tris = input_tris;
   p->cap_jobs = opt->max_jobs;
+  p->q_part_pairs = opt->part_pairs > 0 ? opt->part_pairs : 1024;
+  p->q_wpc = opt->waves_per_cu > 0 ? opt->waves_per_cu : 16;
   const int bins = (int)std::min<long long>(opt->max_pixels / 64 + 1024, 1ll << 30);
   int rc = create_common(p, opt->input_kind, opt->subpixel_bits, opt->max_tris, opt->max_pixels, bins,
                          opt->pair_capacity, opt->raster_mode);
@@ -2541,6 +2844,7 @@ int piko_batch_query(piko_raster_pipe *p) {
 // Bin (tile) edge a job is rasterised with: its own in the binned and direct
 // paths, the tiled path's tile edge otherwise.
 static int job_T(const piko_raster_pipe *p, const piko_raster_job &jb) {
+  if (p->mode == PIKO_RASTER_QUEUE) return queue_T(jb.bin_size, jb.cs.screenSizeX, jb.cs.screenSizeY);
   return p->mode == PIKO_RASTER_TILED ? tiled_T(jb.bin_size, jb.cs.screenSizeX, jb.cs.screenSizeY) : jb.bin_size;
 }
 
@@ -2613,6 +2917,7 @@ int piko_batch_submit(piko_raster_pipe *p, const piko_raster_job *jobs, int njob
   p->npix = (long long)fbb;
   p->passes = radix_passes(p->nbins);
   if (p->mode == PIKO_RASTER_TILED && (rc = setup_chunks(p, p->jobs.data(), njobs, tris)) != PIKO_OK) return rc;
+  if (p->mode == PIKO_RASTER_QUEUE && (rc = q_setup(p, p->jobs.data(), njobs, tris)) != PIKO_OK) return rc;
   std::memcpy(p->h_jobs, p->jobs.data(), (size_t)njobs * sizeof(FrameArgs));
   HIPOK(hipMemcpyAsync(p->d_jobs, p->h_jobs, (size_t)njobs * sizeof(FrameArgs), hipMemcpyHostToDevice, p->stream));
   rc = enqueue_frame(p, nullptr, nullptr);

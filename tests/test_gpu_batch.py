@@ -11,7 +11,8 @@ from piko_amd import scenes
 
 pytestmark = pytest.mark.gpu
 
-MODES = {"binned": piko_amd.RASTER_BINNED, "direct": piko_amd.RASTER_DIRECT, "tiled": piko_amd.RASTER_TILED}
+MODES = {"binned": piko_amd.RASTER_BINNED, "direct": piko_amd.RASTER_DIRECT, "tiled": piko_amd.RASTER_TILED,
+         "queue": piko_amd.RASTER_QUEUE}
 
 
 def log(msg):

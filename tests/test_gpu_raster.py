@@ -1,8 +1,9 @@
 """GPU parity: the HIP raster pipeline against the oracle, bit for bit
 (colour words, depth bits and the per-bin queues), through the C-ABI.  Every
 frame is checked in every raster mode: binned (queues sorted by bin, one wave
-per bin), direct (per-pixel visibility words, no queues) and tiled (per-chunk
-tile lists, LDS visibility per tile)."""
+per bin), direct (per-pixel visibility words, no queues), tiled (per-chunk
+tile lists, LDS visibility per tile) and queue (one persistent launch pulling
+bin and tile-part items from a device queue)."""
 import sys
 
 import numpy as np
@@ -18,12 +19,13 @@ def log(msg):
     sys.stdout.flush()
 
 
-MODES = ("binned", "direct", "tiled")
+MODES = ("binned", "direct", "tiled", "queue")
 
 
 def gpu_frame(scene, T, S, pair_capacity=0, mode="tiled"):
     import piko_amd
-    rm = {"binned": piko_amd.RASTER_BINNED, "direct": piko_amd.RASTER_DIRECT, "tiled": piko_amd.RASTER_TILED}[mode]
+    rm = {"binned": piko_amd.RASTER_BINNED, "direct": piko_amd.RASTER_DIRECT, "tiled": piko_amd.RASTER_TILED,
+          "queue": piko_amd.RASTER_QUEUE}[mode]
     with piko_amd.RasterPipe().allocate(scene, T, S, pair_capacity, raster_mode=rm) as pipe:
         pipe.prepare()
         pipe.run_single()
