@@ -43,7 +43,9 @@ typedef enum {
   PIKO_ERR_CAPACITY = -4,    /* a queue would overflow: reported, never
                                 wrapped (Bin::insert wraps, datatypes.h:88-93) */
   PIKO_ERR_OOM = -5,         /* device allocation failed */
-  PIKO_ERR_BUSY = -6         /* admission refused (serving budget) */
+  PIKO_ERR_BUSY = -6,        /* admission refused (serving budget) */
+  PIKO_ERR_SCRATCH = -7      /* larger than the fixed scratch of
+                                piko_raster_frame_noalloc */
 } piko_status;
 
 const char *piko_status_string(int status);
@@ -153,6 +155,23 @@ int piko_raster_profile_frames(piko_raster_pipe *pipe, int nframes, float *kerne
 int piko_raster_get_data(piko_raster_pipe *pipe, uint32_t *color);
 /* Depth buffer, W*H floats (MutableState::zBuffer, builtinTypes.h:35-40). */
 int piko_raster_get_depth(piko_raster_pipe *pipe, float *depth);
+
+/* One frame that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The same frame as piko_raster_allocate_device + run_single + get_data /
+ * get_depth on the DIRECT path, with the inputs (d_pos, d_attr as for
+ * piko_raster_allocate_device) and the outputs (d_color: W*H ABGR8 words,
+ * d_depth: W*H floats, row 0 at the bottom) in caller-owned device memory.
+ * Every per-frame buffer lives in one file-scope device array of about
+ * 65 MB that each GPU holds until the process ends: at most 1,048,576
+ * triangles, 2,097,152 pixels and 65,536 large triangles (bbox over 16,384
+ * pixels) per call; larger calls return PIKO_ERR_SCRATCH.  Calls that
+ * overlap on one GPU share that array and race: issue them one at a time.
+ * Runs on the null stream of the current device and returns when the frame
+ * is done. */
+int piko_raster_frame_noalloc(const piko_constant_state *cs, const float *d_pos, const void *d_attr, int count,
+                              int input_kind, int subpixel_bits, uint32_t *d_color, float *d_depth);
 
 /* Raster-stage bin queues of the last frame (Stage::getData, stage.h:
  * 331-333): CSR, bin_start[nbins+1], bin_tris[pairs] in submission order.

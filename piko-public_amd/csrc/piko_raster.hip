@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -2581,6 +2582,55 @@ int plain_pipe(piko_raster_pipe **out, const piko_constant_state *cs, const floa
   return PIKO_OK;
 }
 
+// ---- allocation-free frame ---------------------------------------------------
+//
+// piko_raster_frame_noalloc renders one direct-path frame with every per-frame
+// buffer in `scratch`, a file-scope device array of fixed size: the frame
+// state, a 48-byte setup record per triangle, a visibility word per pixel and
+// the large-triangle list.  Each device holds its own copy until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress
+// returns for the current device.
+constexpr long long kNaMaxTris = 1ll << 20;          // triangles per call
+constexpr long long kNaMaxPix = 1ll << 21;           // pixels per call (1920 x 1080 fits)
+constexpr uint32_t kNaMaxLarge = 1u << 16;           // large triangles per frame
+constexpr size_t kNaGeoOff = 256;                    // after the frame state
+constexpr size_t kNaVisOff = kNaGeoOff + (size_t)kNaMaxTris * 48;
+constexpr size_t kNaLargeOff = kNaVisOff + (size_t)kNaMaxPix * 8;
+constexpr size_t kNaTileOff = kNaLargeOff + (size_t)kNaMaxLarge * 4;
+constexpr size_t kNaBytes = kNaTileOff + (size_t)kNaMaxLarge * 4;
+static_assert(sizeof(FrameState) <= kNaGeoOff, "frame state fits its slot");
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+// The frame state zeroed and the frame's visibility words set to empty: the
+// array keeps whatever the last call on the device left there.
+__global__ void k_na_clear(FrameState *fs, unsigned long long *vis, long long npix) {
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g == 0) *fs = FrameState{};
+  for (long long k = g; k < npix; k += (long long)gridDim.x * blockDim.x) vis[k] = kVisSentinel;
+}
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu;
+unsigned char *g_na_ptr[kNaMaxDevices] = {};
+
+// Device address of the current device's scratch array, looked up once per
+// device (with that device current) and kept.
+int na_scratch(unsigned char **out) {
+  int dev = 0;
+  HIPOK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNaMaxDevices) return PIKO_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    void *ptr = nullptr;
+    HIPOK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return PIKO_ERR_HIP;
+    g_na_ptr[dev] = (unsigned char *)ptr;
+  }
+  *out = g_na_ptr[dev];
+  return PIKO_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2594,6 +2644,7 @@ const char *piko_status_string(int s) {
     case PIKO_ERR_CAPACITY: return "queue capacity exceeded";
     case PIKO_ERR_OOM: return "device out of memory";
     case PIKO_ERR_BUSY: return "admission refused";
+    case PIKO_ERR_SCRATCH: return "larger than the fixed scratch of the allocation-free frame";
     default: return "unknown status";
   }
 }
@@ -2709,6 +2760,56 @@ int piko_raster_get_data(piko_raster_pipe *p, uint32_t *color) {
   if (!p || !color || p->multi) return PIKO_ERR_INVALID;
   HIPOK(hipMemcpyAsync(color, p->d_color, (size_t)p->npix * 4, hipMemcpyDeviceToHost, p->stream));
   HIPOK(hipStreamSynchronize(p->stream));
+  return PIKO_OK;
+}
+
+int piko_raster_frame_noalloc(const piko_constant_state *cs, const float *d_pos, const void *d_attr, int count,
+                              int input_kind, int subpixel_bits, uint32_t *d_color, float *d_depth) {
+  if (!cs || count < 0 || (count > 0 && (!d_pos || !d_attr)) || !d_color || !d_depth) return PIKO_ERR_INVALID;
+  if (!valid_screen(cs->screenSizeX, cs->screenSizeY)) return PIKO_ERR_INVALID;
+  if (subpixel_bits < 1 || subpixel_bits > 8) return PIKO_ERR_INVALID;
+  if (input_kind != PIKO_INPUT_PRETRANSFORMED && input_kind != PIKO_INPUT_WORLD) return PIKO_ERR_INVALID;
+  const long long npix = (long long)cs->screenSizeX * cs->screenSizeY;
+  if ((long long)count > kNaMaxTris || npix > kNaMaxPix) return PIKO_ERR_SCRATCH;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PIKO_ERR_NO_DEVICE;
+  unsigned char *base = nullptr;
+  int rc = na_scratch(&base);
+  if (rc != PIKO_OK) return rc;
+  FrameState *fs = (FrameState *)base;
+  int4 *geo = (int4 *)(base + kNaGeoOff);
+  unsigned long long *vis = (unsigned long long *)(base + kNaVisOff);
+  uint32_t *large = (uint32_t *)(base + kNaLargeOff);
+  uint32_t *tile_off = (uint32_t *)(base + kNaTileOff);
+  const FrameArgs fa = make_args(cs, subpixel_bits, 16);
+  int dev = 0, cus = 256;
+  HIPOK(hipGetDevice(&dev));
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+  hipStream_t st = nullptr;  // the null stream; the call returns when the frame is done
+  k_na_clear<<<(unsigned)std::min<long long>(std::max<long long>(cdiv(npix, 256), 1), 2048), 256, 0, st>>>(fs, vis,
+                                                                                                          npix);
+  const int blocks = (int)std::max<long long>(cdiv(count, kDirThreads), 1);
+  const int S = subpixel_bits;
+  if (input_kind == PIKO_INPUT_PRETRANSFORMED) {
+    k_vs_direct<PIKO_INPUT_PRETRANSFORMED, false><<<blocks, kDirThreads, 0, st>>>(
+        d_pos, count, S, fa, nullptr, 0, geo, vis, large, tile_off, fs, kNaMaxLarge, 0);
+  } else {
+    k_vs_direct<PIKO_INPUT_WORLD, false><<<blocks, kDirThreads, 0, st>>>(d_pos, count, S, fa, nullptr, 0, geo, vis,
+                                                                        large, tile_off, fs, kNaMaxLarge, 0);
+  }
+  k_large_raster<false><<<cus * 2, 256, 0, st>>>(geo, large, tile_off, fs, kNaMaxLarge, fa, nullptr, 0, vis);
+  const unsigned rblocks = (unsigned)(8 * cdiv(std::max<long long>(cdiv(npix, 256), 1), 8));
+  if (input_kind == PIKO_INPUT_PRETRANSFORMED)
+    k_resolve<PIKO_INPUT_PRETRANSFORMED, false><<<rblocks, 256, 0, st>>>(geo, d_attr, vis, fa, nullptr, 0, npix,
+                                                                         d_color, d_depth, fs);
+  else
+    k_resolve<PIKO_INPUT_WORLD, false><<<rblocks, 256, 0, st>>>(geo, d_attr, vis, fa, nullptr, 0, npix, d_color,
+                                                                d_depth, fs);
+  HIPOK(hipGetLastError());
+  FrameState h{};
+  HIPOK(hipMemcpy(&h, fs, sizeof(FrameState), hipMemcpyDeviceToHost));  // waits for the frame
+  // the large-triangle list cannot grow here: a frame that overflowed it is reported
+  if (h.overflow & 6u) return PIKO_ERR_SCRATCH;
   return PIKO_OK;
 }
 

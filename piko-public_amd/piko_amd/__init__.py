@@ -30,7 +30,8 @@ RASTER_AUTO, RASTER_BINNED, RASTER_DIRECT, RASTER_TILED, RASTER_QUEUE = 0, 1, 2,
 SAMPLE_AUTO, SAMPLE_GLOBAL, SAMPLE_TILED = 0, 1, 2  # PIKO_SAMPLE_* (include/piko_hip.h)
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "no HIP device",
-          -4: "queue capacity exceeded", -5: "device out of memory", -6: "admission refused"}
+          -4: "queue capacity exceeded", -5: "device out of memory", -6: "admission refused",
+          -7: "larger than the fixed scratch of the allocation-free frame"}
 
 
 class PikoError(RuntimeError):
@@ -490,6 +491,20 @@ class DeviceArray:
         if self._d:
             lib().piko_rt_free(self._d)
             self._d = ctypes.c_void_p()
+
+
+def frame_noalloc(cs, d_pos, d_attr, count, kind, subpixel_bits, d_color, d_depth):
+    """piko_raster_frame_noalloc: one DIRECT-path frame of `count` device-resident
+    triangles into caller-owned device colour and depth buffers, with no
+    device allocation (per-frame buffers in the library's fixed scratch)."""
+    L = lib()
+    L.piko_raster_frame_noalloc.argtypes = [ctypes.POINTER(ConstantState), ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                            ctypes.c_void_p]
+    k = PIKO_INPUT_PRETRANSFORMED if kind == "pretransformed" else PIKO_INPUT_WORLD
+    return L.piko_raster_frame_noalloc(ctypes.byref(cs), ctypes.c_void_p(d_pos), ctypes.c_void_p(d_attr),
+                                       int(count), k, int(subpixel_bits), ctypes.c_void_p(d_color),
+                                       ctypes.c_void_p(d_depth))
 
 
 def concat_scenes(scene_list):
