@@ -272,12 +272,37 @@ typedef struct piko_server piko_server;
 int piko_server_create(piko_server **server, const float *pos, const void *attr, long long tris, int on_device,
                        const long long *scene_base, const int *scene_count, int nscenes,
                        const piko_server_options *opt);
-/* Replays `reqs` (sorted by arrival) in real time with continuous batching;
- * done_s[i] receives request i's completion time (same clock as arrival_s).
- * keep_color (optional, per request, may hold NULLs) receives framebuffers.
- * PIKO_ERR_BUSY if a request can never fit the batch budget. */
+/* A finished request: its id (from piko_server_submit), completion time in
+ * seconds on the server's clock, and status (PIKO_OK, or the error its batch
+ * met). */
+typedef struct {
+  long long id;
+  double done_s;
+  int status;
+} piko_completion;
+
+/* Online serving (new API above RasterPipe::prepare / run_single,
+ * rasterPipelineFixPt/main.cpp:172-193).  Any thread may submit while batches
+ * run: the request joins the next batch a free pipe forms (continuous
+ * batching by a worker thread that owns the pipes; admission in arrival
+ * order under the per-batch budget).  `color` (optional, host, W*H words)
+ * receives the frame and must stay valid until the request completes;
+ * `arrival_s` is ignored.  *id names the request in completions.
+ * PIKO_ERR_BUSY if the request can never fit the batch budget. */
+int piko_server_submit(piko_server *server, const piko_request *req, uint32_t *color, long long *id);
+/* Moves up to `cap` completions to `out` and returns their number, waiting
+ * up to wait_ms for one when none is ready and requests are outstanding. */
+int piko_server_poll(piko_server *server, piko_completion *out, int cap, int wait_ms);
+/* Replays `reqs` (sorted by arrival) in real time through submit / poll: each
+ * request is submitted at its arrival time.  done_s[i] receives request i's
+ * completion time (same clock as arrival_s).  keep_color (optional, per
+ * request, may hold NULLs) receives framebuffers.  PIKO_ERR_BUSY if a
+ * request can never fit the batch budget; PIKO_ERR_INVALID while submitted
+ * requests are outstanding. */
 int piko_server_replay(piko_server *server, const piko_request *reqs, int n, double *done_s,
                        uint32_t **keep_color, piko_server_stats *st);
+/* Finishes the submitted requests, then frees the pipes and the geometry it
+ * uploaded. */
 int piko_server_destroy(piko_server *server);
 
 /* ---- Reyes pipeline ------------------------------------------------------ */

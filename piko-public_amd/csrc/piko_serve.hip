@@ -1,26 +1,46 @@
 // piko_serve.hip -- serving loop over resident scenes.
 //
 // Scene geometry is uploaded once and stays resident in HBM; K batch pipes
-// (each with its own HIP stream) render from it.  Requests arrive over time
-// (a trace of arrival times); whenever a pipe is free, it takes the requests
-// that have arrived, in arrival order, as long as the batch fits the pipe's
-// memory budget (framebuffer + per-triangle + queue footprint), and launches
-// them as one batch.  While one batch runs, the next forms and launches on
-// another pipe, so batches overlap on the device.  The reference has no
-// serving path; this sits above the pipe methods (rasterPipelineFixPt/
-// main.cpp:172-193 drives one frame at a time).
+// (each with its own HIP stream) render from it.  Clients submit render
+// requests (piko_server_submit) from any thread while batches run and collect
+// finished frames with piko_server_poll.  One worker thread owns the pipes:
+// whenever a pipe is free it takes the requests that have arrived, in arrival
+// order, as long as the batch fits the per-batch device-memory budget
+// (framebuffer + per-triangle + queue footprint), and launches them as one
+// batch; while one batch runs the next forms and launches on another pipe, so
+// batches overlap on the device.  A finished batch's frames are copied to the
+// buffers their requests named and reported as completions.
+// piko_server_replay drives the same loop from a trace of arrival times (the
+// benchmark's harness).  The reference has no serving path; this sits above
+// the pipe methods (rasterPipelineFixPt/main.cpp:172-193 drives one frame at
+// a time).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <new>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "piko_hip.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Pending {
+  long long id;
+  piko_request req;
+  uint32_t *color;  // host framebuffer of the request, or null
+};
+
+}  // namespace
 
 struct piko_server {
   int kind = 0, S = 8;
@@ -33,23 +53,127 @@ struct piko_server {
   float *d_pos = nullptr;
   void *d_attr = nullptr;
   bool owns = false;
+  // requests and completions, guarded by mu
+  std::mutex mu;
+  std::condition_variable cv_work;  // the worker: new requests or stop
+  std::condition_variable cv_done;  // pollers: new completions
+  std::deque<Pending> incoming;
+  std::deque<piko_completion> completed;
+  long long next_id = 0;
+  long long outstanding = 0;  // submitted, not yet completed
+  long long batches = 0, batched_jobs = 0;
+  bool stop = false;
+  std::thread worker;
+  bool worker_started = false;
+  Clock::time_point t0 = Clock::now();
 };
 
 namespace {
 
-// Device bytes one request adds to a batch: colour + depth, setup record
-// (64 B) + count/queue entries per triangle (2 x 8 B pairs, double-buffered
-// radix keys/values).
+// Device bytes one request adds to a batch: colour + depth + the direct
+// path's visibility word per pixel; setup record, bin count and queue entries
+// per triangle.
 long long footprint(const piko_server *s, const piko_request &r) {
-  // colour + depth + the direct path's visibility word per pixel; setup
-  // record, bin count and queue entries per triangle
   const long long px = (long long)r.cs.screenSizeX * r.cs.screenSizeY;
   const long long n = s->scene_count[r.scene];
   return px * 16 + n * (64 + 4 + 4 * 16);
 }
 
-double now_s(std::chrono::steady_clock::time_point t0) {
-  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+double now_s(const piko_server *s) { return std::chrono::duration<double>(Clock::now() - s->t0).count(); }
+
+void complete(piko_server *s, const std::vector<Pending> &batch, int status) {
+  const double t = now_s(s);
+  std::lock_guard<std::mutex> lk(s->mu);
+  for (const Pending &p : batch) s->completed.push_back(piko_completion{p.id, t, status});
+  s->outstanding -= (long long)batch.size();
+  s->cv_done.notify_all();
+}
+
+// The worker: retire finished batches, then give every free pipe the
+// requests that have arrived and fit its budget.  It is the only thread
+// that touches the pipes while it runs.
+void work(piko_server *s) {
+  const int K = (int)s->pipes.size();
+  std::vector<std::vector<Pending>> inflight(K);
+  std::deque<Pending> pending;
+  std::vector<piko_raster_job> jobs;
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      while (!s->incoming.empty()) {
+        pending.push_back(s->incoming.front());
+        s->incoming.pop_front();
+      }
+      bool busy = !pending.empty();
+      for (int k = 0; k < K; ++k) busy = busy || !inflight[k].empty();
+      if (!busy) {
+        if (s->stop) return;
+        s->cv_work.wait(lk, [&] { return s->stop || !s->incoming.empty(); });
+        continue;
+      }
+    }
+    bool progressed = false;
+    for (int k = 0; k < K; ++k) {
+      if (inflight[k].empty()) continue;
+      const int q = piko_batch_query(s->pipes[k]);
+      if (q == 0) continue;
+      int rc = q < 0 ? q : piko_batch_wait(s->pipes[k]);
+      for (size_t j = 0; rc == PIKO_OK && j < inflight[k].size(); ++j)
+        if (inflight[k][j].color) rc = piko_batch_get_color(s->pipes[k], (int)j, inflight[k][j].color);
+      complete(s, inflight[k], rc);
+      inflight[k].clear();
+      progressed = true;
+    }
+    for (int k = 0; k < K && !pending.empty(); ++k) {
+      if (!inflight[k].empty()) continue;
+      long long used = 0;
+      jobs.clear();
+      while (!pending.empty() && (int)jobs.size() < s->max_jobs) {
+        const piko_request &r = pending.front().req;
+        const long long f = footprint(s, r);
+        if (used + f > s->budget) break;
+        used += f;
+        piko_raster_job jb{};
+        jb.count = s->scene_count[r.scene];
+        jb.src_base = s->scene_base[r.scene];
+        jb.bin_size = r.bin_size;
+        jb.cs = r.cs;
+        jobs.push_back(jb);
+        inflight[k].push_back(pending.front());
+        pending.pop_front();
+      }
+      if (jobs.empty()) continue;
+      const int rc = piko_batch_submit(s->pipes[k], jobs.data(), (int)jobs.size());
+      if (rc != PIKO_OK) {
+        complete(s, inflight[k], rc);
+        inflight[k].clear();
+      } else {
+        std::lock_guard<std::mutex> lk(s->mu);
+        ++s->batches;
+        s->batched_jobs += (long long)jobs.size();
+      }
+      progressed = true;
+    }
+    if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
+}
+
+int start_worker(piko_server *s) {
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->worker_started) return PIKO_OK;
+  // the worker issues HIP calls: the device of the creating thread
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return PIKO_ERR_HIP;
+  try {
+    s->worker = std::thread([s, dev] {
+      (void)hipSetDevice(dev);
+      work(s);
+    });
+  } catch (...) {
+    return PIKO_ERR_OOM;
+  }
+  s->worker_started = true;
+  return PIKO_OK;
 }
 
 }  // namespace
@@ -132,6 +256,12 @@ int piko_server_create(piko_server **out, const float *pos, const void *attr, lo
 
 int piko_server_destroy(piko_server *s) {
   if (!s) return PIKO_ERR_INVALID;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    s->cv_work.notify_all();
+  }
+  if (s->worker.joinable()) s->worker.join();  // finishes the requests already submitted
   for (piko_raster_pipe *p : s->pipes) piko_raster_destroy(p);
   if (s->owns) {
     if (s->d_pos) (void)hipFree(s->d_pos);
@@ -141,80 +271,91 @@ int piko_server_destroy(piko_server *s) {
   return PIKO_OK;
 }
 
+int piko_server_submit(piko_server *s, const piko_request *req, uint32_t *color, long long *id) {
+  if (!s || !req || !id) return PIKO_ERR_INVALID;
+  if (req->scene < 0 || req->scene >= (int)s->scene_count.size()) return PIKO_ERR_INVALID;
+  if (footprint(s, *req) > s->budget) return PIKO_ERR_BUSY;  // never admissible
+  int rc = start_worker(s);
+  if (rc != PIKO_OK) return rc;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->stop) return PIKO_ERR_INVALID;
+  *id = s->next_id++;
+  s->incoming.push_back(Pending{*id, *req, color});
+  ++s->outstanding;
+  s->cv_work.notify_one();
+  return PIKO_OK;
+}
+
+int piko_server_poll(piko_server *s, piko_completion *out, int cap, int wait_ms) {
+  if (!s || cap < 0 || (cap > 0 && !out)) return PIKO_ERR_INVALID;
+  std::unique_lock<std::mutex> lk(s->mu);
+  if (s->completed.empty() && wait_ms > 0 && s->outstanding > 0)
+    s->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms),
+                        [&] { return !s->completed.empty() || s->outstanding == 0; });
+  int n = 0;
+  while (n < cap && !s->completed.empty()) {
+    out[n++] = s->completed.front();
+    s->completed.pop_front();
+  }
+  return n;
+}
+
 int piko_server_replay(piko_server *s, const piko_request *reqs, int n, double *done_s, uint32_t **keep_color,
                        piko_server_stats *st) {
-  if (!s || (n > 0 && (!reqs || !done_s))) return PIKO_ERR_INVALID;
+  if (!s || n < 0 || (n > 0 && (!reqs || !done_s))) return PIKO_ERR_INVALID;
   for (int i = 0; i < n; ++i) {
     const piko_request &r = reqs[i];
     if (r.scene < 0 || r.scene >= (int)s->scene_count.size()) return PIKO_ERR_INVALID;
     if (i > 0 && r.arrival_s < reqs[i - 1].arrival_s) return PIKO_ERR_INVALID;  // trace in arrival order
     if (footprint(s, r) > s->budget) return PIKO_ERR_BUSY;  // never admissible
   }
-  const int K = (int)s->pipes.size();
-  std::vector<std::vector<int>> inflight(K);
-  std::deque<int> pending;
-  int next = 0, done = 0, batches = 0;
-  long long batched_jobs = 0;
-  const auto t0 = std::chrono::steady_clock::now();
-  std::vector<piko_raster_job> jobs;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->outstanding > 0 || !s->completed.empty()) return PIKO_ERR_INVALID;  // the server is in use
+  }
+  long long b0, j0;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    b0 = s->batches;
+    j0 = s->batched_jobs;
+  }
+  std::unordered_map<long long, int> index;
+  index.reserve((size_t)n * 2 + 1);
+  const double start = now_s(s);
+  int next = 0, done = 0, rc = PIKO_OK;
+  std::vector<piko_completion> got(64);
   while (done < n) {
-    double t = now_s(t0);
-    while (next < n && reqs[next].arrival_s <= t) pending.push_back(next++);
-    bool progressed = false;
-    // retire finished batches
-    for (int k = 0; k < K; ++k) {
-      if (inflight[k].empty()) continue;
-      const int q = piko_batch_query(s->pipes[k]);
-      if (q < 0) return q;
-      if (q == 0) continue;
-      int rc = piko_batch_wait(s->pipes[k]);
-      if (rc != PIKO_OK) return rc;
-      const double tf = now_s(t0);
-      for (size_t j = 0; j < inflight[k].size(); ++j) {
-        const int r = inflight[k][j];
-        if (keep_color && keep_color[r] && (rc = piko_batch_get_color(s->pipes[k], (int)j, keep_color[r])) != PIKO_OK)
-          return rc;
-        done_s[r] = tf;
-      }
-      done += (int)inflight[k].size();
-      inflight[k].clear();
-      progressed = true;
+    // submit every request whose arrival time has passed
+    const double t = now_s(s) - start;
+    while (next < n && reqs[next].arrival_s <= t) {
+      long long id = -1;
+      const int r = piko_server_submit(s, &reqs[next], keep_color ? keep_color[next] : nullptr, &id);
+      if (r != PIKO_OK) return r;
+      index[id] = next++;
     }
-    // launch: each free pipe takes the arrived requests that fit its budget
-    for (int k = 0; k < K && !pending.empty(); ++k) {
-      if (!inflight[k].empty()) continue;
-      long long used = 0;
-      jobs.clear();
-      while (!pending.empty() && (int)jobs.size() < s->max_jobs) {
-        const piko_request &r = reqs[pending.front()];
-        const long long f = footprint(s, r);
-        if (used + f > s->budget) break;
-        used += f;
-        piko_raster_job jb{};
-        jb.count = s->scene_count[r.scene];
-        jb.src_base = s->scene_base[r.scene];
-        jb.bin_size = r.bin_size;
-        jb.cs = r.cs;
-        jobs.push_back(jb);
-        inflight[k].push_back(pending.front());
-        pending.pop_front();
-      }
-      if (jobs.empty()) continue;
-      const int rc = piko_batch_submit(s->pipes[k], jobs.data(), (int)jobs.size());
-      if (rc != PIKO_OK) return rc;
-      ++batches;
-      batched_jobs += (long long)jobs.size();
-      progressed = true;
+    // wait for completions, at most until the next arrival
+    int wait_ms = 1;
+    if (next < n) wait_ms = (int)std::max(0.0, (reqs[next].arrival_s - (now_s(s) - start)) * 1e3);
+    wait_ms = std::min(wait_ms, 5);
+    const int k = piko_server_poll(s, got.data(), (int)got.size(), wait_ms);
+    if (k < 0) return k;
+    for (int i = 0; i < k; ++i) {
+      const auto it = index.find(got[i].id);
+      if (it == index.end()) continue;
+      done_s[it->second] = got[i].done_s - start;
+      if (got[i].status != PIKO_OK && rc == PIKO_OK) rc = got[i].status;
+      ++done;
     }
-    if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    if (next < n && wait_ms == 0 && k == 0) std::this_thread::yield();
   }
   if (st) {
+    std::lock_guard<std::mutex> lk(s->mu);
     st->requests = n;
-    st->batches = batches;
-    st->wall_s = now_s(t0);
-    st->mean_batch = batches ? (double)batched_jobs / batches : 0.0;
+    st->batches = s->batches - b0;
+    st->wall_s = now_s(s) - start;
+    st->mean_batch = st->batches ? (double)(s->batched_jobs - j0) / st->batches : 0.0;
   }
-  return PIKO_OK;
+  return rc;
 }
 
 }  // extern "C"

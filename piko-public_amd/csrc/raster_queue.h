@@ -413,7 +413,7 @@ __global__ void __launch_bounds__(kQThreads) k_qraster(QArgs qa, FrameArgs fa) {
         int4 gn[3];
         setup(v_nxt, gn);
         const uint32_t v_nn = entry(b0 + 2u * kQThreads + (uint32_t)t);
-        // int32-path triangles go by bbox columns, the others by 8-row spans
+        // each staged triangle goes by bbox columns or by 8-row spans
         uint32_t ucol = 0, uspan = 0;
         if (v_cur != ~0u) {
           const int p[6] = {gc[0].x, gc[0].y, gc[0].z, gc[0].w, gc[1].x, gc[1].y};
@@ -426,8 +426,23 @@ __global__ void __launch_bounds__(kQThreads) k_qraster(QArgs qa, FrameArgs fa) {
           const int w = max(0, x1 - x0), h = max(0, y1 - y0);
           if (w > 0 && h > 0) {
             const bool small = small_bbox(p, S);
-            if (small) ucol = (uint32_t)w;
-            else uspan = (uint32_t)w * (uint32_t)((h + kTileSpan - 1) / kTileSpan);
+            const uint32_t spans = (uint32_t)w * (uint32_t)((h + kTileSpan - 1) / kTileSpan);
+            // int32-path triangles take the cheaper of the two: 8-row spans
+            // test every bbox pixel (about 200 instructions a span), columns
+            // find each column's covered rows (about 100) and then touch only
+            // covered pixels (about 45 each).  Slivers, whose bbox is mostly
+            // empty, go by columns; compact triangles by spans.  The area in
+            // pixels is |2 x signed area| / 2^(2S+1) of the fixed-point edges.
+            bool by_col = false;
+            if (small) {
+              const float area = fabsf(__int_as_float(gc[1].z)) > 0.0f
+                                     ? 0.5f / (fabsf(__int_as_float(gc[1].z)) * (float)(1 << (2 * S)))
+                                     : 0.0f;
+              const float area_clip = fminf(area, (float)(w * h));
+              by_col = 200.0f * (float)spans > 100.0f * (float)w + 45.0f * area_clip;
+            }
+            if (by_col) ucol = (uint32_t)w;
+            else uspan = spans;
             s.xy01[lane] = make_int4(p[0], p[1], p[2], p[3]);
             s.xy2z[lane] = make_int4(p[4], p[5], gc[1].w, gc[2].x);
             s.misc[lane] = make_uint4((uint32_t)gc[2].y, (uint32_t)gc[1].z, (uint32_t)x0 | ((uint32_t)y0 << 16),
@@ -439,7 +454,7 @@ __global__ void __launch_bounds__(kQThreads) k_qraster(QArgs qa, FrameArgs fa) {
 #pragma unroll
         for (int q = 0; q < 3; ++q) gc[q] = gn[q];
         v_nxt = v_nn;
-        // int64-path triangles (rare): 8-row spans, every bbox pixel tested
+        // 8-row spans: every bbox pixel tested (int64-path triangles always)
         const uint32_t sinc = wave_scan_dpp(uspan);
         const uint32_t stot = wave_last(sinc);
         if (stot) {
@@ -454,9 +469,9 @@ __global__ void __launch_bounds__(kQThreads) k_qraster(QArgs qa, FrameArgs fa) {
           }
           wave_fence();
         }
-        // int32-path triangles: each lane takes a bbox column and finds its
-        // covered rows exactly, then the wave's covered pixels are spread
-        // evenly over the lanes
+        // columns (int32-path triangles): each lane takes a bbox column and
+        // finds its covered rows exactly, then the wave's covered pixels are
+        // spread evenly over the lanes
         const uint32_t cinc = wave_scan_dpp(ucol);
         const uint32_t ctot = wave_last(cinc);
         const uint32_t cpre = cinc - ucol;

@@ -14,6 +14,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <thread>
+#include <vector>
 
 #include "../../include/piko_scene.h"
 
@@ -73,6 +75,27 @@ double value_noise(uint64_t seed, int octaves, double base_freq, double x, doubl
   return sum / norm;
 }
 
+// f(begin, end) over [0, n) split into contiguous ranges on the host's
+// threads.  Every generator writes each element from its own counter-based
+// stream, so the output does not depend on the split.
+template <typename F>
+void parallel_for(long long n, F f) {
+  const long long min_per = 1 << 14;
+  unsigned hw = std::thread::hardware_concurrency();
+  long long nt = std::min<long long>(hw ? hw : 1, (n + min_per - 1) / min_per);
+  if (nt <= 1) {
+    f(0ll, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve((size_t)nt);
+  for (long long k = 0; k < nt; ++k) {
+    const long long b = n * k / nt, e = n * (k + 1) / nt;
+    th.emplace_back([=] { f(b, e); });
+  }
+  for (auto &t : th) t.join();
+}
+
 }  // namespace
 
 extern "C" {
@@ -81,7 +104,8 @@ int piko_scene_tris_uniform(uint64_t seed, int n, int W, int H, float area_media
                             float sigma, float z_lo, float z_hi, float *pos, uint32_t *col) {
   if (n < 0 || W <= 0 || H <= 0 || !pos || !col) return -1;
   const double k = std::sqrt(4.0 / std::sqrt(3.0));  // side = k*sqrt(area)
-  for (int i = 0; i < n; ++i) {
+  parallel_for(n, [&](long long i0, long long i1) {
+  for (long long i = i0; i < i1; ++i) {
     Stream r(seed, (uint64_t)i);
     const double cx = r.u01() * W, cy = r.u01() * H;
     const double area = (double)area_median * std::exp((double)sigma * r.normal());
@@ -100,6 +124,7 @@ int piko_scene_tris_uniform(uint64_t seed, int n, int W, int H, float area_media
       col[3 * (size_t)i + v] = to_abgr_host(cr, cg, cb);
     }
   }
+  });
   return 0;
 }
 
@@ -118,14 +143,17 @@ int piko_scene_heightfield_freq(uint64_t seed, int grid, float amplitude, int oc
   // heights on an (nv+2)^2 lattice with a one-vertex apron for central differences
   const int na = nv + 2;
   double *h = new double[(size_t)na * na];
-  for (int j = 0; j < na; ++j)
+  parallel_for(na, [&](long long j0, long long j1) {
+  for (long long j = j0; j < j1; ++j)
     for (int i = 0; i < na; ++i) {
       const double x = -1.0 + (i - 1) * step, z = -1.0 + (j - 1) * step;
       h[(size_t)j * na + i] = (double)amplitude * value_noise(seed, octaves, base_freq, x, z);
     }
+  });
   float *vp = new float[(size_t)nv * nv * 3];
   float *vn = new float[(size_t)nv * nv * 3];
-  for (int j = 0; j < nv; ++j)
+  parallel_for(nv, [&](long long j0, long long j1) {
+  for (long long j = j0; j < j1; ++j)
     for (int i = 0; i < nv; ++i) {
       const size_t a = (size_t)(j + 1) * na + (i + 1);
       const double dhx = (h[a + 1] - h[a - 1]) / (2.0 * step);
@@ -140,19 +168,22 @@ int piko_scene_heightfield_freq(uint64_t seed, int grid, float amplitude, int oc
       vn[3 * v + 1] = (float)(ny / l);
       vn[3 * v + 2] = (float)(nz / l);
     }
+  });
   // two triangles per quad, row-major; winding gives a +y geometric normal,
   // i.e. counter-clockwise on screen for a camera above the surface.
-  size_t t = 0;
-  for (int j = 0; j < grid; ++j)
+  parallel_for(grid, [&](long long j0, long long j1) {
+  for (long long j = j0; j < j1; ++j)
     for (int i = 0; i < grid; ++i) {
       const size_t v00 = (size_t)j * nv + i, v10 = v00 + 1, v01 = v00 + nv, v11 = v01 + 1;
       const size_t tris[2][3] = {{v00, v01, v10}, {v10, v01, v11}};
+      size_t t = 2 * ((size_t)j * grid + i);
       for (int q = 0; q < 2; ++q, ++t)
         for (int k = 0; k < 3; ++k) {
           std::memcpy(pos + 9 * t + 3 * k, vp + 3 * tris[q][k], 12);
           std::memcpy(nrm + 9 * t + 3 * k, vn + 3 * tris[q][k], 12);
         }
     }
+  });
   delete[] h;
   delete[] vp;
   delete[] vn;
@@ -192,7 +223,8 @@ int piko_camera_view_proj(const float *eye, const float *target, const float *up
 int piko_scene_soup_world(uint64_t seed, int n, float extent, float size_median, float sigma, float *pos,
                           float *nrm) {
   if (n < 0 || !pos || !nrm) return -1;
-  for (int i = 0; i < n; ++i) {
+  parallel_for(n, [&](long long i0, long long i1) {
+  for (long long i = i0; i < i1; ++i) {
     Stream r(seed ^ 0x5EEDull, (uint64_t)i);
     double c[3], nv[3];
     for (double &x : c) x = (2.0 * r.u01() - 1.0) * extent;
@@ -220,6 +252,7 @@ int piko_scene_soup_world(uint64_t seed, int n, float extent, float size_median,
       }
     }
   }
+  });
   return 0;
 }
 

@@ -25,6 +25,10 @@ class ServerOptions(ctypes.Structure):
                 ("max_jobs", ctypes.c_int), ("batch_budget_bytes", ctypes.c_longlong)]
 
 
+class Completion(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_longlong), ("done_s", ctypes.c_double), ("status", ctypes.c_int)]
+
+
 class ServerStats(ctypes.Structure):
     _fields_ = [("requests", ctypes.c_longlong), ("batches", ctypes.c_longlong), ("wall_s", ctypes.c_double),
                 ("mean_batch", ctypes.c_double)]
@@ -38,23 +42,35 @@ def _bind():
                                          ctypes.POINTER(ServerOptions)]
         L.piko_server_replay.argtypes = [P, ctypes.POINTER(Request), I, P, P, ctypes.POINTER(ServerStats)]
         L.piko_server_destroy.argtypes = [P]
+        L.piko_server_submit.argtypes = [P, ctypes.POINTER(Request), P, ctypes.POINTER(LL)]
+        L.piko_server_poll.argtypes = [P, ctypes.POINTER(Completion), I, I]
         L._serve_bound = True
     return L
 
 
-def resident_scenes(nscenes=64, tris_per_scene=200_000, seed=7):
-    """A mix of the two generators in world space: even scenes are triangle
-    soups (uniform centroids in the unit cube), odd ones height fields.
-    Returns (pos, nrm, scene_base, scene_count)."""
+def scene_geometry(k, tris, seed=7):
+    """Scene k of the resident mix in world space: even k a triangle soup
+    (uniform centroids in the unit cube), odd k a height field of about
+    `tris` triangles.  Returns (pos, nrm), [n, 9] float32 each."""
+    if k % 2 == 0:
+        return scenes.soup_world(seed * 1000 + k, tris)
+    g = max(2, int(math.sqrt(tris / 2)))
+    hf = scenes.heightfield(seed=seed * 1000 + k, grid=g, W=64, H=64)
+    return hf.pos, hf.nrm
+
+
+def scene_count(k, tris):
+    return tris if k % 2 == 0 else 2 * max(2, int(math.sqrt(tris / 2))) ** 2
+
+
+def resident_scenes(nscenes=64, tris_per_scene=200_000, seed=7, sizes=None):
+    """The mix of the two generators on the host (scene k has sizes[k]
+    triangles, or tris_per_scene).  Returns (pos, nrm, scene_base,
+    scene_count)."""
     pos_l, nrm_l, base, count = [], [], [], []
     off = 0
     for k in range(nscenes):
-        if k % 2 == 0:
-            p, n = scenes.soup_world(seed * 1000 + k, tris_per_scene)
-        else:
-            g = max(2, int(math.sqrt(tris_per_scene / 2)))
-            hf = scenes.heightfield(seed=seed * 1000 + k, grid=g, W=64, H=64)
-            p, n = hf.pos, hf.nrm
+        p, n = scene_geometry(k, sizes[k] if sizes else tris_per_scene, seed)
         pos_l.append(p)
         nrm_l.append(n)
         base.append(off)
@@ -62,6 +78,54 @@ def resident_scenes(nscenes=64, tris_per_scene=200_000, seed=7):
         off += p.shape[0]
     return (np.concatenate(pos_l), np.concatenate(nrm_l), np.asarray(base, np.int64),
             np.asarray(count, np.int32))
+
+
+class DeviceScenes:
+    """The same mix built scene by scene straight into device memory, so the
+    host never holds more than one scene (the generators are counter-based:
+    host_scene(k) regenerates scene k for checking)."""
+
+    def __init__(self, nscenes=64, tris_per_scene=200_000, seed=7, sizes=None, log=None):
+        L = lib()
+        P, SZ = ctypes.c_void_p, ctypes.c_size_t
+        L.piko_rt_malloc.argtypes = [ctypes.POINTER(P), SZ]
+        L.piko_rt_memcpy_htod.argtypes = [P, P, SZ]
+        L.piko_rt_free.argtypes = [P]
+        self.seed = seed
+        self.sizes = [sizes[k] if sizes else tris_per_scene for k in range(nscenes)]
+        self.count = np.asarray([scene_count(k, t) for k, t in enumerate(self.sizes)], np.int32)
+        self.base = np.concatenate([[0], np.cumsum(self.count.astype(np.int64))[:-1]]).astype(np.int64)
+        self.tris = int(self.count.astype(np.int64).sum())
+        self.nbytes = 72 * self.tris
+        self._pos, self._nrm = P(), P()
+        for ptr in (self._pos, self._nrm):
+            _check(L.piko_rt_malloc(ctypes.byref(ptr), max(36 * self.tris, 4)), "rt_malloc")
+        for k in range(nscenes):
+            p, n = scene_geometry(k, self.sizes[k], seed)
+            assert p.shape[0] == self.count[k]
+            off = 36 * int(self.base[k])
+            _check(L.piko_rt_memcpy_htod(P(self._pos.value + off), _p(p), p.nbytes), "rt_memcpy_htod")
+            _check(L.piko_rt_memcpy_htod(P(self._nrm.value + off), _p(n), n.nbytes), "rt_memcpy_htod")
+            if log and (k % 16 == 15 or k == nscenes - 1):
+                log("resident scenes: %d of %d uploaded (%.1f GB)" % (
+                    k + 1, nscenes, 72 * float(self.base[k] + self.count[k]) / 1e9))
+
+    @property
+    def pos_ptr(self):
+        return self._pos.value
+
+    @property
+    def nrm_ptr(self):
+        return self._nrm.value
+
+    def host_scene(self, k):
+        return scene_geometry(k, self.sizes[k], self.seed)
+
+    def free(self):
+        for ptr in (self._pos, self._nrm):
+            if ptr:
+                lib().piko_rt_free(ptr)
+        self._pos, self._nrm = ctypes.c_void_p(), ctypes.c_void_p()
 
 
 def poisson_trace(n, rate_hz, nscenes, seed=11, bin_size=8, resolutions=RESOLUTIONS):
@@ -95,19 +159,29 @@ def poisson_trace(n, rate_hz, nscenes, seed=11, bin_size=8, resolutions=RESOLUTI
 
 
 class Server:
-    def __init__(self, pos, nrm, scene_base, scene_count, pipes=4, max_jobs=64, budget_bytes=8 << 30,
-                 subpixel_bits=8):
+    """pos/nrm: host arrays of the scenes (uploaded), or a DeviceScenes as
+    `pos` (adopted; nrm, scene_base and scene_count are then taken from it)."""
+
+    def __init__(self, pos, nrm=None, scene_base=None, scene_count=None, pipes=4, max_jobs=64,
+                 budget_bytes=8 << 30, subpixel_bits=8):
         L = _bind()
-        self.pos = np.ascontiguousarray(pos, np.float32)
-        self.nrm = np.ascontiguousarray(nrm, np.float32)
-        self.base = np.ascontiguousarray(scene_base, np.int64)
-        self.count = np.ascontiguousarray(scene_count, np.int32)
         opt = ServerOptions(1, subpixel_bits, pipes, max_jobs, budget_bytes)
         h = ctypes.c_void_p()
-        _check(L.piko_server_create(ctypes.byref(h), _p(self.pos), _p(self.nrm), int(self.pos.shape[0]), 0,
-                                    _p(self.base), _p(self.count), len(self.base), ctypes.byref(opt)),
-               "server_create")
+        if isinstance(pos, DeviceScenes):
+            self.base = np.ascontiguousarray(pos.base, np.int64)
+            self.count = np.ascontiguousarray(pos.count, np.int32)
+            rc = L.piko_server_create(ctypes.byref(h), ctypes.c_void_p(pos.pos_ptr), ctypes.c_void_p(pos.nrm_ptr),
+                                      pos.tris, 1, _p(self.base), _p(self.count), len(self.base), ctypes.byref(opt))
+        else:
+            self.pos = np.ascontiguousarray(pos, np.float32)
+            self.nrm = np.ascontiguousarray(nrm, np.float32)
+            self.base = np.ascontiguousarray(scene_base, np.int64)
+            self.count = np.ascontiguousarray(scene_count, np.int32)
+            rc = L.piko_server_create(ctypes.byref(h), _p(self.pos), _p(self.nrm), int(self.pos.shape[0]), 0,
+                                      _p(self.base), _p(self.count), len(self.base), ctypes.byref(opt))
+        _check(rc, "server_create")
         self._h = h
+        self._live = {}
 
     def replay(self, reqs, keep=False):
         """Returns (done_s [n], stats dict, colours or None).  keep: True for
@@ -137,10 +211,36 @@ class Server:
                     for c, r in zip(cols, reqs)]
         return done, stats, cols
 
+    def submit(self, req, keep=True):
+        """Online submission (piko_server_submit): returns (id, framebuffer or
+        None); the framebuffer is filled when the request completes."""
+        col = np.empty(req.cs.screenSizeX * req.cs.screenSizeY, np.uint32) if keep else None
+        rid = ctypes.c_longlong()
+        one = Request()
+        ctypes.pointer(one)[0] = req
+        rc = _bind().piko_server_submit(self._h, ctypes.byref(one), _p(col) if col is not None else None,
+                                        ctypes.byref(rid))
+        if rc != 0:
+            raise PikoError(rc, "server_submit")
+        self._live[rid.value] = col  # keeps the buffer alive until its completion
+        return rid.value, (col.reshape(req.cs.screenSizeY, req.cs.screenSizeX) if col is not None else None)
+
+    def poll(self, cap=64, wait_ms=10):
+        """Completions so far (piko_server_poll): [(id, done_s, status)]."""
+        buf = (Completion * cap)()
+        n = _bind().piko_server_poll(self._h, buf, cap, int(wait_ms))
+        if n < 0:
+            raise PikoError(n, "server_poll")
+        out = [(buf[i].id, buf[i].done_s, buf[i].status) for i in range(n)]
+        for rid, _, _ in out:
+            self._live.pop(rid, None)
+        return out
+
     def destroy(self):
         if self._h is not None:
             _bind().piko_server_destroy(self._h)
             self._h = None
+            self._live = {}
 
     def __enter__(self):
         return self

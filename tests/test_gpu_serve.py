@@ -68,3 +68,31 @@ def test_pipes_fall_back_when_up_front_sizing_does_not_fit(oracle_mod):
         sc, view = meta[i]
         oc, od, _ = oracle_mod.frame(view_scene(pos, nrm, base, count, sc, view), 8, 8)
         assert np.array_equal(cols[i], oc), i
+
+
+def test_online_submit_while_batches_run():
+    # requests submitted from a client thread while earlier batches run:
+    # every frame equals the same request replayed alone
+    import threading
+    pos, nrm, base, count = serve.resident_scenes(6, 3000, seed=8)
+    reqs, _ = serve.poisson_trace(24, 50000.0, 6, seed=9, resolutions=SMALL_RES)
+    with serve.Server(pos, nrm, base, count, pipes=2, max_jobs=4, budget_bytes=64 << 20) as s:
+        got = {}
+        def client():
+            for i, r in enumerate(reqs):
+                rid, col = s.submit(r)
+                got[rid] = (i, col)
+        th = threading.Thread(target=client)
+        th.start()
+        done = []
+        while len(done) < len(reqs):
+            done += s.poll(16, 50)
+        th.join()
+        assert all(st == 0 for _, _, st in done)
+        for rid, _, _ in done:
+            i, col = got[rid]
+            one = (serve.Request * 1)()
+            one[0] = reqs[i]
+            one[0].arrival_s = 0.0
+            _, _, c1 = s.replay(one, keep=True)
+            assert np.array_equal(c1[0], col), i
