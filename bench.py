@@ -1,10 +1,12 @@
 """Throughput of the Piko raster pipeline on one MI355X.
 
 Workload: the GPU-scale raster configuration -- a 2,000,000-triangle
-height field (1000x1000 grid, 6-octave value noise, amplitude 0.2, seed 2)
-rendered at 1920x1080 with a perspective camera (60 deg), per-vertex normals
-and one directional Lambert light, 8 subpixel bits, FP32 depth, bin size
-swept over 8/16/32/64.  One step = one frame (vertex stage, binning, raster,
+height field (1000x1000 grid, 6-octave value noise from 16 lattice cells per
+unit, amplitude 0.2, seed 2) rendered at 1920x1080 with a perspective camera
+(60 deg) low over the terrain (mean depth complexity 3.94 over 99.8% of the
+screen), per-vertex normals and one directional Lambert light, 8 subpixel
+bits, FP32 depth, raster path and bin size swept (queue, tiled, direct,
+binned; 8/16/32/64).  One step = one frame (vertex stage, binning, raster,
 depth, Gouraud, framebuffer) with the geometry resident in HBM.
 
 Prints one JSON line.  `value` is Mtri/s at the fastest bin size.  The same
@@ -29,6 +31,12 @@ sys.path.insert(0, os.path.join(REPO, "piko-public_amd"))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+SERVE_BUDGET = 12 << 30  # device bytes one serving batch may use
+# fragments (covered pixel samples before the depth test) and covered pixels
+# of the 2M-triangle height field at 1080p, from the oracle's coverage
+# counts (profiles/heightfield_depth_complexity_sweep.md)
+HF_FRAGMENTS = 8159414
+HF_COVERED = 2069833
 
 
 def log(msg):
@@ -43,6 +51,24 @@ def cpu_model():
     except OSError:
         pass
     return platform.processor()
+
+
+def host_info():
+    """The CPU the baselines ran on: lscpu's model name, nproc (the CPUs this
+    process may use) and the logical CPU count."""
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True, timeout=10).stdout.strip())
+    except (OSError, ValueError, subprocess.SubprocessError):
+        nproc = len(os.sched_getaffinity(0))
+    return {"model": model or cpu_model(), "nproc": nproc, "logical_cpus": os.cpu_count()}
 
 
 def cpu_baseline(scene, T, S, budget_s=12.0):
@@ -60,9 +86,11 @@ def cpu_baseline(scene, T, S, budget_s=12.0):
             break
     times.sort()
     med = times[len(times) // 2]
+    h = host_info()
     return {"value": round(scene.n / med / 1e6, 4), "unit": "Mtri/s", "cores": 1, "kind": "port",
+            "threads": 1, "host": h,
             "sample": "%d full frames of the same 2M-triangle 1080p scene (bin %d), median %.3f s/frame, "
-                      "single-threaded oracle, host %s" % (len(times), T, med, cpu_model())}
+                      "single-threaded oracle, host %s (nproc %d)" % (len(times), T, med, h["model"], h["nproc"])}
 
 
 def cpu_reference_schedule(scene, T, S, budget_s=6.0):
@@ -88,7 +116,7 @@ def cpu_reference_schedule(scene, T, S, budget_s=6.0):
         med = times[len(times) // 2]
         runs.append({"threads": threads, "value": round(scene.n / med / 1e6, 4), "unit": "Mtri/s",
                      "frames": len(times), "median_s": round(med, 4)})
-    return {"kind": "port", "bin_size": T, "runs": runs, "host": cpu_model(),
+    return {"kind": "port", "bin_size": T, "runs": runs, "host": host_info(),
             "sample": "whole frames of the same scene; raster bins over host threads"}
 
 
@@ -130,7 +158,10 @@ def copy_calibration(mb):
 
 def bench_batch(log, reps=5):
     """256 scenes from seeds 1000-1255 (uniform soups, log-uniform sizes and
-    resolutions), autotuned per class; one batch vs sequential plain pipes."""
+    resolutions), autotuned; one batch vs sequential plain pipes.  The batch
+    runs on the tiled path (tile size per scene class, tuned in the batch) and
+    on the queue path (one persistent raster launch over every job's tile
+    parts; tile size, waves per CU and part size tuned in the batch)."""
     import numpy as np
 
     import piko_amd
@@ -141,9 +172,9 @@ def bench_batch(log, reps=5):
     ntris = sum(s.n for s in scs)
     npix = sum(s.W * s.H for s in scs)
     log("batch: %d scenes, %d triangles, %d pixels; autotuning" % (len(scs), ntris, npix))
-    # plain pipes: the fastest mode and bin size per class; the batch (one
-    # tiled launch sequence): the fastest tile size per class
-    per, table = autotune.autotune(scs, log=None, modes=(piko_amd.RASTER_DIRECT, piko_amd.RASTER_TILED))
+    # plain pipes: the fastest mode and bin size per class
+    per, table = autotune.autotune(scs, log=None, modes=(piko_amd.RASTER_DIRECT, piko_amd.RASTER_TILED,
+                                                         piko_amd.RASTER_QUEUE))
     pipes = []
     for s, cfg in zip(scs, per):
         pipes.append(piko_amd.RasterPipe().allocate(s, cfg["bin_size"], 8, waves_per_cu=cfg["waves_per_cu"],
@@ -156,58 +187,77 @@ def bench_batch(log, reps=5):
         for p in pipes:
             p.run_single()
     t_seq = (time.perf_counter() - t0) / reps
-    pos, attr, parts = piko_amd.concat_scenes(scs)
-    tiles, btable = autotune.tune_batch(scs, pos, attr, parts)
-    bp = piko_amd.BatchPipe(pos, attr, "pretransformed", max_jobs=len(scs), max_tris=ntris, max_pixels=npix,
-                            raster_mode=piko_amd.RASTER_TILED)
-    bp.submit([(n, base, T, piko_amd.ConstantState.for_scene(s)) for (n, base, s), T in zip(parts, tiles)])
-    bp.wait()
-    bp.run_frames(1)
-    t0 = time.perf_counter()
-    bp.run_frames(reps)
-    t_batch = (time.perf_counter() - t0) / reps
-    rng = np.random.default_rng(0)
-    checked = [int(k) for k in rng.choice(len(scs), 12, replace=False)]
-    exact = all(np.array_equal(bp.color(k), pipes[k].pikoScreen.getData()) for k in checked)
-    bp.destroy()
+    plain_cols = [p.pikoScreen.getData() for p in pipes]
     for p in pipes:
         p.destroy()
+    pos, attr, parts = piko_amd.concat_scenes(scs)
+
+    def run_batch(mode, tiles, **kw):
+        bp = piko_amd.BatchPipe(pos, attr, "pretransformed", max_jobs=len(scs), max_tris=ntris, max_pixels=npix,
+                                raster_mode=mode, **kw)
+        try:
+            bp.submit([(n, base, T, piko_amd.ConstantState.for_scene(s)) for (n, base, s), T in zip(parts, tiles)])
+            bp.wait()
+            bp.run_frames(1)
+            t0 = time.perf_counter()
+            bp.run_frames(reps)
+            dt = (time.perf_counter() - t0) / reps
+            kernels, _ = bp.profile_frames(3)
+            exact = sum(1 for k in range(len(scs)) if np.array_equal(bp.color(k), plain_cols[k]))
+        finally:
+            bp.destroy()
+        return dt, kernels, exact
+
+    tiles, btable = autotune.tune_batch(scs, pos, attr, parts)
+    t_batch, k_tiled, exact = run_batch(piko_amd.RASTER_TILED, tiles)
+    qcfg, _ = autotune.tune_queue_batch(scs, pos, attr, parts)
+    t_queue, k_queue, exact_q = run_batch(piko_amd.RASTER_QUEUE, [qcfg["bin_size"]] * len(scs),
+                                          part_pairs=qcfg["part_pairs"], waves_per_cu=qcfg["waves_per_cu"])
     out = {"workload": "256 uniform-soup scenes, seeds 1000-1255, N log-uniform [1K,500K], "
-                       "pixels log-uniform [256^2,2048^2], aspect {1,4:3,16:9}, autotuned per class (plain pipes: raster path and bin size; batch: tile size timed in the batch)",
+                       "pixels log-uniform [256^2,2048^2], aspect {1,4:3,16:9}, autotuned (plain pipes: raster "
+                       "path and bin size per class; batch: tile size per class, timed in the batch)",
            "scenes": len(scs), "triangles": ntris, "pixels": npix, "classes": len(table),
            "seq_ms_per_batch": round(t_seq * 1e3, 3), "batch_ms": round(t_batch * 1e3, 3),
            "seq_frames_per_s": round(len(scs) / t_seq, 1), "batch_frames_per_s": round(len(scs) / t_batch, 1),
            "speedup": round(t_seq / t_batch, 2), "batch_Mtri_per_s": round(ntris / t_batch / 1e6, 1),
-           "plain_modes": {"direct": sum(1 for c in per if c["raster_mode"] == piko_amd.RASTER_DIRECT),
-                           "tiled": sum(1 for c in per if c["raster_mode"] == piko_amd.RASTER_TILED)},
+           "plain_modes": {m: sum(1 for c in per if c["raster_mode"] == v)
+                           for m, v in (("direct", piko_amd.RASTER_DIRECT), ("tiled", piko_amd.RASTER_TILED),
+                                        ("queue", piko_amd.RASTER_QUEUE))},
            "batch_tile_sizes": {str(T): tiles.count(T) for T in sorted(set(tiles))},
-           "bit_exact_vs_sequential": exact, "scenes_compared": len(checked)}
+           "batch_kernels_ms": [[n, round(m, 4)] for n, m in k_tiled],
+           "bit_exact_vs_sequential": exact == len(scs), "scenes_compared": len(scs), "scenes_equal": exact,
+           "queue": {"form": "queue path: bin launch + one persistent raster launch over every job's tile parts "
+                             "(+ merge of split tiles)",
+                     "batch_ms": round(t_queue * 1e3, 3), "speedup": round(t_seq / t_queue, 2),
+                     "frames_per_s": round(len(scs) / t_queue, 1), "tuned": qcfg,
+                     "kernels_ms": [[n, round(m, 4)] for n, m in k_queue],
+                     "bit_exact_vs_sequential": exact_q == len(scs), "scenes_equal": exact_q}}
     log("batch: %s" % out)
     return out
 
 
-def bench_serve(log, tris_per_scene, nreq, rate_hz=2000.0):
-    """64 resident world-space scenes (soups and height fields), Poisson
-    requests with random cameras and 540p/720p/1080p screens."""
+def bench_serve(log, ds, nreq, rate_hz=2000.0):
+    """64 resident world-space scenes (soups and height fields, `ds`, built
+    in device memory), Poisson requests with random cameras and
+    540p/720p/1080p screens, served with continuous batching."""
     from piko_amd import serve
-    pos, nrm, base, count = serve.resident_scenes(64, tris_per_scene)
-    gb = (pos.nbytes + nrm.nbytes) / 1e9
+    gb = ds.nbytes / 1e9
     log("serve: 64 scenes, %.2f GB resident, %d requests at %.0f/s" % (gb, nreq, rate_hz))
     reqs, _ = serve.poisson_trace(nreq, rate_hz, 64, seed=11, bin_size=8)
     warm, _ = serve.poisson_trace(64, 1e6, 64, seed=12, bin_size=8)
-    with serve.Server(pos, nrm, base, count, pipes=4, max_jobs=64, budget_bytes=4 << 30) as srv:
+    with serve.Server(ds, pipes=4, max_jobs=48, budget_bytes=SERVE_BUDGET) as srv:
         srv.replay(warm)
         _, st, _ = srv.replay(reqs)
-    out = {"workload": "64 resident world-space scenes (triangle soups + height fields), Poisson %.0f req/s, "
-                       "camera radius U[2,5], FOV U[40,90], {540p,720p,1080p}" % rate_hz,
-           "resident_gb": round(gb, 2), "triangles_per_scene": int(count.mean()), "requests": nreq,
+    out = {"workload": "64 resident world-space scenes (triangle soups + height fields, %.1f GB), Poisson "
+                       "%.0f req/s, camera radius U[2,5], FOV U[40,90], {540p,720p,1080p}" % (gb, rate_hz),
+           "resident_gb": round(gb, 2), "triangles_per_scene": int(ds.count.mean()), "requests": nreq,
            "frames_per_s": round(st["frames_per_s"], 1), "p50_ms": round(st["p50_ms"], 3),
            "p99_ms": round(st["p99_ms"], 3), "batches": st["batches"], "mean_batch": round(st["mean_batch"], 2)}
     log("serve: %s" % out)
     return out
 
 
-def bench_serve_saturated(log, tris_per_scene, nreq=512):
+def bench_serve_saturated(log, ds, nreq=512):
     """The served form at saturation (every request has arrived before the
     first batch ends) against sequential plain launches of the same requests:
     one plain pipe per (scene, resolution) over the same resident geometry,
@@ -216,15 +266,9 @@ def bench_serve_saturated(log, tris_per_scene, nreq=512):
 
     import piko_amd
     from piko_amd import serve
-    pos, nrm, base, count = serve.resident_scenes(64, tris_per_scene)
     reqs, _ = serve.poisson_trace(nreq, 1e7, 64, seed=13, bin_size=8)
-    rng = np.random.default_rng(1)
-    sample = sorted(int(k) for k in rng.choice(nreq, 16, replace=False))
-    log("serve (saturated): %d requests at once over 64 resident scenes" % nreq)
-    # 48 requests per batch on 4 pipes: the fastest shape of a steady-state
-    # sweep over 4..64 requests per batch and 2 or 4 pipes on these scenes
-    # (8: 24.7K frames/s, 16: 28.0K, 32: 29.1K, 48: 33.4K, 64: 32.2K)
-    with serve.Server(pos, nrm, base, count, pipes=4, max_jobs=48, budget_bytes=4 << 30) as srv:
+    log("serve (saturated): %d requests at once over 64 resident scenes (%.1f GB)" % (nreq, ds.nbytes / 1e9))
+    with serve.Server(ds, pipes=4, max_jobs=48, budget_bytes=SERVE_BUDGET) as srv:
         # steady state: one untimed replay of the trace sizes every pipe's
         # buffers for it; then the median of three timed replays
         srv.replay(reqs)
@@ -232,15 +276,14 @@ def bench_serve_saturated(log, tris_per_scene, nreq=512):
         st = runs[1]
         # the frames to compare come from a second, untimed replay of the same
         # trace: reading framebuffers back inside the timed loop would stall it
-        _, _, cols = srv.replay(reqs, keep=sample)
-    dpos, dnrm = piko_amd.DeviceArray(pos), piko_amd.DeviceArray(nrm)
+        _, _, cols = srv.replay(reqs, keep=True)
     pipes = {}
     try:
         def pipe_of(r):
             key = (r.scene, r.cs.screenSizeX, r.cs.screenSizeY)
             if key not in pipes:
-                b, n = int(base[r.scene]), int(count[r.scene])
-                pipes[key] = piko_amd.RasterPipe().allocate_device(dpos.ptr + 36 * b, dnrm.ptr + 36 * b, n, r.cs,
+                b, n = int(ds.base[r.scene]), int(ds.count[r.scene])
+                pipes[key] = piko_amd.RasterPipe().allocate_device(ds.pos_ptr + 36 * b, ds.nrm_ptr + 36 * b, n, r.cs,
                                                                    bin_size=8, raster_mode=piko_amd.RASTER_DIRECT)
             return pipes[key]
         for r in reqs:
@@ -256,25 +299,24 @@ def bench_serve_saturated(log, tris_per_scene, nreq=512):
                 p.run_single()
             passes.append(time.perf_counter() - t0)
         t_seq = sorted(passes)[1]
-        exact = True
-        for k in sample:
-            p = pipe_of(reqs[k])
-            p.prepare(cs=reqs[k].cs)
+        exact = 0
+        for k, r in enumerate(reqs):  # every request against its plain launch
+            p = pipe_of(r)
+            p.prepare(cs=r.cs)
             p.run_single()
-            exact = exact and bool(np.array_equal(p.pikoScreen.getData(), cols[k]))
+            exact += int(np.array_equal(p.pikoScreen.getData(), cols[k]))
     finally:
         for p in pipes.values():
             p.destroy()
-        dpos.free()
-        dnrm.free()
     seq_fps = nreq / t_seq
-    out = {"workload": "%d requests arriving at once (Poisson at 1e7 req/s) over 64 resident world-space scenes, "
-                       "camera radius U[2,5], FOV U[40,90], {540p,720p,1080p}" % nreq,
+    out = {"workload": "%d requests arriving at once (Poisson at 1e7 req/s) over 64 resident world-space scenes "
+                       "(%.1f GB), camera radius U[2,5], FOV U[40,90], {540p,720p,1080p}" % (nreq, ds.nbytes / 1e9),
+           "resident_gb": round(ds.nbytes / 1e9, 2),
            "requests": nreq, "served_frames_per_s": round(st["frames_per_s"], 1),
            "sequential_plain_frames_per_s": round(seq_fps, 1),
            "speedup": round(st["frames_per_s"] / seq_fps, 2), "p50_ms": round(st["p50_ms"], 3),
            "p99_ms": round(st["p99_ms"], 3), "batches": st["batches"], "mean_batch": round(st["mean_batch"], 2),
-           "bit_exact_vs_plain": exact, "requests_compared": len(sample)}
+           "bit_exact_vs_plain": exact == nreq, "requests_compared": nreq, "requests_equal": exact}
     log("serve (saturated): %s" % out)
     return out
 
@@ -332,13 +374,14 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--bins", default="8,16,32,64", help="bin sizes of the binned raster mode")
-    ap.add_argument("--modes", default="tiled,direct,binned", help="raster modes to sweep")
+    ap.add_argument("--modes", default="queue,tiled,direct,binned", help="raster modes to sweep")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="all", choices=["raster", "batch", "serve", "reyes", "all"])
     ap.add_argument("--reyes-sampling", default="auto", choices=["auto", "global", "tiled"],
                     help="how the Reyes frame composites samples (auto = global)")
     ap.add_argument("--reyes-tile", type=int, default=0, help="tiled sampling's tile edge: 16 or 32 (0 = 16)")
-    ap.add_argument("--serve-tris", type=int, default=300_000, help="triangles per resident scene when serving")
+    ap.add_argument("--serve-tris", type=int, default=8_700_000,
+                    help="triangles per resident scene when serving (8.7M: ~40 GB over 64 scenes)")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (HBM counter calibration)")
@@ -368,7 +411,10 @@ def main():
         runs += [("binned", int(t)) for t in args.bins.split(",")]
     if "direct" in modes:
         runs.append(("direct", 16))  # the bin size only shapes the on-demand queues
-    rmodes = {"binned": piko_amd.RASTER_BINNED, "direct": piko_amd.RASTER_DIRECT, "tiled": piko_amd.RASTER_TILED}
+    if "queue" in modes:  # tiles of 32 and 64 (a 1080p screen at 8 or 16 is binned in 32-pixel tiles)
+        runs += [("queue", int(t)) for t in args.bins.split(",") if int(t) >= 32]
+    rmodes = {"binned": piko_amd.RASTER_BINNED, "direct": piko_amd.RASTER_DIRECT, "tiled": piko_amd.RASTER_TILED,
+              "queue": piko_amd.RASTER_QUEUE}
     for mode, T in runs:
         rm = rmodes[mode]
         pipe = piko_amd.RasterPipe().allocate(scene, T, S, raster_mode=rm)
@@ -391,18 +437,18 @@ def main():
     # dominant kernel: the longest single launch of the frame (HIP events on
     # the pipe's stream); the raster kernel at every bin size measured so far
     dom, dom_ms = max(kernels, key=lambda k: k[1])
-    dom_key = "%s<%d>" % (dom, T) if dom in ("k_raster", "k_tile_raster") else dom
+    dom_key = "%s<%d>" % (dom, T) if dom in ("k_raster", "k_tile_raster", "k_qraster") else dom
     if dom == "k_raster":
         # framebuffer written once (colour + depth), each emitted triangle's
         # setup (3 fixed-point xy pairs, 3 depths, 3 packed colours) read once,
         # each (triangle, bin) queue entry read once
         units = {"pixels": scene.W * scene.H, "emitted_triangles": st["emitted"], "queue_entries": st["pairs"]}
         alg_k = 8 * units["pixels"] + 48 * units["emitted_triangles"] + 4 * units["queue_entries"]
-    elif dom in ("k_tile_bin", "k_vs_direct"):
+    elif dom in ("k_tile_bin", "k_vs_direct", "k_qbin"):
         # the input's vertex positions (3 x xyz f32) read once per triangle
         units = {"input_triangles": scene.n}
         alg_k = 36 * scene.n
-    elif dom in ("k_tile_raster", "k_resolve"):
+    elif dom in ("k_tile_raster", "k_resolve", "k_qraster"):
         # the input's vertex attributes (normals 36 B, or packed colours 12 B)
         # read once per triangle, the framebuffer (colour + depth) written once
         b_attr = 36 if scene.kind == "world" else 12
@@ -433,6 +479,12 @@ def main():
                    "triangles": scene.n, "W": scene.W, "H": scene.H, "subpixel_bits": S, "raster_mode": mode,
                    "bin_size": None if mode == "direct" else T,
                    "emitted": st["emitted"], "pairs": st["pairs"],
+                   "base_freq": scenes.HF_FREQ, "eye": list(scenes.HF_EYE), "target": list(scenes.HF_TARGET),
+                   # the scene's fragment statistics (oracle coverage counts, recorded
+                   # with the pose sweep in profiles/heightfield_depth_complexity_sweep.md)
+                   "fragments": HF_FRAGMENTS, "covered_pixels": HF_COVERED,
+                   "depth_complexity": round(HF_FRAGMENTS / HF_COVERED, 2),
+                   "coverage": round(HF_COVERED / (scene.W * scene.H), 4),
                    "sweep": sweep},
         "roofline": {"bound": "hbm", "achieved": round(achieved_k, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved_k / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -451,8 +503,14 @@ def main():
     if args.workload in ("batch", "all"):
         out["batch"] = bench_batch(log)
     if args.workload in ("serve", "all"):
-        out["serve"] = bench_serve(log, args.serve_tris, args.serve_requests)
-        out["serve_saturated"] = bench_serve_saturated(log, args.serve_tris)
+        from piko_amd import serve
+        log("building the resident scene set (64 scenes x %d triangles) in device memory" % args.serve_tris)
+        ds = serve.DeviceScenes(64, args.serve_tris, log=log)
+        try:
+            out["serve"] = bench_serve(log, ds, args.serve_requests)
+            out["serve_saturated"] = bench_serve_saturated(log, ds)
+        finally:
+            ds.free()
     if args.workload in ("reyes", "all"):
         out["reyes"] = bench_reyes(log, args.steps, args.warmup, cpu=not args.no_cpu, sampling=args.reyes_sampling,
                                    tile=args.reyes_tile)

@@ -3,7 +3,9 @@
 A scene class is (log2 of the triangle count, log2 of the pixel-count edge).
 For one representative scene per class, every candidate is timed on the GPU
 with the plain pipe; the fastest wins.  Candidates are the bin (tile) size T
-and, for the binned raster, the persistent raster waves per CU.  The queue
+and, for the binned raster, the persistent raster waves per CU (the queue
+path's persistent grid and part size are tuned for batches by
+tune_queue_batch).  The queue
 depth recorded for the class is the measured (triangle, bin) pairs per
 triangle at the chosen T, with 25% headroom, so binned batches can size their
 queues up front instead of growing them after an overflow.
@@ -14,7 +16,7 @@ import time
 
 import numpy as np
 
-from . import RASTER_AUTO, RASTER_BINNED, RasterPipe
+from . import RASTER_AUTO, RASTER_BINNED, RASTER_QUEUE, RasterPipe
 
 BIN_SIZES = (8, 16, 32, 64)
 WAVES_PER_CU = (8, 16, 24)
@@ -26,6 +28,8 @@ def candidates(raster_mode):
     shape the binned raster's persistent grid."""
     if raster_mode == RASTER_BINNED:
         return BIN_SIZES, WAVES_PER_CU
+    if raster_mode == RASTER_QUEUE:
+        return (32, 64), (16,)
     return TILE_SIZES, (0,)
 
 
@@ -124,10 +128,45 @@ def tune_batch(scene_list, pos, attr, parts, S=8, tile_sizes=TILE_SIZES, frames=
     return per_scene, table
 
 
+def tune_queue_batch(scene_list, pos, attr, parts, S=8, tile_sizes=(32, 64), waves=(16, 24),
+                     part_pairs=(1024, 2048), frames=3, log=None):
+    """The queue path's batch settings, timed in the batch itself: one tile
+    size for every job (the raster is one persistent launch over all of the
+    batch's tile parts), the persistent grid's waves per CU and the queue
+    entries per part item.  Returns ({"bin_size", "waves_per_cu",
+    "part_pairs", "ms"}, trials)."""
+    from . import BatchPipe, ConstantState, RASTER_QUEUE
+    kind = scene_list[0].kind if scene_list else "pretransformed"
+    ntris = sum(sc.n for sc in scene_list)
+    npix = sum(sc.W * sc.H for sc in scene_list)
+    best, trials = None, []
+    for w in waves:
+        for pp in part_pairs:
+            bp = BatchPipe(pos, attr, kind, subpixel_bits=S, max_jobs=max(len(scene_list), 1), max_tris=ntris,
+                           max_pixels=npix, raster_mode=RASTER_QUEUE, part_pairs=pp, waves_per_cu=w)
+            try:
+                for T in tile_sizes:
+                    bp.submit([(n, base, T, ConstantState.for_scene(sc)) for (n, base, sc) in parts])
+                    bp.wait()
+                    bp.run_frames(1)
+                    t0 = time.perf_counter()
+                    bp.run_frames(frames)
+                    dt = (time.perf_counter() - t0) / frames
+                    trials.append((T, w, pp, dt * 1e3))
+                    if log:
+                        log("queue batch T=%d waves/CU=%d part pairs=%d: %.3f ms" % (T, w, pp, dt * 1e3))
+                    if best is None or dt < best[3]:
+                        best = (T, w, pp, dt)
+            finally:
+                bp.destroy()
+    T, w, pp, dt = best
+    return {"bin_size": T, "waves_per_cu": w, "part_pairs": pp, "ms": dt * 1e3}, trials
+
+
 def save(table, path):
     with open(path, "w") as f:
         json.dump({"%d,%d" % k: {kk: (vv if kk != "best_bin_size" else {str(m): t for m, t in vv.items()})
                                  for kk, vv in v.items() if kk != "trials"} for k, v in table.items()}, f, indent=1)
 
 
-__all__ = ["scene_class", "candidates", "tune", "autotune", "tune_batch", "save", "np"]
+__all__ = ["scene_class", "candidates", "tune", "autotune", "tune_batch", "tune_queue_batch", "save", "np"]
