@@ -48,12 +48,22 @@ def _bind():
     return L
 
 
+def soup_size(tris):
+    """Median circumradius of a soup scene's triangles: scaled as 1/sqrt(N) so
+    that the projected depth complexity stays near config 1's (50,000
+    triangles of median 20 px^2 on 512^2, about 6.3) at any triangle count --
+    6.3 to 8.3 for 8.7M triangles at 1080p from 2 to 5 units (oracle coverage
+    counts)."""
+    return 0.007 * math.sqrt(300_000 / max(tris, 1))
+
+
 def scene_geometry(k, tris, seed=7):
     """Scene k of the resident mix in world space: even k a triangle soup
-    (uniform centroids in the unit cube), odd k a height field of about
-    `tris` triangles.  Returns (pos, nrm), [n, 9] float32 each."""
+    (uniform centroids in [-1, 1]^3, triangle size from soup_size), odd k a
+    height field of about `tris` triangles.  Returns (pos, nrm), [n, 9]
+    float32 each."""
     if k % 2 == 0:
-        return scenes.soup_world(seed * 1000 + k, tris)
+        return scenes.soup_world(seed * 1000 + k, tris, 1.0, soup_size(tris))
     g = max(2, int(math.sqrt(tris / 2)))
     hf = scenes.heightfield(seed=seed * 1000 + k, grid=g, W=64, H=64)
     return hf.pos, hf.nrm
