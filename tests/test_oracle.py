@@ -289,3 +289,41 @@ def test_config1_scene_statistics(oracle_mod):
     assert z.min() >= 0.1 and z.max() < 1.0
     st = oracle_mod.vertex_stage(sc, 8)
     assert st.keep.sum() > 49000
+
+
+def test_world_vertex_stage_perspective_divide(oracle_mod):
+    """vtxTransform with w != 1 (rasterMacros.h:14-21): a perspective matrix
+    with clip w = -z_eye and every quantity dyadic, so 1/w, x/w and the
+    viewport maps are exact and the fixed-point coordinates have a closed
+    form: x_fx = ((x / w + 1) * W / 2) * 2^S."""
+    from fractions import Fraction as F
+    W, H, S = 64, 32, 8
+    # column-major: x' = x, y' = y, z' = z / 2 + 1, w' = -z
+    m = np.zeros(16, np.float32)
+    m[0], m[5], m[10], m[14], m[11] = 1.0, 1.0, 0.5, 1.0, -1.0
+    verts = [(-0.5, -0.5, -2.0), (1.0, -1.0, -4.0), (0.25, 0.75, -1.0)]  # w = 2, 4, 1
+    pos = np.array([sum(verts, ())], np.float32)
+    nrm = np.array([[0.0, 0.0, 1.0] * 3], np.float32)
+    sc = scenes.WorldScene(pos, nrm, W, H, m, np.array([0, 0, 1], np.float32), np.array([0.5, 0.5, 0.5], np.float32),
+                           np.array([0.1, 0.1, 0.1], np.float32))
+    st = oracle_mod.vertex_stage(sc, S)
+    want_xy, want_z = [], []
+    for x, y, z in verts:
+        w = F(-z)
+        want_xy += [int(((F(x) / w + 1) * F(W, 2)) * 2 ** S), int(((F(y) / w + 1) * F(H, 2)) * 2 ** S)]
+        want_z.append(float((F(z) / 2 + 1) / w))
+    assert list(st.xy[0]) == want_xy
+    assert list(st.z[0]) == [np.float32(v) for v in want_z]
+    # the triangle keeps its winding through the divide: counter-clockwise, kept
+    assert st.keep[0] == 1
+
+
+def test_threaded_schedule_under_thread_sanitizer(oracle_mod):
+    """The threaded schedule (bins over 8 host threads, CPUBackend.cpp:522-571)
+    built with ThreadSanitizer: no race is reported and the frame equals the
+    single-threaded one at every bin size (oracle/tsan_main.c)."""
+    import subprocess
+    r = subprocess.run(["make", "-s", "-C", oracle_mod.HERE, "tsan"], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "WARNING: ThreadSanitizer" not in r.stdout + r.stderr
+    assert r.stdout.count("equal the single-threaded frame") == 4

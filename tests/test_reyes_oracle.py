@@ -242,3 +242,29 @@ def test_sphere_frame_is_finite_and_closed(oracle_mod):
     assert nup == 64 * leaves.shape[0]
     # the silhouette is a disc: the centre pixel and its neighbourhood are covered
     assert covered(z[80:100, 150:170]).all()
+
+
+@pytest.mark.parametrize("disp", [0.5, -0.25])
+def test_displacement_closed_form_on_planar_patch(oracle_mod, disp):
+    # a linear patch in the z = 0 plane (u along +x, v along +y: unit normal
+    # exactly (0, 0, 1)); displacement d = disp * (x^2 - y^2) along the normal
+    # (dice.pikostage:206-229 with the displacement shader) moves each corner
+    # to z = d, and x, y stay.  The identity camera gives screen z = z.  All
+    # coordinates are dyadic with few bits, so every product and sum is exact
+    # and the micropolygon's flat depth (((z0+z1)+z2)+z3)/4 has a closed form.
+    from fractions import Fraction as F
+    x0, y0, hx, hy = -0.5, -0.25, 0.125, 0.0625
+    cp = linear_patch(x0, y0, hx, hy, z=0.0)
+    up = oracle_mod.reyes_dice(cp, np.array([2], np.int32), np.array([2], np.int32), 8, 2, 2, IDENTITY, disp)
+    assert up.shape[0] == 64
+    for k in range(8):
+        for j in range(8):
+            q = up[k * 8 + j]
+            zs = []
+            for cidx, (du, dv) in enumerate(((0, 0), (1, 0), (1, 1), (0, 1))):
+                x = F(x0) + F(3 * hx).limit_denominator() * F(k + du, 8)
+                y = F(y0) + F(3 * hy).limit_denominator() * F(j + dv, 8)
+                zs.append(F(disp) * (x * x - y * y))
+                # x and y are unmoved: the normal has no x or y part
+                assert q["x"][cidx] == np.float32(float(x + 1)) and q["y"][cidx] == np.float32(float(y + 1))
+            assert q["z"] == np.float32(float(sum(zs) / 4)), (k, j)
