@@ -2,6 +2,8 @@
 
   piko_amd/libpiko_scene.so  scene generators (g++, host only)
   piko_amd/libpiko_hip.so    HIP kernels for gfx950 + the C-ABI runtime
+  piko_amd/vertex_module.hsaco  a gfx950 code object (vertex stage + clear)
+                              that the runtime shim's module calls load
 
 Every call recompiles (hipcc cross-compiles gfx950 without a GPU).  Flags:
   -ffp-contract=off           no FMA contraction: the raster path replays the
@@ -56,8 +58,18 @@ def build_hip():
     return out
 
 
+def build_module():
+    out = os.path.join(OUT, "vertex_module.hsaco")
+    tmp = out + ".tmp%d" % os.getpid()
+    flags = [f for f in HIP_FLAGS if f not in ("-fPIC", "-shared")]
+    _run(["/opt/rocm/bin/hipcc", "--genco"] + flags + ["-I", INC, "-I", CSRC,
+                                                        os.path.join(CSRC, "module", "vertex_module.hip"), "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all():
-    return [build_scene(), build_hip()]
+    return [build_scene(), build_hip(), build_module()]
 
 
 if __name__ == "__main__":

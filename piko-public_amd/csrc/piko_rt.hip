@@ -13,8 +13,13 @@
 #include "piko_hip.h"
 
 namespace {
+// A failed HIP call also sets the calling thread's last-error state, which a
+// later, unrelated launch check (hipGetLastError after a kernel launch) would
+// report as its own failure.  The CUDA driver API has no such state, and the
+// status returned here already carries the failure, so it is consumed.
 int st(hipError_t e) {
   if (e == hipSuccess) return PIKO_OK;
+  (void)hipGetLastError();
   if (e == hipErrorOutOfMemory) return PIKO_ERR_OOM;
   if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return PIKO_ERR_NO_DEVICE;
   if (e == hipErrorInvalidValue) return PIKO_ERR_INVALID;

@@ -391,6 +391,24 @@ class ReyesPipe:
         self._h = h
         return self
 
+    def allocate_device(self, scene, d_patches, count, roots_per_wave=0, keep_samples=False, max_splits=20,
+                        sampling=SAMPLE_AUTO, tile=0):
+        """As allocate, with the patches already in device memory
+        (piko_reyes_allocate_device): d_patches is the device address of
+        count patches of 16 control points x 4 floats; `scene` supplies the
+        camera, screen and Reyes settings only."""
+        if self._h is not None:
+            raise RuntimeError("pipe already allocated")
+        self.W, self.H, self.spp = scene.W, scene.H, scene.spp
+        self.cs = ConstantState.for_scene(scene)
+        opt = ReyesOptions(float(scene.threshold), int(max_splits), int(scene.dice), int(scene.spp),
+                           float(scene.disp), int(roots_per_wave), 1 if keep_samples else 0, int(sampling), int(tile))
+        h = ctypes.c_void_p()
+        _check(lib().piko_reyes_allocate_device(ctypes.byref(h), ctypes.byref(self.cs), ctypes.c_void_p(d_patches),
+                                                int(count), ctypes.byref(opt)), "reyes_allocate_device")
+        self._h = h
+        return self
+
     def prepare(self, scene=None):
         if scene is not None:
             self.cs = ConstantState.for_scene(scene)

@@ -27,6 +27,17 @@ def test_library_exports_header(header, lib):
     assert not missing, missing
 
 
+def test_vertex_module_holds_pipeline_names():
+    """The code object the runtime shim loads (piko_rt_module_load) carries
+    the module global and entry names the generated host code fetches."""
+    path = os.path.join(PKG, "vertex_module.hsaco")
+    assert os.path.exists(path), "build the code object first (__graft_entry__.build())"
+    data = open(path, "rb").read()
+    assert b"amdgcn-amd-amdhsa" in data and b"gfx950" in data
+    for name in (b"constState", b"kernel0", b"kernel1"):
+        assert name in data, name
+
+
 def test_status_strings():
     L = ctypes.CDLL(os.path.join(PKG, "libpiko_hip.so"))
     L.piko_status_string.restype = ctypes.c_char_p
