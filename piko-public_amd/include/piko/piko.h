@@ -10,7 +10,9 @@
  * run_single / run / destroy methods over the C-ABI of libpiko_hip.so
  * (include/piko_hip.h).  The stage bodies are not compiled to device code:
  * pikoc_hip maps the stage graph onto the library's hand-written gfx950
- * kernels.
+ * kernels, and does so only when every stage body (read with
+ * __PIKOC_DEVICE__ defined) is one those kernels implement; any other body
+ * is refused (exit status 5).
  */
 #ifndef PIKO_PIKO_H
 #define PIKO_PIKO_H

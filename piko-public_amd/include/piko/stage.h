@@ -5,9 +5,10 @@
  * datatypes.h:114-144).
  *
  * On this backend a stage class is a description: pikoc_hip reads its bin
- * size, threads per tile, primitive types and directives from the source and
- * maps the pipe onto the kernel library.  The directive functions therefore
- * do nothing when called from host code.
+ * size, threads per tile, primitive types, directives and bodies from the
+ * source (with __PIKOC_DEVICE__ defined) and maps the pipe onto the kernel
+ * library when every stage body is one the library implements.  The
+ * directive functions therefore do nothing when called from host code.
  */
 #ifndef PIKO_STAGE_H
 #define PIKO_STAGE_H
@@ -17,6 +18,9 @@
 #include <vector>
 
 #include "piko/builtinTypes.h"
+#ifdef __PIKOC_DEVICE__
+#include "piko/device.h"
+#endif
 
 enum AssignPolicy { PREVIOUS_BINS, BOUNDING_BOX, POSITION };
 enum WaitPolicy { CUSTOM, BATCH, END_STAGE, END_BIN };
@@ -35,11 +39,27 @@ class Stage : public StageBase {
   static constexpr int binSizeX = BinSizeX, binSizeY = BinSizeY, threadsPerTile = ThreadsPerTile;
 
  protected:
-  void assignToBin(const InPrim &, int) {}
-  void assignToBin(const InPrim &, AssignPolicy) {}
   void specifySchedule(SchedulePolicy, int tileSplitSize = 0) { (void)tileSplitSize; }
   void specifyWait(WaitPolicy) {}
   void specifyMaxOutPrims(int) {}
+#ifndef __PIKOC_DEVICE__
+  void assignToBin(const InPrim &, int) {}
+  void assignToBin(const InPrim &, AssignPolicy) {}
+#else
+  /* What a stage body sees (stage.h:22-24, deviceFunctions.h:63-109): bin
+   * assignment, the frame's uniforms, the mutable state (the depth buffer),
+   * the bin being processed and the bin grid, and emit to an output port.
+   * Declared only: whatever executes the bodies defines them. */
+  void assignToBin(const InPrim &p, int bin);
+  void assignToBin(const InPrim &p, AssignPolicy policy);
+  ConstantState constState;
+  MutableState *mutableState;
+  int getBinID() const;
+  int getNumBins() const;
+  int getNumBinsX() const;
+  int getNumBinsY() const;
+  void emit(const OutPrim &p, int outPort);
+#endif
 };
 
 /* PikoScreen (stage.h:53-136): the colour target.  getData() returns the last

@@ -4,8 +4,14 @@
 
 writes outdir/__pikoDefines.h, outdir/__pikoCompiledPipe.h and
 outdir/__pikoCompiledPipe.kernels and prints the pipe summary and kernel
-plan.  Exit status: 0 done, 2 usage, 4 output not writable, 5 stage graph not
-supported by the HIP kernel library, 6 source not understood.
+plan.  Exit status: 0 done, 2 usage, 4 output not writable, 5 a stage graph,
+stage body or setting the HIP kernel library does not implement (nothing is
+written), 6 source not understood.
+
+  python -m pikoc_hip --fingerprints [-I dir]... [-D NAME[=VALUE]]... pipe.h
+
+prints the fingerprints of each stage's normalised source, the form of the
+recognised-body table in pikoc_hip/stage_library.py.
 """
 import argparse
 import sys
@@ -20,6 +26,7 @@ def main(argv=None):
     ap.add_argument("-I", dest="include", action="append", default=[])
     ap.add_argument("-D", dest="define", action="append", default=[])
     ap.add_argument("-o", dest="outdir", default=".")
+    ap.add_argument("--fingerprints", action="store_true")
     ap.add_argument("pipe")
     a = ap.parse_args(argv)
     if a.target.upper() != "HIP":
@@ -29,6 +36,17 @@ def main(argv=None):
     for d in a.define:
         k, _, v = d.partition("=")
         defines[k] = v or "1"
+    if a.fingerprints:
+        from .frontend import parse_pipe
+        from .stage_library import fingerprints
+        try:
+            ps = parse_pipe(a.pipe, a.include, defines)
+        except FrontendError as e:
+            print("pikoc_hip: %s" % e, file=sys.stderr)
+            return 6
+        for st in ps.order:
+            print("%s (%s): %s" % (st.name, st.type, fingerprints(st)))
+        return 0
     try:
         compile_pipe(a.pipe, a.outdir, a.include, defines)
     except FrontendError as e:

@@ -239,10 +239,25 @@ def _statements(body):
     return [s.strip() for s in body.split(";") if s.strip()]
 
 
+def normalise(code):
+    """Source text with layout removed: whitespace runs become one space, and
+    spaces next to punctuation go, so re-indenting or re-wrapping a body does
+    not change it, while changing, adding or removing an identifier, number
+    or operator does (comments are already gone)."""
+    code = re.sub(r"\s+", " ", code).strip()
+    return re.sub(r" ?([^\w ]) ?", r"\1", code)
+
+
 def parse_stage(name, args, body, macros):
     if len(args) != 5:
         raise FrontendError("Stage<> of %s needs 5 template arguments" % name)
     s = StageSummary(name=name, type=name)
+    # the class body and each phase's body as the backend sees them: after
+    # the preprocessor (inactive branches removed), macros not expanded
+    s.source = {"class": normalise(body)}
+    for meth in ("assignBin", "schedule", "process"):
+        mb = _method_body(body, meth)
+        s.source[meth] = None if mb is None else normalise(mb)
     s.binsize = (eval_int(args[0], macros), eval_int(args[1], macros))
     s.threads_per_tile = eval_int(args[2], macros)
     s.prim_in, s.prim_out = args[3], args[4]
@@ -310,6 +325,9 @@ def parse_struct_fields(text, name):
 
 
 def parse_pipe(path, include_dirs=(), defines=None):
+    # stage bodies sit in #ifdef __PIKOC_DEVICE__ blocks, as pikoc reads them
+    defines = dict(defines or {})
+    defines.setdefault("__PIKOC_DEVICE__", "1")
     text, macros = preprocess(path, include_dirs, defines)
     stages = {}
     for m in re.finditer(r"\bclass\s+(\w+)\s*:\s*public\s+Stage\s*<", text):

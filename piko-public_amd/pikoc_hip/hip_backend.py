@@ -4,17 +4,26 @@ emit methods.
 
 The generated host code calls the C-ABI of libpiko_hip.so
 (include/piko_hip.h); the device code is the library's hand-written gfx950
-kernels.  emitDeviceCode matches the pipe's stage graph and kernel plan
-against the pipelines those kernels implement and writes the kernel map:
-  raster  VertexShade-like stage (any assignBin) -> Raster stage with square
-          bins of 8/16/32/64 px -> PikoScreen
+kernels.  A pipe is bound to them only when both its stage graph and every
+stage body are ones the kernels implement:
+  raster  VertexShade stage -> Raster stage with square bins of 8/16/32/64 px
+          -> PikoScreen, with GORAUD (vertex colours interpolated)
   Reyes   Split stage looping on itself through out port 1, out port 0 ->
           Dice -> Shade -> PikoScreen
-Any other graph is reported as unsupported (no code is emitted for it).
+Stage bodies are recognised by the fingerprint of their normalised source
+(stage_library.py); any other graph, an edited or unimplemented body, or a
+shading variant the kernels lack raises BackendError (pikoc_hip exits with
+status 5) and nothing is emitted.  emitDeviceCode writes the kernel map: the
+library kernels the generated pipe launches, in launch order.
 """
 import os
 
+from . import stage_library
 from .summary import PipeSummary
+
+# raster paths the generated pipe may select with PIKO_RASTER_PATH
+# (piko_raster_options.raster_mode)
+RASTER_PATHS = {"BINNED": 1, "DIRECT": 2, "TILED": 3, "QUEUE": 4}
 
 
 class BackendError(Exception):
@@ -55,6 +64,15 @@ class HIPBackend:
             if not 1 <= self.S <= 8:
                 raise BackendError("PIKO_SUBPIXEL_BITS must be 1..8")
             self.pretransformed = "VTX_PRETRANSFORM" in ps.macros
+            if "GORAUD" not in ps.macros:
+                raise BackendError("%s: shading from interpolated per-pixel normals (GORAUD not defined) is "
+                                   "not implemented by the HIP kernel library" % st[1].name)
+            self._bodies([("raster.vertex", st[0], "pretransformed" if self.pretransformed else "world"),
+                          ("raster.raster", st[1], "gouraud")])
+            path = ps.macros.get("PIKO_RASTER_PATH", "DIRECT").strip()
+            if path not in RASTER_PATHS:
+                raise BackendError("PIKO_RASTER_PATH must be one of %s" % ", ".join(sorted(RASTER_PATHS)))
+            self.path = path
             self.f_pos = _field(ps.input_fields, ("cvec3f",), 3, 0)
             self.f_attr = (_field(ps.input_fields, ("uint32_t", "unsigned", "unsignedint"), 3, 0)
                            if self.pretransformed else _field(ps.input_fields, ("cvec3f",), 3, 1))
@@ -80,31 +98,74 @@ class HIPBackend:
                 self.f_cp = _field(ps.input_fields, ("cvec4f",), 16, 0)
                 if not self.f_cp:
                     raise BackendError("input %s needs cvec4f CP[16]" % ps.input_type)
+                self._bodies([("reyes.split", split, "split"), ("reyes.dice", dice, "dice"),
+                              ("reyes.shade", shade, "shade")])
                 return "reyes"
         raise BackendError("stage graph of %s is not one the HIP kernel library implements "
                            "(VertexShade -> Raster, or Split(loop) -> Dice -> Shade)" % ps.name)
 
+    def _bodies(self, checks):
+        """Every (role, stage, expected variant) must carry a recognised body."""
+        for role, stage, want in checks:
+            variant, diff = stage_library.match(role, stage)
+            if diff:
+                what = ("code outside assignBin / schedule / process" if diff == ["class"]
+                        else " and ".join("%s.%s" % (stage.name, ph) for ph in diff))
+                raise BackendError("%s of %s is not a body the HIP kernel library implements (compared with "
+                                   "its %s stage); pikoc_hip does not compile stage code" % (what, stage.name,
+                                                                                             variant))
+            if variant != want:
+                raise BackendError("%s: the %s body does not match the pipe's settings (%s expected)"
+                                   % (stage.name, variant, want))
+
     def kernel_map(self):
-        """(planned phases, library kernels) in launch order."""
+        """(phase, library kernels, launched every frame) in launch order:
+        the kernels the generated pipe runs (kernel names without template
+        arguments are what a kernel trace shows)."""
         if self.kind == "raster":
             a, b = self.ps.order
+            if self.path == "DIRECT":
+                return [
+                    ("kernel0 + %s.process + %s.assignBin + %s.process of the triangles with at most 16,384 "
+                     "bounding-box pixels: per-pixel 64-bit visibility atomics" % (a.name, b.name, b.name),
+                     ["k_vs_direct"], True),
+                    ("%s.process of the larger triangles, one workgroup each (frames that have any)" % b.name,
+                     ["k_large_raster"], False),
+                    ("depth test winner -> Gouraud colour -> PikoScreen", ["k_resolve"], True),
+                ]
+            if self.path == "TILED":
+                return [
+                    ("kernel0 + %s.process + %s.assignBin into per-chunk tile tables" % (a.name, b.name),
+                     ["k_tile_bin"], True),
+                    ("%s.process per tile in LDS -> PikoScreen (tile = max(T, 16), doubled until the grid "
+                     "fits)" % b.name, ["k_tile_raster"], True),
+                ]
+            if self.path == "QUEUE":
+                return [
+                    ("kernel0 + %s.process + %s.assignBin into per-tile ranges" % (a.name, b.name),
+                     ["k_qbin"], True),
+                    ("%s.process: persistent workgroups pull tile parts, LDS depth test -> PikoScreen"
+                     % b.name, ["k_qraster"], True),
+                    ("merge of tiles split into parts (frames that have any)", ["k_qmerge"], False),
+                ]
             return [
                 ("kernel0 + %s.process + emit -> %s.assignBin (bin walk, counts)" % (a.name, b.name),
-                 ["k_vs_count"]),
-                ("queue offsets", ["k_scan_tiles"]),
-                ("%s.assignBin emits (triangle, bin) entries" % b.name, ["k_emit_pairs"]),
-                ("bin queues in submission order (two 8-bit digit passes)",
-                 ["k_sort_count", "k_sort_rows", "k_sort_scatter"]),
-                ("queue ranges", ["k_bounds"]),
-                ("%s.process (coverage, depth, Gouraud) -> PikoScreen" % b.name, ["k_raster<%d>" % self.T]),
+                 ["k_vs_count"], True),
+                ("queue offsets", ["k_scan_tiles"], True),
+                ("%s.assignBin emits (triangle, bin) entries" % b.name, ["k_emit_pairs"], True),
+                ("bin queues in submission order (8-bit digit passes)",
+                 ["k_sort_count", "k_sort_rows", "k_sort_scatter"], True),
+                ("queue ranges", ["k_bounds"], True),
+                ("%s.process (coverage, depth, Gouraud) -> PikoScreen" % b.name, ["k_raster"], True),
             ]
         s, d, sh = self.ps.order
         return [
-            ("kernel0 + %s.process looping on port 1 until empty" % s.name, ["k_reyes_split"]),
-            ("emission order of the leaves", ["k_reyes_levels", "k_reyes_rank"]),
+            ("kernel0 + %s.process looping on port 1 until empty" % s.name, ["k_reyes_split"], True),
+            ("emission order of the leaves", ["k_reyes_levels", "k_reyes_rank"], True),
             ("%s.process + emit -> %s.assignBin + %s.process" % (d.name, sh.name, sh.name),
-             ["k_reyes_dice<%d,%d>" % (9 if 0 < self.dice <= 8 else 34, self.spp)]),
-            ("PikoScreen resolve", ["k_reyes_resolve"]),
+             ["k_reyes_dice"] if self.sampling != 2 else
+             ["k_reyes_dice_bin", "k_reyes_tile_count", "k_reyes_tile_scan", "k_reyes_tile_scatter"], True),
+            ("PikoScreen resolve", ["k_reyes_resolve"] if self.sampling != 2 else ["k_reyes_tile_sample"], True),
         ]
 
     # -- emitters ------------------------------------------------------------
@@ -174,6 +235,7 @@ class HIPBackend:
             out.write("  opt.subpixel_bits = %d;\n" % self.S)
             out.write("  opt.input_kind = %s;\n" % ("PIKO_INPUT_PRETRANSFORMED" if self.pretransformed
                                                   else "PIKO_INPUT_WORLD"))
+            out.write("  opt.raster_mode = PIKO_RASTER_%s;\n" % self.path)
             out.write("  PIKO_CHECK(piko_raster_allocate(&piko_hip_pipe_, &constState, pos.data(), attr.data(), "
                       "count, &opt));\n")
         else:
@@ -217,10 +279,10 @@ class HIPBackend:
     def emitDeviceCode(self, filename):
         """Writes the kernel map of the pipe to `filename`.kernels."""
         with open(filename + ".kernels", "w") as f:
-            f.write("# %s: planned phases -> libpiko_hip.so kernels (%s)\n" % (self.ps.name,
-                                                                             self.getTargetTriple()))
-            for phases, kernels in self.kernel_map():
-                f.write("%-70s %s\n" % (phases, " ".join(kernels)))
+            f.write("# %s: phases -> libpiko_hip.so kernels launched (%s); '?' = only in frames "
+                    "that need it\n" % (self.ps.name, self.getTargetTriple()))
+            for phase, kernels, always in self.kernel_map():
+                f.write("%-70s %s%s\n" % (phase, " ".join(kernels), "" if always else " ?"))
         return True
 
 

@@ -28,6 +28,9 @@ class StageSummary:
         self.tile_split_size = 0
         self.max_out_prims = 0
         self.process_policy = "Empty"
+        # normalised source of the class body and of its assignBin /
+        # schedule / process bodies (None where a phase is absent)
+        self.source = {}
         # EndStage wait: the stage whose end this stage waits for (pikoc's
         # schedule endStageName / endStagePtr, PikoSummary.hpp:198-199, set
         # from the name in PipeSummary's link pass, PikoSummary.cpp:609-611)

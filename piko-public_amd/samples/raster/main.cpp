@@ -109,9 +109,16 @@ int main(int argc, char **argv) {
   }
   const double ms_per = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() /
                         (runs > 0 ? runs : 1);
+  // the library kernels one frame of the generated pipe launches
+  float kms[64], frame_ms = 0.0f;
+  const char *knames[64];
+  int nk = 0;
+  PIKO_CHECK(piko_raster_profile_frames(pipe.piko_hip_pipe_, 1, kms, knames, 64, &nk, &frame_ms));
+  std::string kernels;
+  for (int k = 0; k < nk; ++k) kernels += std::string(k ? ", \"" : "\"") + knames[k] + "\"";
   pipe.destroy();
   printf("{\"scene\": \"%s\", \"W\": %d, \"H\": %d, \"triangles\": %d, \"color_fnv1a\": \"%016llx\", "
-         "\"depth_fnv1a\": \"%016llx\", \"ms_per_frame\": %.4f}\n",
-         scene.c_str(), W, H, n, (unsigned long long)hc, (unsigned long long)hd, ms_per);
+         "\"depth_fnv1a\": \"%016llx\", \"ms_per_frame\": %.4f, \"kernels\": [%s]}\n",
+         scene.c_str(), W, H, n, (unsigned long long)hc, (unsigned long long)hd, ms_per, kernels.c_str());
   return 0;
 }

@@ -1,5 +1,11 @@
-// Raster stage of the raster sample: trivial-reject bin walk in
-// assignBin, coverage, depth test and Gouraud colour in process.
+// Raster stage of the raster sample (the role of rasterPipelineFixPt/
+// raster.pikostage with GORAUD): assignBin walks the bins of the triangle's
+// pixel-rounded bounding box and keeps those no edge rejects at the bin's
+// most inside corner; process tests every pixel centre of the bin inside
+// that box against the three edge functions (a centre on an edge counts),
+// interpolates depth and the vertex colours barycentrically, passes the
+// fragment when its depth is not above the stored one (a later triangle
+// wins a tie) and emits the pixel.
 #ifndef SAMPLE_RASTER_H
 #define SAMPLE_RASTER_H
 
@@ -10,10 +16,82 @@
 #define RASTER_THREADCOUNT 64
 
 class RasterStage : public Stage<RASTER_BINSIZE, RASTER_BINSIZE, RASTER_THREADCOUNT, raster_stri, Pixel> {
+#ifdef __PIKOC_DEVICE__
  public:
-  void assignBin(raster_stri p) { this->assignToBin(p, 0); }
+  void assignBin(raster_stri p) {
+    const int shift = PIKO_SUBPIXEL_BITS + RASTER_BINSIZE_LG2, size = 1 << shift;
+    int lo[2], hi[2];
+    pixelBox(p, lo, hi);
+    const int bx0 = piko::imax(lo[0] >> shift, 0), bx1 = piko::imin(hi[0] >> shift, getNumBinsX() - 1);
+    const int by0 = piko::imax(lo[1] >> shift, 0), by1 = piko::imin(hi[1] >> shift, getNumBinsY() - 1);
+    for (int by = by0; by <= by1; ++by)
+      for (int bx = bx0; bx <= bx1; ++bx) {
+        bool keep = true;
+        for (int e = 0; e < 3; ++e) {
+          const long long dx = edgeDx(p, e), dy = edgeDy(p, e);
+          // the corner of the bin where this edge function is largest
+          const long long cx = ((long long)bx << shift) + (dy < 0 ? size : 0);
+          const long long cy = ((long long)by << shift) + (dx >= 0 ? size : 0);
+          keep = keep && edgeAt(p, e, cx, cy) >= 0;
+        }
+        if (keep) this->assignToBin(p, by * getNumBinsX() + bx);
+      }
+  }
+
   void schedule(int binID) { specifySchedule(LOAD_BALANCE); }
-  void process(raster_stri p) {}
+
+  void process(raster_stri p) {
+    const int S = PIKO_SUBPIXEL_BITS;
+    const int W = constState.screenSizeX, H = constState.screenSizeY;
+    const int bx = getBinID() % getNumBinsX(), by = getBinID() / getNumBinsX();
+    int lo[2], hi[2];
+    pixelBox(p, lo, hi);
+    const int x0 = piko::imax(lo[0] >> S, bx * RASTER_BINSIZE);
+    const int x1 = piko::imin(piko::imin(hi[0] >> S, (bx + 1) * RASTER_BINSIZE), W);
+    const int y0 = piko::imax(lo[1] >> S, by * RASTER_BINSIZE);
+    const int y1 = piko::imin(piko::imin(hi[1] >> S, (by + 1) * RASTER_BINSIZE), H);
+    const long long area = -edgeDy(p, 2) * edgeDx(p, 0) + edgeDx(p, 2) * edgeDy(p, 0);
+    const float invArea = 1.0f / (float)area;
+    const cvec3f c0 = piko::unpackABGR(p.col[0]), c1 = piko::unpackABGR(p.col[1]), c2 = piko::unpackABGR(p.col[2]);
+    for (int y = y0; y < y1; ++y)
+      for (int x = x0; x < x1; ++x) {
+        const long long cx = ((long long)x << S) + (1 << (S - 1)), cy = ((long long)y << S) + (1 << (S - 1));
+        const long long e0 = edgeAt(p, 0, cx, cy), e1 = edgeAt(p, 1, cx, cy), e2 = edgeAt(p, 2, cx, cy);
+        if ((e0 | e1 | e2) < 0) continue;
+        // weights of vertices 0 and 1 (vertex 2 takes the rest)
+        const float a = (float)e1 * invArea, b = (float)e2 * invArea;
+        const float z = blend(p.z[0] - p.z[2], p.z[1] - p.z[2], p.z[2], a, b);
+        int *stored = (int *)&mutableState->zBuffer[y * W + x];
+        if (piko::atomicMinInt(stored, piko::floatBits(z)) < piko::floatBits(z)) continue;
+        Pixel out;
+        out.pos.x = x;
+        out.pos.y = y;
+        out.color = piko::packABGR(blend(c0.x - c2.x, c1.x - c2.x, c2.x, a, b),
+                                   blend(c0.y - c2.y, c1.y - c2.y, c2.y, a, b),
+                                   blend(c0.z - c2.z, c1.z - c2.z, c2.z, a, b));
+        this->emit(out, 0);
+      }
+  }
+
+ private:
+  // edge e runs from vertex e to vertex (e+1)%3
+  static long long edgeDx(const raster_stri &p, int e) { return (long long)p.xy[2 * ((e + 1) % 3)] - p.xy[2 * e]; }
+  static long long edgeDy(const raster_stri &p, int e) {
+    return (long long)p.xy[2 * ((e + 1) % 3) + 1] - p.xy[2 * e + 1];
+  }
+  // >= 0 on the inner side of edge e (and on the edge)
+  static long long edgeAt(const raster_stri &p, int e, long long x, long long y) {
+    return (y - p.xy[2 * e + 1]) * edgeDx(p, e) - (x - p.xy[2 * e]) * edgeDy(p, e);
+  }
+  // bounding box rounded out to whole pixels of 8 fractional bits
+  static void pixelBox(const raster_stri &p, int lo[2], int hi[2]) {
+    for (int k = 0; k < 2; ++k) {
+      lo[k] = piko::imin(piko::imin(p.xy[k], p.xy[k + 2]), p.xy[k + 4]) & ~0xff;
+      hi[k] = (piko::imax(piko::imax(p.xy[k], p.xy[k + 2]), p.xy[k + 4]) + 0xff) & ~0xff;
+    }
+  }
+  static float blend(float d0, float d1, float base, float a, float b) { return base + a * d0 + b * d1; }
+#endif
 };
 
 #endif

@@ -1,5 +1,6 @@
 // Primitive types of the Reyes sample (reyesPipe/pikoTypes.h:65-79 of the
-// reference): the bicubic patch and the micropolygon.
+// reference): the bicubic patch and the micropolygon, plus the shaded
+// sample the Shade stage emits.
 #ifndef SAMPLE_REYES_TYPES_H
 #define SAMPLE_REYES_TYPES_H
 
@@ -18,6 +19,15 @@ struct piko_patch : public Primitive {
 struct piko_upoly : public Primitive {
   cvec4f screenPos[4];
   cvec3f normal[4];
+};
+
+// One covered sample that passed the depth test: the PikoScreen keeps
+// PIKO_REYES_SPP float RGB samples per pixel and averages them (a box
+// filter) into the frame's FP32 colour.
+struct piko_sample : public Primitive {
+  cvec2i pos;
+  int index;
+  float rgb[3];
 };
 
 #endif

@@ -1,6 +1,8 @@
 """GPU: the C++ samples, compiled by pikoc_hip --target=HIP and g++ against
 the C-ABI, render the same frames as the oracle, bit for bit (PikoScreen
-words and depth)."""
+words and depth), and one profiled frame launches exactly the library
+kernels the generated kernel map names."""
+import json
 import os
 import subprocess
 import sys
@@ -15,6 +17,18 @@ pytestmark = pytest.mark.gpu
 SAMPLES = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "piko-public_amd", "samples")
 
 
+def kernel_map(binary):
+    """(kernels launched every frame, kernels launched only when needed) from
+    the .kernels map pikoc_hip generated for the sample."""
+    d, name = os.path.split(binary)
+    path = os.path.join(SAMPLES, d, "gen_" + name, "__pikoCompiledPipe.kernels")
+    always, maybe = [], []
+    for line in open(path).read().splitlines()[1:]:
+        ks = [t for t in line.split() if t.startswith("k_")]
+        (maybe if line.rstrip().endswith("?") else always).extend(ks)
+    return always, maybe
+
+
 def run_sample(binary, args, tmp_path):
     path = os.path.join(SAMPLES, binary)
     if not os.path.exists(path):
@@ -25,6 +39,10 @@ def run_sample(binary, args, tmp_path):
     assert r.returncode == 0, r.stderr
     print(r.stdout.strip(), flush=True)
     sys.stdout.flush()
+    # the kernels one frame launched are the ones the generated map names
+    launched = [k for entry in json.loads(r.stdout.strip().splitlines()[-1])["kernels"] for k in entry.split("+")]
+    always, maybe = kernel_map(binary)
+    assert set(always) <= set(launched) <= set(always + maybe), (launched, always, maybe)
     return np.fromfile(tmp_path / "color.bin", np.uint32), np.fromfile(tmp_path / "depth.bin", np.float32)
 
 

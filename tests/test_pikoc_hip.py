@@ -172,8 +172,106 @@ def test_emitted_raster_host_code(tmp_path):
         assert needle in h, needle
     d = (tmp_path / "__pikoDefines.h").read_text()
     assert "#define __PIKOC_HIP__" in d and "piko_raster_pipe *piko_hip_pipe_" in d
+    # the path the generated pipe selects, and the kernels that path launches
+    assert "opt.raster_mode = PIKO_RASTER_DIRECT;" in h
     k = (tmp_path / "__pikoCompiledPipe.kernels").read_text()
-    assert "k_raster<8>" in k and "k_vs_count" in k
+    assert [line.split()[-1] for line in k.splitlines()[1:]] == ["k_vs_direct", "?", "k_resolve"]
+    assert "k_large_raster ?" in k and "k_vs_count" not in k
+
+
+@pytest.mark.parametrize("path,kernels", [("QUEUE", ["k_qbin", "k_qraster", "k_qmerge"]),
+                                          ("TILED", ["k_tile_bin", "k_tile_raster"]),
+                                          ("BINNED", ["k_vs_count", "k_scan_tiles", "k_emit_pairs", "k_sort_count",
+                                                      "k_sort_rows", "k_sort_scatter", "k_bounds", "k_raster"])])
+def test_raster_path_setting(tmp_path, path, kernels):
+    be = compile_pipe(os.path.join(SAMPLES, "raster", "rasterPipe.h"), str(tmp_path), [INC],
+                      defines={"PIKO_RASTER_PATH": path}, log=None)
+    assert [k for _, ks, _ in be.kernel_map() for k in ks] == kernels
+    assert ("opt.raster_mode = PIKO_RASTER_%s;" % path) in (tmp_path / "__pikoCompiledPipe.h").read_text()
+    with pytest.raises(BackendError):
+        compile_pipe(os.path.join(SAMPLES, "raster", "rasterPipe.h"), str(tmp_path / "x"), [INC],
+                     defines={"PIKO_RASTER_PATH": "FAST"}, log=None)
+
+
+def copy_sample(tmp_path, sub):
+    import shutil
+    dst = tmp_path / sub
+    shutil.copytree(os.path.join(SAMPLES, sub), dst, ignore=shutil.ignore_patterns("gen_*", "*_sample*"))
+    return dst
+
+
+def run_cli(pipe, outdir, *extra):
+    env = dict(os.environ, PYTHONPATH=PKG)
+    return subprocess.run([sys.executable, "-m", "pikoc_hip", "--target=HIP", "-I", INC, "-o", str(outdir)]
+                          + list(extra) + [str(pipe)], env=env, capture_output=True, text=True)
+
+
+@pytest.mark.parametrize("sub,header,old,new,names", [
+    # a changed depth rule in the raster stage's process
+    ("raster", "raster.h", "< piko::floatBits(z)) continue;", "<= piko::floatBits(z)) continue;",
+     "raster.process"),
+    # a changed bin assignment in the vertex stage
+    ("raster", "vertexShader.h", "(p.id / 512) % getNumBins()", "(p.id / 256) % getNumBins()",
+     "vertexShader.assignBin"),
+    # an off-by-one in the dice stage's grid
+    ("reyes", "stages.h", "const int lastU = PIKO_REYES_DICE > 0 ? nu - 1 : nu",
+     "const int lastU = PIKO_REYES_DICE > 0 ? nu : nu", "dice.process"),
+    # a changed helper outside the three phases
+    ("reyes", "stages.h", "out[0] = piko::saturate(0.2f + d * 0.1773f);",
+     "out[0] = piko::saturate(0.25f + d * 0.1773f);", "code outside"),
+])
+def test_edited_stage_body_exits_5(tmp_path, sub, header, old, new, names):
+    src = copy_sample(tmp_path, sub)
+    text = (src / header).read_text()
+    assert old in text
+    (src / header).write_text(text.replace(old, new, 1))
+    pipe = src / ("rasterPipe.h" if sub == "raster" else "reyesPipe.h")
+    r = run_cli(pipe, tmp_path / "out")
+    assert r.returncode == 5, r.stderr
+    assert names in r.stderr and "does not compile stage code" in r.stderr
+    assert not (tmp_path / "out").exists(), "nothing is written for a refused pipe"
+
+
+def test_emptied_process_body_exits_5(tmp_path):
+    src = copy_sample(tmp_path, "raster")
+    text = (src / "raster.h").read_text()
+    start = text.index("  void process(raster_stri p) {")
+    end = text.index("\n  }\n", start) + 4
+    (src / "raster.h").write_text(text[:start] + "  void process(raster_stri p) {}\n" + text[end:])
+    r = run_cli(src / "rasterPipe.h", tmp_path / "out")
+    assert r.returncode == 5 and "raster.process" in r.stderr
+
+
+def test_reformatted_body_accepted(tmp_path):
+    # layout and comments are not part of a body: re-indented, re-wrapped and
+    # commented stages are still the recognised ones
+    src = copy_sample(tmp_path, "raster")
+    text = (src / "raster.h").read_text()
+    text = text.replace("    ", "\t").replace(" = ", "   =   ").replace("; ", ";\n /* note */ ")
+    (src / "raster.h").write_text(text)
+    assert run_cli(src / "rasterPipe.h", tmp_path / "out").returncode == 0
+
+
+def test_missing_goraud_refused(tmp_path):
+    src = copy_sample(tmp_path, "raster")
+    text = (src / "rasterPipe.h").read_text()
+    assert "#define GORAUD\n" in text
+    (src / "rasterPipe.h").write_text(text.replace("#define GORAUD\n", ""))
+    with pytest.raises(BackendError, match="GORAUD"):
+        compile_pipe(str(src / "rasterPipe.h"), str(tmp_path / "out"), [INC], log=None)
+    assert run_cli(src / "rasterPipe.h", tmp_path / "out").returncode == 5
+
+
+def test_sample_stages_hash_to_library_table():
+    from pikoc_hip import stage_library
+    cases = [("raster", "rasterPipe.h", {}, ["raster.vertex", "raster.raster"], ["pretransformed", "gouraud"]),
+             ("raster", "rasterPipe.h", {"RASTER_WORLD_INPUT": "1"}, ["raster.vertex", "raster.raster"],
+              ["world", "gouraud"]),
+             ("reyes", "reyesPipe.h", {}, ["reyes.split", "reyes.dice", "reyes.shade"], ["split", "dice", "shade"])]
+    for sub, header, defines, roles, variants in cases:
+        ps = parse_pipe(os.path.join(SAMPLES, sub, header), [INC], defines)
+        for st, role, want in zip(ps.order, roles, variants):
+            assert stage_library.match(role, st) == (want, []), (st.name, stage_library.fingerprints(st))
 
 
 def test_emitted_reyes_host_code(tmp_path):
