@@ -154,6 +154,36 @@ __device__ __forceinline__ uint32_t wave_last(uint32_t x) {
   return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
 }
 
+// Inclusive max-scan over the 64 lanes, with the shifts of wave_scan_dpp (a
+// lane without a source takes -1, below every value scanned here).  Every
+// lane must be active.
+__device__ __forceinline__ int wave_max_scan_dpp(int x) {
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x111, 0xf, 0xf, false));  // row_shr:1
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x112, 0xf, 0xf, false));  // row_shr:2
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x114, 0xf, 0xf, false));  // row_shr:4
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x118, 0xf, 0xf, false));  // row_shr:8
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x142, 0xa, 0xf, false));  // row_bcast:15 -> rows 1, 3
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x143, 0xc, 0xf, false));  // row_bcast:31 -> rows 2, 3
+  return x;
+}
+
+// The wave's items are cut into consecutive runs, one per lane: lane i owns
+// [start_i, start_i + len_i) (starts non-decreasing with i, empty runs
+// allowed).  Returns the lane owning item w0 + lane, for every item of the
+// window [w0, w0 + 64) below the total (beyond it, the last run's lane).
+// Each run that reaches into the window marks its first item there in
+// `mark` (64 words of the wave's own), and a max-scan carries each mark up
+// to the next: two LDS round trips, where a binary search over the starts
+// costs six dependent ones.  Every lane must be active.
+__device__ __forceinline__ int wave_owner(uint32_t start, uint32_t len, uint32_t w0, uint32_t *mark) {
+  const int lane = (int)(threadIdx.x & 63);
+  mark[lane] = ~0u;
+  wave_fence();
+  if (len && start < w0 + 64u && start + len > w0) mark[(start > w0 ? start : w0) - w0] = (uint32_t)lane;
+  wave_fence();
+  return wave_max_scan_dpp((int)mark[lane]);
+}
+
 // Rows j in [0, h) of one bbox column where the edge function E + j*D is
 // >= 0 (exact int32 values: an int32-path triangle inside its bbox), as
 // [lo, hi] (empty: lo > hi).  The float quotient locates the boundary to
@@ -186,6 +216,7 @@ struct QColumns {
   uint32_t pre[64];   // exclusive prefix of the columns' covered-pixel counts
   uint32_t info[64];  // staged triangle | px << 6 | first covered row << 18
   int e1[64], e2[64];
+  uint32_t mark[64];  // wave_owner's marks
 };
 
 // Tiles of a kept triangle's clipped bbox that pass the trivial-reject test
@@ -383,7 +414,16 @@ __global__ void __launch_bounds__(kQThreads) k_qraster(QArgs qa, FrameArgs fa) {
     const uint32_t pend = min(it.cap, pbeg + (uint32_t)qa.part_pairs);
     const uint32_t *list = qa.pairs + it.base;
     const uint32_t fill = qa.tile_fill[gt];
-    auto raw = [&](uint32_t j) -> uint32_t { return j < pend ? list[j] : ~0u; };
+    // Every load of the pipeline below is issued unconditionally (an index
+    // out of range reads a valid word, the pair buffer's first or the setup
+    // buffer's first record, and the value is then dropped).  A load under
+    // a branch that a whole wave can skip leaves the compiler unable to count
+    // the loads in flight, and it then waits for all of them, prefetches
+    // included, at the first use of any.
+    auto raw = [&](uint32_t j) -> uint32_t {
+      const uint32_t x = *(j < pend ? list + j : qa.pairs);
+      return j < pend ? x : ~0u;
+    };
     uint32_t r_cur = raw(pbeg + (uint32_t)t), r_nxt = raw(pbeg + (uint32_t)(kQThreads + t));
     // a tile whose queue overflowed has runs that were refused and left
     // unwritten slots behind them: it draws nothing (the frame is re-run)
@@ -393,15 +433,16 @@ __global__ void __launch_bounds__(kQThreads) k_qraster(QArgs qa, FrameArgs fa) {
     __syncthreads();  // visibility words initialised
     TileStage &s = sm.st[t >> 6];
     {
-      auto entry = [&](uint32_t j) -> uint32_t { return j < end ? list[j] : ~0u; };
+      auto entry = [&](uint32_t j) -> uint32_t {
+        const uint32_t x = *(j < end ? list + j : qa.pairs);
+        return j < end ? x : ~0u;
+      };
+      // the record of an absent triangle (~0u) is loaded from record 0 and
+      // never read
       auto setup = [&](uint32_t v, int4 *out) {
-        if (v != ~0u) {
+        const int4 *g = qa.geo + 3 * (size_t)(v != ~0u ? v : 0u);
 #pragma unroll
-          for (int q = 0; q < 3; ++q) out[q] = qa.geo[3 * (size_t)v + q];
-        } else {
-#pragma unroll
-          for (int q = 0; q < 3; ++q) out[q] = make_int4(0, 0, 0, 0);
-        }
+        for (int q = 0; q < 3; ++q) out[q] = g[q];
       };
       // software pipeline: the setup records of the next batch and the
       // entries of the one after are in flight while a batch is rasterised
@@ -457,15 +498,13 @@ __global__ void __launch_bounds__(kQThreads) k_qraster(QArgs qa, FrameArgs fa) {
         // 8-row spans: every bbox pixel tested (int64-path triangles always)
         const uint32_t sinc = wave_scan_dpp(uspan);
         const uint32_t stot = wave_last(sinc);
+        QColumns &cl = sm.cols[t >> 6];
         if (stot) {
           s.pre[lane] = sinc - uspan;
-          wave_fence();
-          for (uint32_t cidx = (uint32_t)lane; cidx < stot; cidx += 64) {
-            int o = 0;  // owner: largest o with pre[o] <= cidx
-#pragma unroll
-            for (int step = 32; step > 0; step >>= 1)
-              if (s.pre[o + step] <= cidx) o += step;
-            q_span(s, o, cidx - s.pre[o], S, tl.xlo, tl.ylo, T, vis);
+          for (uint32_t c0 = 0; c0 < stot; c0 += 64) {
+            const uint32_t cidx = c0 + (uint32_t)lane;
+            const int o = wave_owner(sinc - uspan, uspan, c0, cl.mark);  // fences s.pre too
+            if (cidx < stot) q_span(s, o, cidx - s.pre[o], S, tl.xlo, tl.ylo, T, vis);
           }
           wave_fence();
         }
@@ -476,15 +515,10 @@ __global__ void __launch_bounds__(kQThreads) k_qraster(QArgs qa, FrameArgs fa) {
         const uint32_t ctot = wave_last(cinc);
         const uint32_t cpre = cinc - ucol;
         s.pre[lane] = cpre;
-        QColumns &cl = sm.cols[t >> 6];
         const uint32_t half = 1u << (S - 1);
-        wave_fence();
         for (uint32_t c0 = 0; c0 < ctot; c0 += 64) {
           const uint32_t cidx = c0 + (uint32_t)lane;
-          int o = 0;  // owner: largest o with pre[o] <= cidx
-#pragma unroll
-          for (int step = 32; step > 0; step >>= 1)
-            if (s.pre[o + step] <= cidx) o += step;
+          const int o = wave_owner(cpre, ucol, c0, cl.mark);  // fences s.pre too
           uint32_t cnt = 0;
           if (cidx < ctot) {
             const int4 a = s.xy01[o];
@@ -516,13 +550,9 @@ __global__ void __launch_bounds__(kQThreads) k_qraster(QArgs qa, FrameArgs fa) {
           const uint32_t ftot = wave_last(finc);
           const uint32_t fpre = finc - cnt;
           cl.pre[lane] = fpre;
-          wave_fence();
           for (uint32_t f0 = 0; f0 < ftot; f0 += 64) {
             const uint32_t f = f0 + (uint32_t)lane;
-            int c = 0;  // column: largest c with pre[c] <= f
-#pragma unroll
-            for (int step = 32; step > 0; step >>= 1)
-              if (cl.pre[c + step] <= f) c += step;
+            const int c = wave_owner(fpre, cnt, f0, cl.mark);  // fences cl.pre, info, e1, e2 too
             if (f < ftot) {
               const uint32_t r = f - cl.pre[c];
               const uint32_t info = cl.info[c];
