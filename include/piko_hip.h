@@ -253,7 +253,10 @@ typedef struct {
   int subpixel_bits;
   int pipes;                    /* batch pipes (HIP streams) in flight, <= 16 */
   int max_jobs;                 /* requests per batch at most */
-  long long batch_budget_bytes; /* device bytes one batch may use (admission) */
+  long long batch_budget_bytes; /* device bytes one batch may use (admission);
+                                   at most each pipe's share of nine tenths of
+                                   the device memory free once the geometry is
+                                   resident */
 } piko_server_options;
 
 typedef struct {

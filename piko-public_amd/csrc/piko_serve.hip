@@ -6,7 +6,9 @@
 // finished frames with piko_server_poll.  One worker thread owns the pipes:
 // whenever a pipe is free it takes the requests that have arrived, in arrival
 // order, as long as the batch fits the per-batch device-memory budget
-// (framebuffer + per-triangle + queue footprint), and launches them as one
+// (framebuffer + per-triangle + queue footprint; the budget asked for, at
+// most each pipe's share of the device memory free at creation), and
+// launches them as one
 // batch; while one batch runs the next forms and launches on another pipe, so
 // batches overlap on the device.  A finished batch's frames are copied to the
 // buffers their requests named and reported as completions.
@@ -213,6 +215,14 @@ int piko_server_create(piko_server **out, const float *pos, const void *attr, lo
                           hipMemcpy(s->d_attr, attr, ab, hipMemcpyHostToDevice) != hipSuccess))
       rc = PIKO_ERR_HIP;
   }
+  // admission never promises more than the device holds: a pipe's batch
+  // budget is at most its share of the memory free once the geometry is
+  // resident (nine tenths of it, leaving room for the runtime)
+  size_t free_b = 0, total_b = 0;
+  if (rc == PIKO_OK && hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+    const long long share = (long long)(free_b / 10 * 9) / opt->pipes;
+    if (share > 0) s->budget = std::min(s->budget, share);
+  }
   long long max_scene = 1;
   for (int c : s->scene_count) max_scene = std::max<long long>(max_scene, c);
   piko_batch_options bo{};
@@ -274,6 +284,7 @@ int piko_server_destroy(piko_server *s) {
 int piko_server_submit(piko_server *s, const piko_request *req, uint32_t *color, long long *id) {
   if (!s || !req || !id) return PIKO_ERR_INVALID;
   if (req->scene < 0 || req->scene >= (int)s->scene_count.size()) return PIKO_ERR_INVALID;
+  if (req->cs.screenSizeX <= 0 || req->cs.screenSizeY <= 0) return PIKO_ERR_INVALID;
   if (footprint(s, *req) > s->budget) return PIKO_ERR_BUSY;  // never admissible
   int rc = start_worker(s);
   if (rc != PIKO_OK) return rc;

@@ -1,5 +1,6 @@
 """GPU serving: requests replayed with continuous batching give, for every
 request, exactly the frame it gives alone and the oracle's frame."""
+import ctypes
 import sys
 
 import numpy as np
@@ -55,10 +56,11 @@ def test_admission_refuses_what_never_fits():
 
 
 def test_pipes_fall_back_when_up_front_sizing_does_not_fit(oracle_mod):
-    # a request cap and batch budget far beyond device memory: the pipes cannot
-    # be sized for the largest admissible batch up front (1.4e11 pixels), so
-    # they start at one request and grow with their batches; the server still
-    # starts and every frame is the oracle's
+    # a request cap and batch budget far beyond device memory: the budget is
+    # cut to each pipe's share of the free memory, the first pipe is sized to
+    # it up front, and the second no longer fits beside it, so it starts at
+    # one request and grows with its batches; the server still starts and
+    # every frame is the oracle's
     pos, nrm, base, count = serve.resident_scenes(4, 3000, seed=4)
     reqs, meta = serve.poisson_trace(12, 50000.0, 4, seed=6, resolutions=SMALL_RES)
     with serve.Server(pos, nrm, base, count, pipes=2, max_jobs=1 << 16, budget_bytes=1 << 44) as s:
@@ -96,3 +98,25 @@ def test_online_submit_while_batches_run():
             one[0].arrival_s = 0.0
             _, _, c1 = s.replay(one, keep=True)
             assert np.array_equal(c1[0], col), i
+
+
+def test_admission_bounded_by_device_memory():
+    # a batch budget far beyond the device: admission uses each pipe's share
+    # of the free device memory instead, so a request that could never be
+    # rendered beside the other pipes (a 40000 x 40000 screen, 25.6 GB of
+    # framebuffer and visibility words, against 16 pipes' shares of one GPU)
+    # is refused at submission, while an ordinary one is served
+    pos, nrm, base, count = serve.resident_scenes(2, 2000, seed=11)
+    reqs, _ = serve.poisson_trace(2, 1000.0, 2, seed=3, resolutions=SMALL_RES)
+    with serve.Server(pos, nrm, base, count, pipes=16, max_jobs=4, budget_bytes=1 << 60) as s:
+        huge = serve.Request()
+        ctypes.pointer(huge)[0] = reqs[0]
+        huge.cs.screenSizeX = huge.cs.screenSizeY = 40000
+        with pytest.raises(piko_amd.PikoError) as e:
+            s.submit(huge, keep=False)
+        assert e.value.status == -6
+        rid, col = s.submit(reqs[1])
+        done = []
+        while not done:
+            done = s.poll(4, 100)
+        assert done[0][0] == rid and done[0][2] == 0
