@@ -90,7 +90,7 @@ enum { PIKO_INPUT_PRETRANSFORMED = 0, PIKO_INPUT_WORLD = 1 };
  *           (the frame re-runs) when a later frame needs more.
  *   DIRECT, TILED and QUEUE keep no sorted queues: the bin queues are built
  *   on demand when asked for (piko_raster_get_bins).
- *   AUTO:   DIRECT for a plain pipe, TILED for a batch.  The environment
+ *   AUTO:   DIRECT for a plain pipe, QUEUE for a batch.  The environment
  *           variable PIKO_RASTER_MODE=binned|direct|tiled|queue overrides AUTO. */
 enum {
   PIKO_RASTER_AUTO = 0,
@@ -218,8 +218,8 @@ typedef struct {
   long long max_pixels;    /* initial per-batch pixel capacity (grows) */
   long long pair_capacity; /* 0 = automatic */
   int raster_mode;         /* PIKO_RASTER_* for every batch; 0 = AUTO */
-  int part_pairs;          /* QUEUE: queue entries per part item; 0 = 1024 */
-  int waves_per_cu;        /* QUEUE: persistent waves per CU; 0 = 16 */
+  int part_pairs;          /* QUEUE: queue entries per part item; 0 = 2048 */
+  int waves_per_cu;        /* QUEUE: persistent waves per CU; 0 = 24 */
 } piko_batch_options;
 
 /* Creates a batch pipe over `input_tris` resident triangles (pos/attr as for

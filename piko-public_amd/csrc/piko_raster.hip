@@ -2476,12 +2476,14 @@ int create_common(piko_raster_pipe *p, int kind, int S, long long tris, long lon
   p->kind = kind;
   p->S = S;
   if (mode == PIKO_RASTER_AUTO) {
-    // plain frames: direct (fewest passes over one scene); batches: tiled
-    // (many fragments per pixel across uneven screens stay in LDS)
+    // plain frames: direct (fewest passes over one scene); batches: queue
+    // (one persistent raster launch over every job's tile parts, each tile
+    // resolved in LDS)
     const char *m = std::getenv("PIKO_RASTER_MODE");
-    mode = p->multi ? PIKO_RASTER_TILED : PIKO_RASTER_DIRECT;
+    mode = p->multi ? PIKO_RASTER_QUEUE : PIKO_RASTER_DIRECT;
     if (m && std::strcmp(m, "binned") == 0) mode = PIKO_RASTER_BINNED;
     if (m && std::strcmp(m, "direct") == 0) mode = PIKO_RASTER_DIRECT;
+    if (m && std::strcmp(m, "tiled") == 0) mode = PIKO_RASTER_TILED;
     if (m && std::strcmp(m, "queue") == 0) mode = PIKO_RASTER_QUEUE;
   }
   p->mode = p->last_mode = mode;
@@ -2893,8 +2895,10 @@ int piko_batch_create(piko_raster_pipe **out, const float *pos, const void *attr
   p->multi = true;
   p->input_tris = input_tris;
   p->cap_jobs = opt->max_jobs;
-  p->q_part_pairs = opt->part_pairs > 0 ? opt->part_pairs : 1024;
-  p->q_wpc = opt->waves_per_cu > 0 ? opt->waves_per_cu : 16;
+  // a batch's queue path: longer parts and a wider persistent grid (the
+  // 256-scene batch's tuned setting; a single 1080p frame tunes to 1024 / 16)
+  p->q_part_pairs = opt->part_pairs > 0 ? opt->part_pairs : 2048;
+  p->q_wpc = opt->waves_per_cu > 0 ? opt->waves_per_cu : 24;
   const int bins = (int)std::min<long long>(opt->max_pixels / 64 + 1024, 1ll << 30);
   int rc = create_common(p, opt->input_kind, opt->subpixel_bits, opt->max_tris, opt->max_pixels, bins,
                          opt->pair_capacity, opt->raster_mode);
