@@ -218,8 +218,8 @@ def bench_batch(log, reps=5):
                        "pixels log-uniform [256^2,2048^2], aspect {1,4:3,16:9}, autotuned (plain pipes: raster "
                        "path and bin size per class; batch: tile size, waves per CU and part size, timed in "
                        "the batch)",
-           "form": "queue path (the batch default): bin launch + one persistent raster launch over every "
-                   "job's tile parts (+ merge of split tiles)",
+           "form": "queue path: bin launch + one persistent raster launch over every job's tile parts "
+                   "(+ merge of split tiles), frames of one submitted job set",
            "scenes": len(scs), "triangles": ntris, "pixels": npix, "classes": len(table),
            "seq_ms_per_batch": round(t_seq * 1e3, 3), "batch_ms": round(t_queue * 1e3, 3),
            "seq_frames_per_s": round(len(scs) / t_seq, 1), "batch_frames_per_s": round(len(scs) / t_queue, 1),

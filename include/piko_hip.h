@@ -90,7 +90,9 @@ enum { PIKO_INPUT_PRETRANSFORMED = 0, PIKO_INPUT_WORLD = 1 };
  *           (the frame re-runs) when a later frame needs more.
  *   DIRECT, TILED and QUEUE keep no sorted queues: the bin queues are built
  *   on demand when asked for (piko_raster_get_bins).
- *   AUTO:   DIRECT for a plain pipe, QUEUE for a batch.  The environment
+ *   AUTO:   DIRECT for a plain pipe, TILED for a batch (QUEUE is faster
+ *           for repeated frames of one job set, but the first frame after a
+ *           submit runs a counting pass first).  The environment
  *           variable PIKO_RASTER_MODE=binned|direct|tiled|queue overrides AUTO. */
 enum {
   PIKO_RASTER_AUTO = 0,

@@ -11,8 +11,7 @@ from piko_amd import scenes
 
 pytestmark = pytest.mark.gpu
 
-# auto: a batch's default, the queue path with the batch's default part size
-# and persistent grid
+# auto: a batch's default (the tiled path)
 MODES = {"binned": piko_amd.RASTER_BINNED, "direct": piko_amd.RASTER_DIRECT, "tiled": piko_amd.RASTER_TILED,
          "queue": piko_amd.RASTER_QUEUE, "auto": piko_amd.RASTER_AUTO}
 
