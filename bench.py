@@ -241,6 +241,35 @@ def bench_batch(log, reps=5):
     return out
 
 
+def bench_batch_queue(log, T, waves, part_pairs, steps, warmup):
+    """The 256-scene batch on the queue path alone, at fixed settings: one
+    submit, `warmup` untimed frames, `steps` timed ones.  Each frame is one
+    bin launch, one persistent raster launch and (when a tile is split) one
+    merge launch; the submit adds the counting pass of the first frame."""
+    import piko_amd
+    from piko_amd import scenes
+
+    spec = scenes.batch_spec()
+    scs = [scenes.tris_uniform(seed, n, W, H) for seed, n, W, H in spec]
+    ntris = sum(s.n for s in scs)
+    npix = sum(s.W * s.H for s in scs)
+    pos, attr, parts = piko_amd.concat_scenes(scs)
+    log("batch-queue: %d scenes, %d triangles, %d pixels, T=%d, %d waves/CU, parts of %d" % (
+        len(scs), ntris, npix, T, waves, part_pairs))
+    with piko_amd.BatchPipe(pos, attr, "pretransformed", max_jobs=len(scs), max_tris=ntris, max_pixels=npix,
+                            raster_mode=piko_amd.RASTER_QUEUE, part_pairs=part_pairs, waves_per_cu=waves) as bp:
+        bp.submit([(n, base, T, piko_amd.ConstantState.for_scene(s)) for (n, base, s) in parts])
+        bp.wait()
+        bp.run_frames(max(warmup, 1))
+        t0 = time.perf_counter()
+        bp.run_frames(steps)
+        dt = (time.perf_counter() - t0) / steps
+    return {"metric": "batch frames/s (256 scenes)", "workload": "batch-queue", "scenes": len(scs),
+            "triangles": ntris, "pixels": npix, "bin_size": T, "waves_per_cu": waves, "part_pairs": part_pairs,
+            "steps": steps, "warmup": warmup, "batch_ms": round(dt * 1e3, 3),
+            "frames_per_s": round(len(scs) / dt, 1)}
+
+
 def bench_serve(log, ds, nreq, rate_hz=2000.0):
     """64 resident world-space scenes (soups and height fields, `ds`, built
     in device memory), Poisson requests with random cameras and
@@ -381,7 +410,11 @@ def main():
     ap.add_argument("--bins", default="8,16,32,64", help="bin sizes of the binned raster mode")
     ap.add_argument("--modes", default="queue,tiled,direct,binned", help="raster modes to sweep")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="all", choices=["raster", "batch", "serve", "reyes", "all"])
+    ap.add_argument("--workload", default="all", choices=["raster", "batch", "serve", "reyes", "all", "batch-queue"],
+                    help="batch-queue: only the 256-scene batch on the queue path at --batch-queue settings "
+                         "(for kernel traces: one bin, one raster and one merge launch per frame)")
+    ap.add_argument("--batch-queue", default="64,24,2048",
+                    help="tile size, waves per CU and part size of the batch-queue workload")
     ap.add_argument("--reyes-sampling", default="auto", choices=["auto", "global", "tiled"],
                     help="how the Reyes frame composites samples (auto = global)")
     ap.add_argument("--reyes-tile", type=int, default=0, help="tiled sampling's tile edge: 16 or 32 (0 = 16)")
@@ -398,6 +431,10 @@ def main():
         print("bench.py measures one GPU only: this build runs a single MI355X (no multi-GPU path); "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch-queue":
+        T, w, pp = (int(x) for x in args.batch_queue.split(","))
+        print(json.dumps(bench_batch_queue(log, T, w, pp, args.steps, args.warmup)), flush=True)
+        return
 
     import piko_amd
     from piko_amd import scenes
