@@ -91,3 +91,8 @@ def test_batch_resubmit_different_sizes(mode):
             for j, k in enumerate(order):
                 pc, _ = plain(a[k], 16)
                 assert np.array_equal(bp.color(j), pc)
+            # several frames of the job set in one call
+            bp.run_frames(3)
+            for j, k in enumerate(order):
+                pc, _ = plain(a[k], 16)
+                assert np.array_equal(bp.color(j), pc)

@@ -191,6 +191,28 @@ def test_direct_frames_across_camera_changes(oracle_mod):
             assert np.array_equal(c, oc) and np.array_equal(d.view(np.int32), od.view(np.int32)), step
 
 
+@pytest.mark.parametrize("T", [32, 64])
+def test_queue_frames_across_camera_changes(oracle_mod, T):
+    # queue-path runs of several frames with the camera changed between runs
+    # and small parts (many split tiles): moving near the terrain overflows
+    # tile ranges sized on the far view (the ranges grow and the frames
+    # re-run), and each run ends on the oracle's image
+    import piko_amd
+    far = scenes.heightfield(seed=6, grid=80, W=400, H=240, eye=(0.0, 2.5, 3.0), target=(0.0, 0.0, 0.0))
+    near = scenes.heightfield(seed=6, grid=80, W=400, H=240, eye=(0.0, 0.12, 0.25), target=(0.0, 0.0, -0.3))
+    ofar, odfar, _ = oracle_mod.frame(far, 8, T)
+    onear, odnear, _ = oracle_mod.frame(near, 8, T)
+    with piko_amd.RasterPipe().allocate(far, T, 8, raster_mode=piko_amd.RASTER_QUEUE, part_pairs=64) as pipe:
+        pipe.prepare()
+        for step, (sc, oc, od, n) in enumerate([(far, ofar, odfar, 4), (near, onear, odnear, 2), (near, onear, odnear, 5),
+                                                (far, ofar, odfar, 3), (near, onear, odnear, 1), (far, ofar, odfar, 6)]):
+            log("run %d: %d frames" % (step, n))
+            pipe.prepare(sc)
+            pipe.run_frames(n)
+            c, d = pipe.pikoScreen.getData(), pipe.get_depth()
+            assert np.array_equal(c, oc) and np.array_equal(d.view(np.int32), od.view(np.int32)), step
+
+
 def test_tiled_large_lists(oracle_mod):
     # every triangle spans more than 2x2 tiles of 16 pixels: all of them go to
     # the chunks' large lists and are clipped to each tile they touch
