@@ -55,21 +55,30 @@ def cpu_model():
 
 
 def host_info():
-    """The CPU the baselines ran on: lscpu's model name, nproc (the CPUs this
-    process may use) and the logical CPU count."""
+    """The CPU the baselines ran on: lscpu's model name (with its family /
+    model / stepping numbers and base/max clock where lscpu reports them),
+    nproc (the CPUs this process may use) and the logical CPU count."""
     model = None
+    extra = {}
+    keys = {"CPU family:": "family", "Model:": "model_number", "Stepping:": "stepping",
+            "CPU max MHz:": "max_mhz", "Socket(s):": "sockets", "Thread(s) per core:": "threads_per_core"}
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
         for line in out.splitlines():
             if line.startswith("Model name:"):
                 model = line.split(":", 1)[1].strip()
+            for k, name in keys.items():
+                if line.startswith(k):
+                    extra[name] = line.split(":", 1)[1].strip()
     except (OSError, subprocess.SubprocessError):
         pass
     try:
         nproc = int(subprocess.run(["nproc"], capture_output=True, text=True, timeout=10).stdout.strip())
     except (OSError, ValueError, subprocess.SubprocessError):
         nproc = len(os.sched_getaffinity(0))
-    return {"model": model or cpu_model(), "nproc": nproc, "logical_cpus": os.cpu_count()}
+    info = {"model": model or cpu_model(), "nproc": nproc, "logical_cpus": os.cpu_count()}
+    info.update(extra)
+    return info
 
 
 def cpu_baseline(scene, T, S, budget_s=12.0):
