@@ -221,8 +221,9 @@ def bench_batch(log, reps=5):
     tiles, btable = autotune.tune_batch(scs, pos, attr, parts)
     t_batch, k_tiled, exact = run_batch(piko_amd.RASTER_TILED, tiles)
     qcfg, _ = autotune.tune_queue_batch(scs, pos, attr, parts)
-    t_queue, k_queue, exact_q = run_batch(piko_amd.RASTER_QUEUE, [qcfg["bin_size"]] * len(scs),
+    t_queue, k_queue, exact_q = run_batch(piko_amd.RASTER_QUEUE, qcfg["tiles"],
                                           part_pairs=qcfg["part_pairs"], waves_per_cu=qcfg["waves_per_cu"])
+    qcfg["tiles"] = {str(T): qcfg["tiles"].count(T) for T in sorted(set(qcfg["tiles"]))}
     out = {"workload": "256 uniform-soup scenes, seeds 1000-1255, N log-uniform [1K,500K], "
                        "pixels log-uniform [256^2,2048^2], aspect {1,4:3,16:9}, autotuned (plain pipes: raster "
                        "path and bin size per class; batch: tile size, waves per CU and part size, timed in "
