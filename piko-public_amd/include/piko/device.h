@@ -3,10 +3,11 @@
  * api/include/piko/{deviceFunctions,math,atomics}.h that the sample stages
  * use).  Stage bodies sit in #ifdef __PIKOC_DEVICE__ blocks, as in the
  * reference's .pikostage files: pikoc_hip reads them with __PIKOC_DEVICE__
- * defined and maps the recognised ones onto the kernel library; a host
- * compile with -D__PIKOC_DEVICE__ -fsyntax-only type-checks them against
- * this header.  The pure arithmetic helpers are defined here; the two that
- * need the device (the atomic and the emit in Stage) are declarations.
+ * defined, maps the recognised ones onto the kernel library and compiles
+ * the others for gfx950 (the helpers below then run on the device as
+ * written).  The pure arithmetic helpers are defined here; the atomic, like
+ * the Stage members a body calls, is declared only: piko/stage_exec.h
+ * defines it on the device, the tests' host harness on the host.
  */
 #ifndef PIKO_DEVICE_H
 #define PIKO_DEVICE_H

@@ -4,11 +4,13 @@
  * api/include/piko/deviceFunctions.h:22-50, api/include/internal/
  * datatypes.h:114-144).
  *
- * On this backend a stage class is a description: pikoc_hip reads its bin
- * size, threads per tile, primitive types, directives and bodies from the
- * source (with __PIKOC_DEVICE__ defined) and maps the pipe onto the kernel
- * library when every stage body is one the library implements.  The
- * directive functions therefore do nothing when called from host code.
+ * pikoc_hip reads a stage class's bin size, threads per tile, primitive
+ * types, directives and bodies from the source (with __PIKOC_DEVICE__
+ * defined).  It maps the pipe onto the kernel library when every stage body
+ * is one the library implements, and otherwise compiles the bodies for
+ * gfx950, where piko/stage_exec.h defines the members declared below under
+ * __PIKOC_DEVICE__.  The directive functions do nothing: the schedule is
+ * the library's or stage_exec.h's.
  */
 #ifndef PIKO_STAGE_H
 #define PIKO_STAGE_H
@@ -49,11 +51,13 @@ class Stage : public StageBase {
   /* What a stage body sees (stage.h:22-24, deviceFunctions.h:63-109): bin
    * assignment, the frame's uniforms, the mutable state (the depth buffer),
    * the bin being processed and the bin grid, and emit to an output port.
-   * Declared only: whatever executes the bodies defines them. */
+   * Declared only: whatever executes the bodies defines them (the compiled
+   * stage code in piko/stage_exec.h, the host harness of the tests). */
   void assignToBin(const InPrim &p, int bin);
   void assignToBin(const InPrim &p, AssignPolicy policy);
   ConstantState constState;
   MutableState *mutableState;
+  void *pikoRun_ = nullptr;  // the executing schedule's state for this body
   int getBinID() const;
   int getNumBins() const;
   int getNumBinsX() const;

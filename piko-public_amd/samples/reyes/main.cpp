@@ -79,16 +79,22 @@ int main(int argc, char **argv) {
   }
   const double ms_per = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() /
                         (runs > 0 ? runs : 1);
-  // the library kernels one frame of the generated pipe launches
+  // the kernels one frame of the generated pipe launches, with their times
   float kms[64], frame_ms = 0.0f;
   const char *knames[64];
   int nk = 0;
-  PIKO_CHECK(piko_reyes_profile_frames(pipe.piko_hip_pipe_, 1, kms, knames, 64, &nk, &frame_ms));
+  PIKO_CHECK(piko_profile_frame(pipe, kms, knames, 64, &nk, &frame_ms));
   std::string kernels;
-  for (int k = 0; k < nk; ++k) kernels += std::string(k ? ", \"" : "\"") + knames[k] + "\"";
+  std::string kernel_ms;
+  for (int k = 0; k < nk; ++k) {
+    kernels += std::string(k ? ", \"" : "\"") + knames[k] + "\"";
+    char t[32];
+    snprintf(t, sizeof(t), "%s%.4f", k ? ", " : "", kms[k]);
+    kernel_ms += t;
+  }
   pipe.destroy();
   printf("{\"scene\": \"sphere-4096\", \"W\": %d, \"H\": %d, \"patches\": %d, \"color_fnv1a\": \"%016llx\", "
-         "\"depth_fnv1a\": \"%016llx\", \"ms_per_frame\": %.4f, \"kernels\": [%s]}\n",
-         W, H, n, (unsigned long long)hc, (unsigned long long)hd, ms_per, kernels.c_str());
+         "\"depth_fnv1a\": \"%016llx\", \"ms_per_frame\": %.4f, \"kernels\": [%s], \"kernel_ms\": [%s]}\n",
+         W, H, n, (unsigned long long)hc, (unsigned long long)hd, ms_per, kernels.c_str(), kernel_ms.c_str());
   return 0;
 }

@@ -1,7 +1,11 @@
 """GPU: the C++ samples, compiled by pikoc_hip --target=HIP and g++ against
 the C-ABI, render the same frames as the oracle, bit for bit (PikoScreen
-words and depth), and one profiled frame launches exactly the library
-kernels the generated kernel map names."""
+words and depth), and one profiled frame launches exactly the kernels the
+generated kernel map names.  Each sample runs both ways pikoc_hip binds a
+pipe: on the library's kernels, and as its own stage bodies compiled for
+gfx950 (the *_compiled builds).  The per-pixel-lit sample, whose bodies the
+library does not implement, is checked against the same bodies run serially
+on the host (tests/stage_host/run_stages.cpp)."""
 import json
 import os
 import subprocess
@@ -46,27 +50,48 @@ def run_sample(binary, args, tmp_path):
     return np.fromfile(tmp_path / "color.bin", np.uint32), np.fromfile(tmp_path / "depth.bin", np.float32)
 
 
-def test_raster_sample_uniform(oracle_mod, tmp_path):
-    color, depth = run_sample("raster/raster_sample", [], tmp_path)
+@pytest.mark.parametrize("binary", ["raster/raster_sample", "raster/raster_sample_compiled"])
+def test_raster_sample_uniform(oracle_mod, tmp_path, binary):
+    color, depth = run_sample(binary, [], tmp_path)
     sc = scenes.tris_uniform(1, 50000, 512, 512)
     oc, od = oracle_mod.frame(sc, 8, 8)[:2]
     assert np.array_equal(color, oc.reshape(-1))
     assert np.array_equal(depth.view(np.int32), od.reshape(-1).view(np.int32))
 
 
-def test_raster_sample_heightfield(oracle_mod, tmp_path):
-    color, depth = run_sample("raster/raster_sample_world", ["--scene", "heightfield"], tmp_path)
+@pytest.mark.parametrize("binary", ["raster/raster_sample_world", "raster/raster_sample_world_compiled"])
+def test_raster_sample_heightfield(oracle_mod, tmp_path, binary):
+    color, depth = run_sample(binary, ["--scene", "heightfield"], tmp_path)
     sc = scenes.heightfield()
     oc, od = oracle_mod.frame(sc, 8, 8)[:2]
     assert np.array_equal(color, oc.reshape(-1))
     assert np.array_equal(depth.view(np.int32), od.reshape(-1).view(np.int32))
 
 
-def test_reyes_sample(oracle_mod, tmp_path):
-    color, depth = run_sample("reyes/reyes_sample", [], tmp_path)
+@pytest.mark.parametrize("binary", ["reyes/reyes_sample", "reyes/reyes_sample_compiled"])
+def test_reyes_sample(oracle_mod, tmp_path, binary):
+    color, depth = run_sample(binary, [], tmp_path)
     sc = scenes.sphere_patches()
     rgba, z, _ = oracle_mod.reyes_frame(sc)
     c = (rgba * np.float32(255.0)).astype(np.uint32)  # cvec2uintcolor, truncating
     words = (c[..., 3] << 24) | (c[..., 2] << 16) | (c[..., 1] << 8) | c[..., 0]
     assert np.array_equal(color, words.reshape(-1))
     assert np.array_equal(depth.view(np.int32), z.min(axis=2).reshape(-1))
+
+
+def test_lit_sample_equals_its_bodies_run_on_the_host(oracle_mod, tmp_path):
+    """The compiled per-pixel-lit stages on the 2M-triangle height field: the
+    GPU frame equals the same stage bodies executed serially on the host,
+    bit for bit, and its depth is the oracle's."""
+    from test_sample_stages import build as build_host, run as run_host
+    (tmp_path / "gpu").mkdir()
+    color, depth = run_sample("raster_lit/raster_lit_sample", ["--scene", "heightfield"], tmp_path / "gpu")
+    sc = scenes.heightfield()
+    exe = build_host(tmp_path, "raster_lit_host", "raster_lit", {"PIKO_SUBPIXEL_BITS": "8"})
+    raw, _ = run_host(exe, tmp_path, sc, (sc.pos.astype(np.float32), sc.nrm.astype(np.float32)))
+    npix = sc.W * sc.H
+    hc, hd = raw[:4 * npix].view(np.uint32), raw[4 * npix:].view(np.int32)
+    assert np.array_equal(color, hc), "colour differs at %d pixels" % int((color != hc).sum())
+    assert np.array_equal(depth.view(np.int32), hd)
+    od = oracle_mod.frame(sc, 8, 8)[1]
+    assert np.array_equal(depth.view(np.int32), od.reshape(-1).view(np.int32))

@@ -138,10 +138,14 @@ class P : public PikoPipe {
     assert a.assign_policy == "InBin" and a.assign_trivial and b.wait_policy == "EndBin" and b.sched_trivial
     plan = ps.kernel_plan()
     assert len(plan) == 2 and plan[1][1] == ["a.process", "emit[0] -> b.assignBin", "b.process"]
-    # not a graph the kernel library implements (bins 16 is fine, but the
-    # raster stage must drain to the screen from a different input type)
+    # not bodies the kernel library implements: refused when the library is
+    # required, compiled otherwise
     with pytest.raises(BackendError):
-        compile_pipe(str(tmp_path / "p.h"), str(tmp_path / "out"), [INC], log=None)
+        compile_pipe(str(tmp_path / "p.h"), str(tmp_path / "out"), [INC], defines={"PIKO_STAGE_CODE": "LIBRARY"},
+                     log=None)
+    be = compile_pipe(str(tmp_path / "p.h"), str(tmp_path / "out2"), [INC], log=None, build=False)
+    assert be.kind == "compiled" and be.shape == "raster" and be.classes == ["A", "B"]
+    assert "pikox::RasterExec<A, B>" in (tmp_path / "out2" / "__pikoStages.hip").read_text()
 
 
 def test_source_errors(tmp_path):
@@ -206,7 +210,7 @@ def run_cli(pipe, outdir, *extra):
                           + list(extra) + [str(pipe)], env=env, capture_output=True, text=True)
 
 
-@pytest.mark.parametrize("sub,header,old,new,names", [
+EDITS = [
     # a changed depth rule in the raster stage's process
     ("raster", "raster.h", "< piko::floatBits(z)) continue;", "<= piko::floatBits(z)) continue;",
      "raster.process"),
@@ -219,27 +223,69 @@ def run_cli(pipe, outdir, *extra):
     # a changed helper outside the three phases
     ("reyes", "stages.h", "out[0] = piko::saturate(0.2f + d * 0.1773f);",
      "out[0] = piko::saturate(0.25f + d * 0.1773f);", "code outside"),
-])
-def test_edited_stage_body_exits_5(tmp_path, sub, header, old, new, names):
+]
+
+
+def edited_pipe(tmp_path, sub, header, old, new):
     src = copy_sample(tmp_path, sub)
     text = (src / header).read_text()
     assert old in text
     (src / header).write_text(text.replace(old, new, 1))
-    pipe = src / ("rasterPipe.h" if sub == "raster" else "reyesPipe.h")
-    r = run_cli(pipe, tmp_path / "out")
+    return src / ("rasterPipe.h" if sub == "raster" else "reyesPipe.h")
+
+
+@pytest.mark.parametrize("sub,header,old,new,names", EDITS)
+def test_edited_stage_body_refused_by_library_mode(tmp_path, sub, header, old, new, names):
+    pipe = edited_pipe(tmp_path, sub, header, old, new)
+    r = run_cli(pipe, tmp_path / "out", "-D", "PIKO_STAGE_CODE=LIBRARY")
     assert r.returncode == 5, r.stderr
-    assert names in r.stderr and "does not compile stage code" in r.stderr
+    assert names in r.stderr and "is not a body the HIP kernel library implements" in r.stderr
     assert not (tmp_path / "out").exists(), "nothing is written for a refused pipe"
 
 
-def test_emptied_process_body_exits_5(tmp_path):
+@pytest.mark.parametrize("sub,header,old,new,names", EDITS)
+def test_edited_stage_body_compiled(tmp_path, sub, header, old, new, names):
+    # the default binding compiles bodies the library does not implement
+    pipe = edited_pipe(tmp_path, sub, header, old, new)
+    r = run_cli(pipe, tmp_path / "out", "--no-build")
+    assert r.returncode == 0, r.stderr
+    assert "stage code: compiled; not the kernel library's" in r.stdout and names in r.stdout
+    hip = (tmp_path / "out" / "__pikoStages.hip").read_text()
+    assert ('#include "%s"' % pipe.name) in hip and "#pragma clang force_cuda_host_device begin" in hip
+    assert ("pikox::RasterExec<VertexShaderStage, RasterStage>" if sub == "raster" else
+            "pikox::ReyesExec<SplitStage, DiceStage, ShadeStage>") in hip
+    glue = (tmp_path / "out" / "__pikoCompiledPipe.h").read_text()
+    assert "_stages_create(&piko_hip_pipe_" in glue and "piko_raster_allocate" not in glue
+    kmap = (tmp_path / "out" / "__pikoCompiledPipe.kernels").read_text()
+    assert "k_stage_bins" in kmap and "k_qraster" not in kmap
+
+
+def test_compiled_stage_code_builds_and_exports_its_abi(tmp_path):
+    # hipcc builds the generated source for gfx950; the library exports the
+    # C-ABI the generated host methods declare (nothing is called: no GPU)
+    import ctypes
+    pipe = edited_pipe(tmp_path, *EDITS[0][:4])
+    r = run_cli(pipe, tmp_path / "out")
+    assert r.returncode == 0, r.stderr
+    lib = tmp_path / "out" / "libRasterPipe_stages.so"
+    assert lib.exists()
+    so = ctypes.CDLL(str(lib))
+    for fn in ("input_size", "create", "prepare", "run", "get_color", "get_depth", "profile", "destroy"):
+        assert hasattr(so, "RasterPipe_stages_" + fn), fn
+    # raster_wtri of the pretransformed build: id, 3 positions, 3 colour words
+    assert so.RasterPipe_stages_input_size() == 4 + 36 + 12
+
+
+def test_emptied_process_body(tmp_path):
     src = copy_sample(tmp_path, "raster")
     text = (src / "raster.h").read_text()
     start = text.index("  void process(raster_stri p) {")
     end = text.index("\n  }\n", start) + 4
     (src / "raster.h").write_text(text[:start] + "  void process(raster_stri p) {}\n" + text[end:])
-    r = run_cli(src / "rasterPipe.h", tmp_path / "out")
+    r = run_cli(src / "rasterPipe.h", tmp_path / "out", "-D", "PIKO_STAGE_CODE=LIBRARY")
     assert r.returncode == 5 and "raster.process" in r.stderr
+    r = run_cli(src / "rasterPipe.h", tmp_path / "out2", "--no-build")
+    assert r.returncode == 0 and (tmp_path / "out2" / "__pikoStages.hip").exists()
 
 
 def test_reformatted_body_accepted(tmp_path):
@@ -249,17 +295,58 @@ def test_reformatted_body_accepted(tmp_path):
     text = (src / "raster.h").read_text()
     text = text.replace("    ", "\t").replace(" = ", "   =   ").replace("; ", ";\n /* note */ ")
     (src / "raster.h").write_text(text)
-    assert run_cli(src / "rasterPipe.h", tmp_path / "out").returncode == 0
+    assert run_cli(src / "rasterPipe.h", tmp_path / "out", "-D", "PIKO_STAGE_CODE=LIBRARY").returncode == 0
+    assert not (tmp_path / "out" / "__pikoStages.hip").exists()
 
 
-def test_missing_goraud_refused(tmp_path):
+def test_missing_goraud(tmp_path):
     src = copy_sample(tmp_path, "raster")
     text = (src / "rasterPipe.h").read_text()
     assert "#define GORAUD\n" in text
     (src / "rasterPipe.h").write_text(text.replace("#define GORAUD\n", ""))
     with pytest.raises(BackendError, match="GORAUD"):
-        compile_pipe(str(src / "rasterPipe.h"), str(tmp_path / "out"), [INC], log=None)
-    assert run_cli(src / "rasterPipe.h", tmp_path / "out").returncode == 5
+        compile_pipe(str(src / "rasterPipe.h"), str(tmp_path / "out"), [INC],
+                     defines={"PIKO_STAGE_CODE": "LIBRARY"}, log=None)
+    assert run_cli(src / "rasterPipe.h", tmp_path / "out", "-D", "PIKO_STAGE_CODE=LIBRARY").returncode == 5
+    be = compile_pipe(str(src / "rasterPipe.h"), str(tmp_path / "out2"), [INC], log=None, build=False)
+    assert be.kind == "compiled" and "GORAUD" in be.library_miss
+
+
+def test_lit_sample_is_compiled_stage_code(tmp_path):
+    be = compile_pipe(os.path.join(SAMPLES, "raster_lit", "rasterPipe.h"), str(tmp_path), [INC], log=None,
+                      build=False)
+    assert be.kind == "compiled" and be.shape == "raster" and "GORAUD" in be.library_miss
+
+
+def test_stage_code_setting_and_graph_refusals(tmp_path):
+    r = run_cli(os.path.join(SAMPLES, "raster", "rasterPipe.h"), tmp_path / "out", "-D", "PIKO_STAGE_CODE=FAST")
+    assert r.returncode == 5 and "PIKO_STAGE_CODE" in r.stderr
+    # a three-stage chain without the loop is neither graph pikoc_hip takes
+    (tmp_path / "p.h").write_text("""
+#include "piko/pipe.h"
+struct In : public Primitive { cvec3f vertPos[3]; };
+class A : public Stage<16, 16, 64, In, In> {
+  void assignBin(In p) { this->assignToBin(p, 0); }
+  void schedule(int b) { specifySchedule(LOAD_BALANCE); }
+  void process(In p) { this->emit(p, 0); }
+};
+class B : public Stage<16, 16, 64, In, Pixel> {
+  void assignBin(In p) { this->assignToBin(p, 0); }
+  void schedule(int b) { specifySchedule(LOAD_BALANCE); }
+  void process(In p) {}
+};
+class P : public PikoPipe {
+  A a; A a2; B b;
+ public:
+  P() { pikoConnect(a, a2, 0, 0); pikoConnect(a2, b, 0, 0); }
+  void run(ConstantState &c, MutableState &m, In *in, int n);
+  void allocate(ConstantState &c, MutableState &m, In *in, int n);
+  void prepare(); void run_single(); void destroy();
+};
+""")
+    for mode in ("AUTO", "COMPILED", "LIBRARY"):
+        r = run_cli(tmp_path / "p.h", tmp_path / ("out_" + mode), "-D", "PIKO_STAGE_CODE=" + mode, "--no-build")
+        assert r.returncode == 5 and "stage graph" in r.stderr, (mode, r.stderr)
 
 
 def test_sample_stages_hash_to_library_table():
@@ -304,7 +391,10 @@ def test_cli_exit_codes(tmp_path):
 
 @pytest.mark.parametrize("binary,args", [("raster/raster_sample", []),
                                          ("raster/raster_sample_world", ["--scene", "heightfield"]),
-                                         ("reyes/reyes_sample", [])])
+                                         ("reyes/reyes_sample", []),
+                                         ("raster/raster_sample_compiled", []),
+                                         ("reyes/reyes_sample_compiled", []),
+                                         ("raster_lit/raster_lit_sample", ["--scene", "heightfield"])])
 def test_samples_built_and_fail_loudly_without_gpu(binary, args):
     path = os.path.join(SAMPLES, binary)
     if not os.path.exists(path):

@@ -99,3 +99,21 @@ def test_reyes_stage_bodies_equal_oracle(oracle_mod, tmp_path, dice, spp, disp):
     assert (oz != np.float32(1.0).view(np.int32)).sum() > npix * spp // 10
     assert np.array_equal(z, oz), "sample depth differs at %d samples" % int((z != oz).sum())
     assert np.array_equal(rgba.view(np.int32), orgba.view(np.int32)), "colour differs"
+
+
+def test_lit_sample_bodies_share_the_oracle_coverage_and_depth(oracle_mod, tmp_path):
+    """The per-pixel-lit sample changes only the shading: run on the host,
+    its bodies give the oracle's depth words bit for bit, and a colour at
+    every covered pixel that its own normals decide (a Gouraud colour only
+    where the two agree)."""
+    scene = scenes.heightfield(seed=2, grid=60, W=240, H=136)
+    exe = build(tmp_path, "raster_lit", "raster_lit", {"PIKO_SUBPIXEL_BITS": "8"})
+    raw, out = run(exe, tmp_path, scene, (scene.pos.astype(np.float32), scene.nrm.astype(np.float32)))
+    npix = scene.W * scene.H
+    color = raw[:4 * npix].view(np.uint32).reshape(scene.H, scene.W)
+    depth = raw[4 * npix:].view(np.int32).reshape(scene.H, scene.W)
+    oc, od = oracle_mod.frame(scene, 8, 8)[:2]
+    assert np.array_equal(depth, od.view(np.int32)), "depth differs"
+    covered = oc != 0xff663313
+    assert np.array_equal(color == 0xff663313, ~covered), "coverage differs"
+    assert (color[covered] != oc[covered]).mean() > 0.5, "per-pixel shading should differ from Gouraud"
