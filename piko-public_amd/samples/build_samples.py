@@ -28,9 +28,11 @@ SAMPLES = [
 ]
 
 
-def build_one(sub, header, binary, defines, log=None):
+def build_one(sub, header, binary, defines, log=None, root=HERE):
+    """pikoc_hip on root/sub/header, then the host program root/sub/main.cpp
+    into root/sub/binary."""
     from pikoc_hip import compile_pipe
-    src = os.path.join(HERE, sub)
+    src = os.path.join(root, sub)
     gen = os.path.join(src, "gen_" + binary)
     inc = os.path.join(PKG, "include")
     be = compile_pipe(os.path.join(src, header), gen, [inc], defines, log=log)
@@ -39,7 +41,7 @@ def build_one(sub, header, binary, defines, log=None):
     cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(REPO, "include"), "-I", inc, "-I", gen, "-I", src]
     cmd += ["-D%s=%s" % kv for kv in defines.items()]
     cmd += [os.path.join(src, "main.cpp"), "-L", libdir, "-lpiko_hip", "-lpiko_scene",
-            "-Wl,-rpath,$ORIGIN/../../piko_amd", "-Wl,-rpath-link,/opt/rocm/lib"]
+            "-Wl,-rpath,$ORIGIN/" + os.path.relpath(libdir, src), "-Wl,-rpath-link,/opt/rocm/lib"]
     if be.kind == "compiled":
         cmd += ["-L", gen, "-l%s" % be.prefix, "-Wl,-rpath,$ORIGIN/gen_" + binary]
     cmd += ["-o", out + ".tmp"]

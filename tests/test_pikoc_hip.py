@@ -394,9 +394,11 @@ def test_cli_exit_codes(tmp_path):
                                          ("reyes/reyes_sample", []),
                                          ("raster/raster_sample_compiled", []),
                                          ("reyes/reyes_sample_compiled", []),
-                                         ("raster_lit/raster_lit_sample", ["--scene", "heightfield"])])
+                                         ("raster_lit/raster_lit_sample", ["--scene", "heightfield"]),
+                                         ("edge/edge_pipe", [])])
 def test_samples_built_and_fail_loudly_without_gpu(binary, args):
-    path = os.path.join(SAMPLES, binary)
+    path = os.path.join(SAMPLES, binary) if not binary.startswith("edge/") else \
+        os.path.join(os.path.dirname(os.path.abspath(__file__)), "stage_pipes", binary)
     if not os.path.exists(path):
         pytest.skip("samples not built (run __graft_entry__.build())")
     import torch
