@@ -607,13 +607,32 @@ __global__ void __launch_bounds__(kQThreads) k_qmerge(QArgs qa, FrameArgs fa) {
   const FrameArgs &ja = MULTI ? qa.jobs[job_search(qa.jobs, qa.njobs, (uint32_t)gt, true)] : fa;
   const QTile tl = q_tile(ja, gt);
   const int T = 1 << ja.lgT;
-  for (int q = t; q < T * T; q += kQThreads) {
-    long long m = (long long)kVisSentinel;
-    for (uint32_t j = 0; j < it.nparts; ++j) {
-      const long long o = (long long)qa.scratch[(size_t)(it.slot + j) * (size_t)(QT * QT) + q];
-      m = o < m ? o : m;
+  // the minimum over the parts' words: a thread's kPix pixels of two parts at
+  // a time, all their loads issued before any is used (a loop over one part
+  // word at a time waited for each load in turn; four parts at a time were
+  // no faster)
+  constexpr int kPix = 4;
+  const unsigned long long *slots = qa.scratch + (size_t)it.slot * (size_t)(QT * QT);
+  for (int q0 = 0; q0 < T * T; q0 += kPix * kQThreads) {
+    long long m[kPix], a[kPix], b[kPix];
+#pragma unroll
+    for (int k = 0; k < kPix; ++k) m[k] = (long long)kVisSentinel;
+    for (uint32_t j = 0; j < it.nparts; j += 2) {
+      const bool two = j + 1 < it.nparts;
+#pragma unroll
+      for (int k = 0; k < kPix; ++k) {
+        const int q = min(q0 + k * kQThreads + t, T * T - 1);
+        a[k] = (long long)slots[(size_t)j * (size_t)(QT * QT) + q];
+        b[k] = (long long)slots[(size_t)(two ? j + 1 : j) * (size_t)(QT * QT) + q];
+      }
+#pragma unroll
+      for (int k = 0; k < kPix; ++k) m[k] = min(m[k], min(a[k], b[k]));
     }
-    vis[q] = (unsigned long long)m;
+#pragma unroll
+    for (int k = 0; k < kPix; ++k) {
+      const int q = q0 + k * kQThreads + t;
+      if (q < T * T) vis[q] = (unsigned long long)m[k];
+    }
   }
   if (t == 0) {
     qa.tile_need[gt] = qa.tile_fill[gt];
