@@ -202,11 +202,13 @@ def bench_batch(log, reps=5):
         p.destroy()
     pos, attr, parts = piko_amd.concat_scenes(scs)
 
-    def run_batch(mode, tiles, **kw):
+    def run_batch(mode, tiles, covers=None, **kw):
         bp = piko_amd.BatchPipe(pos, attr, "pretransformed", max_jobs=len(scs), max_tris=ntris, max_pixels=npix,
                                 raster_mode=mode, **kw)
+        covers = covers or [0] * len(tiles)
         try:
-            bp.submit([(n, base, T, piko_amd.ConstantState.for_scene(s)) for (n, base, s), T in zip(parts, tiles)])
+            bp.submit([(n, base, T, piko_amd.ConstantState.for_scene(s), 0, 0.0, cv)
+                       for (n, base, s), T, cv in zip(parts, tiles, covers)])
             bp.wait()
             bp.run_frames(1)
             t0 = time.perf_counter()
@@ -221,12 +223,13 @@ def bench_batch(log, reps=5):
     tiles, btable = autotune.tune_batch(scs, pos, attr, parts)
     t_batch, k_tiled, exact = run_batch(piko_amd.RASTER_TILED, tiles)
     qcfg, _ = autotune.tune_queue_batch(scs, pos, attr, parts)
-    t_queue, k_queue, exact_q = run_batch(piko_amd.RASTER_QUEUE, qcfg["tiles"],
+    t_queue, k_queue, exact_q = run_batch(piko_amd.RASTER_QUEUE, qcfg["tiles"], qcfg["covers"],
                                           part_pairs=qcfg["part_pairs"], waves_per_cu=qcfg["waves_per_cu"])
     qcfg["tiles"] = {str(T): qcfg["tiles"].count(T) for T in sorted(set(qcfg["tiles"]))}
+    qcfg["covers"] = {str(c): qcfg["covers"].count(c) for c in sorted(set(qcfg["covers"]))}
     out = {"workload": "256 uniform-soup scenes, seeds 1000-1255, N log-uniform [1K,500K], "
                        "pixels log-uniform [256^2,2048^2], aspect {1,4:3,16:9}, autotuned (plain pipes: raster "
-                       "path and bin size per class; batch: tile size, waves per CU and part size, timed in "
+                       "path and bin size per class; batch: tile size, waves per CU, part size and coverage mode, timed in "
                        "the batch)",
            "form": "queue path: bin launch + one persistent raster launch over every job's tile parts "
                    "(+ merge of split tiles), frames of one submitted job set",

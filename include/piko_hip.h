@@ -102,6 +102,19 @@ enum {
   PIKO_RASTER_QUEUE = 4
 };
 
+/* QUEUE: how a tile part covers a triangle whose edge functions fit int32.
+ *   WALK:  each lane steps its triangle's bbox columns (exact integer edge
+ *          walks, no division per column) and the wave's covered pixels are
+ *          spread evenly over its lanes, 64 to a window; fastest where
+ *          triangles are thin slivers (the height field, whose bboxes are
+ *          about a sixth covered).
+ *   SPANS: 8-row column spans test every bbox pixel, except tall slivers,
+ *          which are walked; fastest where triangles are compact (the
+ *          batched soups, whose bboxes are about half covered).
+ *   AUTO:  WALK for a plain pipe, SPANS for a batch.  Both give the same
+ *          image. */
+enum { PIKO_COVER_AUTO = 0, PIKO_COVER_WALK = 1, PIKO_COVER_SPANS = 2 };
+
 typedef struct {
   int bin_size;           /* raster bin edge T in pixels: 8, 16, 32 or 64 */
   int subpixel_bits;      /* S in [1, 8] (reference: 4) */
@@ -109,9 +122,10 @@ typedef struct {
   long long pair_capacity;/* (triangle, bin) queue capacity; 0 = automatic,
                              grows on demand between frames */
   int waves_per_cu;       /* persistent raster waves per CU; 0 = default (20;
-                             QUEUE: 16, four 256-thread workgroups) */
+                             QUEUE: 20, five 256-thread workgroups) */
   int raster_mode;        /* PIKO_RASTER_*; 0 = AUTO */
   int part_pairs;         /* QUEUE: queue entries per part item; 0 = 1024 */
+  int queue_cover;        /* QUEUE: PIKO_COVER_*; 0 = AUTO (WALK) */
 } piko_raster_options;
 
 typedef struct piko_raster_pipe piko_raster_pipe;
@@ -210,6 +224,7 @@ typedef struct {
   int waves_per_cu;        /* raster waves per CU for this job's bin-size class; 0 = default */
   float queue_depth;       /* expected (triangle, bin) queue entries per triangle; 0 = 2 */
   piko_constant_state cs;
+  int queue_cover;         /* QUEUE: PIKO_COVER_* for this job's scene class; 0 = the batch's */
 } piko_raster_job;
 
 typedef struct {
@@ -222,6 +237,7 @@ typedef struct {
   int raster_mode;         /* PIKO_RASTER_* for every batch; 0 = AUTO */
   int part_pairs;          /* QUEUE: queue entries per part item; 0 = 2048 */
   int waves_per_cu;        /* QUEUE: persistent waves per CU; 0 = 24 */
+  int queue_cover;         /* QUEUE: PIKO_COVER_* of jobs that name none; 0 = AUTO (SPANS) */
 } piko_batch_options;
 
 /* Creates a batch pipe over `input_tris` resident triangles (pos/attr as for

@@ -1682,6 +1682,7 @@ struct piko_raster_pipe {
   int q_QT = 32;                        // LDS tile edge of the launch (largest job tile)
   int q_part_pairs = 1024;
   int q_wpc = 16;                       // persistent waves per CU
+  uint32_t q_walk = 1;                  // PIKO_COVER_WALK (1) or SPANS (0) for jobs naming none
   FrameArgs fa_q{};                     // plain pipe job in queue tile numbering
   long long q_bytes = 0;
   long long large_tiles_hint = -1;  // 8x8 tiles of large triangles last seen (-1: unknown)
@@ -2448,6 +2449,7 @@ int validate_opts(const piko_constant_state *cs, int count, const piko_raster_op
     return PIKO_ERR_INVALID;
   if (opt->raster_mode < PIKO_RASTER_AUTO || opt->raster_mode > PIKO_RASTER_QUEUE) return PIKO_ERR_INVALID;
   if (opt->part_pairs < 0 || opt->part_pairs > (1 << 20) || opt->waves_per_cu < 0) return PIKO_ERR_INVALID;
+  if (opt->queue_cover < PIKO_COVER_AUTO || opt->queue_cover > PIKO_COVER_SPANS) return PIKO_ERR_INVALID;
   return PIKO_OK;
 }
 
@@ -2561,7 +2563,9 @@ int plain_pipe(piko_raster_pipe **out, const piko_constant_state *cs, const floa
   if (rc == PIKO_OK && p->mode == PIKO_RASTER_QUEUE) {
     p->fa_q = make_args(cs, opt->subpixel_bits, queue_T(p->T, p->W, p->H));
     p->q_part_pairs = opt->part_pairs > 0 ? opt->part_pairs : 1024;
-    p->q_wpc = opt->waves_per_cu > 0 ? opt->waves_per_cu : 16;
+    p->q_wpc = opt->waves_per_cu > 0 ? opt->waves_per_cu : 20;
+    p->q_walk = opt->queue_cover == PIKO_COVER_SPANS ? 0u : 1u;
+    p->fa_q.qwalk = p->q_walk;
     rc = q_setup(p, &p->fa_q, 1, count);
   }
   if (rc == PIKO_OK && p->mode == PIKO_RASTER_TILED) {
@@ -2690,7 +2694,10 @@ int piko_raster_prepare(piko_raster_pipe *p, const piko_constant_state *cs) {
   if (cs->screenSizeX != p->W || cs->screenSizeY != p->H) return PIKO_ERR_INVALID;
   p->fa = make_args(cs, p->S, p->T);
   p->jobs.assign(1, p->fa);
-  if (p->mode == PIKO_RASTER_QUEUE) p->fa_q = make_args(cs, p->S, 1 << p->fa_q.lgT);
+  if (p->mode == PIKO_RASTER_QUEUE) {
+    p->fa_q = make_args(cs, p->S, 1 << p->fa_q.lgT);
+    p->fa_q.qwalk = p->q_walk;
+  }
   if (p->mode == PIKO_RASTER_TILED) {
     FrameArgs ft = make_args(cs, p->S, p->tclass.T);
     ft.chunk_base = p->fa_t.chunk_base;
@@ -2891,6 +2898,7 @@ int piko_batch_create(piko_raster_pipe **out, const float *pos, const void *attr
   if (opt->max_jobs <= 0 || opt->max_tris < 0 || opt->max_pixels <= 0) return PIKO_ERR_INVALID;
   if (opt->raster_mode < PIKO_RASTER_AUTO || opt->raster_mode > PIKO_RASTER_QUEUE) return PIKO_ERR_INVALID;
   if (opt->part_pairs < 0 || opt->part_pairs > (1 << 20) || opt->waves_per_cu < 0) return PIKO_ERR_INVALID;
+  if (opt->queue_cover < PIKO_COVER_AUTO || opt->queue_cover > PIKO_COVER_SPANS) return PIKO_ERR_INVALID;
   if (input_tris > 0 && (!pos || !attr)) return PIKO_ERR_INVALID;
   piko_raster_pipe *p = new (std::nothrow) piko_raster_pipe();
   if (!p) return PIKO_ERR_OOM;
@@ -2898,9 +2906,10 @@ int piko_batch_create(piko_raster_pipe **out, const float *pos, const void *attr
   p->input_tris = input_tris;
   p->cap_jobs = opt->max_jobs;
   // a batch's queue path: longer parts and a wider persistent grid (the
-  // 256-scene batch's tuned setting; a single 1080p frame tunes to 1024 / 16)
+  // 256-scene batch's tuned setting; a single 1080p frame tunes to 1024 / 20)
   p->q_part_pairs = opt->part_pairs > 0 ? opt->part_pairs : 2048;
   p->q_wpc = opt->waves_per_cu > 0 ? opt->waves_per_cu : 24;
+  p->q_walk = opt->queue_cover == PIKO_COVER_WALK ? 1u : 0u;  // AUTO: spans (the batched soups)
   const int bins = (int)std::min<long long>(opt->max_pixels / 64 + 1024, 1ll << 30);
   int rc = create_common(p, opt->input_kind, opt->subpixel_bits, opt->max_tris, opt->max_pixels, bins,
                          opt->pair_capacity, opt->raster_mode);
@@ -2964,6 +2973,7 @@ int piko_batch_submit(piko_raster_pipe *p, const piko_raster_job *jobs, int njob
     const piko_raster_job &jb = jobs[j];
     if (jb.count < 0 || jb.src_base < 0 || jb.src_base + jb.count > p->input_tris) return PIKO_ERR_INVALID;
     if (!valid_bin(jb.bin_size) || !valid_screen(jb.cs.screenSizeX, jb.cs.screenSizeY)) return PIKO_ERR_INVALID;
+    if (jb.queue_cover < PIKO_COVER_AUTO || jb.queue_cover > PIKO_COVER_SPANS) return PIKO_ERR_INVALID;
     const int T = job_T(p, jb);
     tris += jb.count;
     pix += (long long)jb.cs.screenSizeX * jb.cs.screenSizeY;
@@ -3007,6 +3017,7 @@ int piko_batch_submit(piko_raster_pipe *p, const piko_raster_job *jobs, int njob
       fa.src_base = (uint32_t)jobs[j].src_base;
       fa.bin_base = bb;
       fa.fb_base = fbb;
+      fa.qwalk = jobs[j].queue_cover == PIKO_COVER_AUTO ? p->q_walk : (jobs[j].queue_cover == PIKO_COVER_WALK ? 1u : 0u);
       vbase += (uint32_t)jobs[j].count;
       bb += fa.nbins;
       fbb += (unsigned long long)fa.W * fa.H;

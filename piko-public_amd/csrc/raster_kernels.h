@@ -27,14 +27,16 @@ constexpr unsigned long long kVisSentinel =
 //   bin_base first bin of the job in the batch's bin numbering
 //   fb_base  first pixel of the job's framebuffer in the batch's buffers
 // The tiled path also reads where the job's chunks and their offset table
-// start (chunk_base, tab_base) and how many chunks it has (nchunks).
+// start (chunk_base, tab_base) and how many chunks it has (nchunks); the
+// queue path whether it walks the columns of every int32-path triangle
+// (qwalk = 1) or covers compact ones by spans (qwalk = 0).
 struct FrameArgs {
   int W, H, S, lgT, nBx, nBy;
   float halfW, halfH;
   float vp[16];
   float light[3], matcol[3], ambient[3];
   uint32_t vbase, src_base, bin_base, nbins;
-  uint32_t chunk_base, nchunks, tab_base, pad_;
+  uint32_t chunk_base, nchunks, tab_base, qwalk;
   unsigned long long fb_base;
 };
 

@@ -83,26 +83,59 @@ struct QArgs {
   int njobs, nchunks, nitems, part_pairs, nsplits;
 };
 
+// Staged triangles of one wave of the queue raster (structure of arrays, so
+// lanes reading neighbouring triangles hit neighbouring LDS banks).
+//   fa, fb   what a fragment needs: the row steps of edge functions 1 and 2,
+//            1/barydenom, z2; z0 - z2, z1 - z2 and the key's low word
+//            0xFFFFFFFE - triangle (raster.pikostage:173-174, 269-272)
+//   xy01, xr the fixed-point vertices and the bbox clipped to the tile
+//            (x2, y2, x | y << 16, w | h << 16 | int32 edges << 31), read by
+//            the span path only, which runs while the ring is empty (so the
+//            two share their memory)
+//   pre      exclusive prefix of the span counts (span path)
+//   rec      the column ring: per nonempty bbox column whose fragments are not
+//            all drawn, its first fragment's number in the wave's running
+//            count, slot | column << 6 | first covered row << 12 (tile-local),
+//            and edge functions 1 and 2 at that pixel
+//   mark     the owner marks of a window of 64 items
+constexpr uint32_t kQRing = 128;
+struct QStage {
+  uint4 fa[64];
+  uint4 fb[64];
+  union {
+    struct {
+      int4 xy01[64];
+      int4 xr[64];
+    } sp;
+    uint4 rec[kQRing];
+  };
+  uint32_t pre[64];
+  uint32_t mark[64];
+};
+
 // Wave-level span rasteriser of the queue path: span k (a column segment of
 // kTileSpan pixels of staged triangle o's bbox clipped to the tile) into the
-// tile's visibility words, row stride T.
-__device__ __forceinline__ void q_span(const TileStage &s, int o, uint32_t k, int S, int xlo, int ylo, int T,
+// tile's visibility words, row stride T.  Every bbox pixel of the span is
+// tested.  Used for the triangles the column walk does not take (int64 edge
+// functions, or an edge wider than the 24-bit products allow).
+__device__ __forceinline__ void q_span(const QStage &s, int o, uint32_t k, int S, int xlo, int ylo, int T,
                                        unsigned long long *vis) {
-  const int4 a = s.xy01[o];
-  const int4 b = s.xy2z[o];
-  const uint4 m = s.misc[o];
-  const uint32_t w = m.w & 0xffffu, h = (m.w >> 16) & 0x7fffu;
+  const int4 a = s.sp.xy01[o];
+  const int4 b = s.sp.xr[o];
+  const uint4 fa = s.fa[o];
+  const uint4 fb = s.fb[o];
+  const uint32_t w = (uint32_t)b.w & 0xffffu, h = ((uint32_t)b.w >> 16) & 0x7fffu;
   // band and column of span k (the quotient truncates exactly: see k_vs_direct)
   const uint32_t q = (uint32_t)(((float)k + 0.5f) * __builtin_amdgcn_rcpf((float)w));
   const uint32_t sx = k - q * w, sy = q * kTileSpan;
-  const int px = (int)(m.z & 0xffffu) + (int)sx, py0 = (int)(m.z >> 16) + (int)sy;
+  const int px = (int)((uint32_t)b.z & 0xffffu) + (int)sx, py0 = (int)((uint32_t)b.z >> 16) + (int)sy;
   const uint32_t nv = min((uint32_t)kTileSpan, h - sy);
-  const float rcp = __uint_as_float(m.y);
-  const float z2 = __int_as_float(b.z), z0mz2 = __int_as_float(b.w), z1mz2 = __uint_as_float(m.x);
-  const unsigned long long low = (unsigned long long)(0xFFFFFFFEu - s.tri[o]);
+  const float rcp = __uint_as_float(fa.z);
+  const float z2 = __uint_as_float(fa.w), z0mz2 = __uint_as_float(fb.x), z1mz2 = __uint_as_float(fb.y);
+  const unsigned long long low = (unsigned long long)fb.z;
   unsigned long long *col = vis + (py0 - ylo) * T + (px - xlo);
   const int xy[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
-  if (m.w >> 31) {
+  if ((uint32_t)b.w >> 31) {
     // int32-path triangle: edge functions at the span's pixel centres in
     // wrapping 32-bit arithmetic equal the exact values (all fit in int32)
     const uint32_t half = 1u << (S - 1);
@@ -184,39 +217,48 @@ __device__ __forceinline__ int wave_owner(uint32_t start, uint32_t len, uint32_t
   return wave_max_scan_dpp((int)mark[lane]);
 }
 
-// Rows j in [0, h) of one bbox column where the edge function E + j*D is
-// >= 0 (exact int32 values: an int32-path triangle inside its bbox), as
-// [lo, hi] (empty: lo > hi).  The float quotient locates the boundary to
-// within one row (its relative error is a few 2^-24 and the boundary is
-// only looked for inside [-2, h + 2]); one exact test on each side fixes it.
-__device__ __forceinline__ void q_edge_rows(int E, int D, int h, int &lo, int &hi) {
-  if (D > 0) {  // increasing: j >= ceil(-E / D)
-    float q = (float)(-E) * __builtin_amdgcn_rcpf((float)D);
-    q = fminf(fmaxf(q, -1.0f), (float)(h + 1));
-    int j = min(max((int)ceilf(q), 0), h);
-    if (j < h && (int)((uint32_t)E + (uint32_t)j * (uint32_t)D) < 0) ++j;
-    if (j > 0 && (int)((uint32_t)E + (uint32_t)(j - 1) * (uint32_t)D) >= 0) --j;
-    lo = max(lo, j);
-  } else if (D < 0) {  // decreasing: j <= floor(E / -D)
-    float q = (float)E * __builtin_amdgcn_rcpf((float)(-D));
-    q = fminf(fmaxf(q, -2.0f), (float)h);
-    int j = min(max((int)floorf(q), -1), h - 1);
-    if (j >= 0 && (int)((uint32_t)E + (uint32_t)j * (uint32_t)D) < 0) --j;
-    if (j < h - 1 && (int)((uint32_t)E + (uint32_t)(j + 1) * (uint32_t)D) >= 0) ++j;
-    hi = min(hi, j);
-  } else if (E < 0) {
-    lo = h;
+// floor(n / d) and its remainder n - d * floor(n / d) in [0, d), exact, for
+// 0 < d < 2^30 and |n| < 2^31.  The float quotient (relative error at most
+// 2^-22: the conversion, the reciprocal and the product) errs by at most
+// |n / d| * 2^-22 + 1, so the remainder of the estimate lies within
+// |n| * 2^-22 + 2d < 2^31 of zero and wrapping 32-bit arithmetic gives it
+// exactly; each loop step moves it by d.  With d >= 2^8 (8 subpixel bits) the
+// estimate is off by at most 2.
+__device__ __forceinline__ int q_floordiv(int n, int d, float rd, int &rem) {
+  int q = (int)floorf((float)n * rd);
+  int r = (int)((uint32_t)n - (uint32_t)q * (uint32_t)d);
+  while (r < 0) {
+    --q;
+    r += d;
   }
+  while (r >= d) {
+    ++q;
+    r -= d;
+  }
+  rem = r;
+  return q;
 }
 
-// One wave's bbox columns of int32-path triangles: per column, the first
-// covered row's pixel and edge values and the exclusive prefix of the
-// columns' covered-pixel counts.
-struct QColumns {
-  uint32_t pre[64];   // exclusive prefix of the columns' covered-pixel counts
-  uint32_t info[64];  // staged triangle | px << 6 | first covered row << 18
-  int e1[64], e2[64];
-  uint32_t mark[64];  // wave_owner's marks
+// One edge of a column walk.  Along a bbox column the edge function is
+// E(c) + j * D (row j, column c, E(c + 1) = E(c) + A).  With F = floor(E(c) /
+// |D|) the rows where it is >= 0 are j >= -F (D > 0) or j <= F (D < 0).  F and
+// its remainder R step from column to column by A = Q * |D| + Rr: F += Q, R +=
+// Rr, and a carry when R reaches |D| (Bresenham's stepping, all integer).
+struct QEdgeWalk {
+  int F, R, Q, Rr, Dabs;
+  __device__ __forceinline__ void init(int E, int A, int D) {
+    Dabs = D < 0 ? -D : D;
+    const float rd = __builtin_amdgcn_rcpf((float)Dabs);
+    F = q_floordiv(E, Dabs, rd, R);
+    Q = q_floordiv(A, Dabs, rd, Rr);
+  }
+  __device__ __forceinline__ void step() {
+    R += Rr;
+    F += Q;
+    const bool c = R >= Dabs;
+    R = c ? R - Dabs : R;
+    F += c ? 1 : 0;
+  }
 };
 
 // Tiles of a kept triangle's clipped bbox that pass the trivial-reject test
@@ -378,8 +420,7 @@ __global__ void __launch_bounds__(kQThreads) k_qbin(QArgs qa, FrameArgs fa) {
 template <int QT>
 struct QRasterShared {
   unsigned long long vis[QT * QT];
-  TileStage st[kQThreads / 64];
-  QColumns cols[kQThreads / 64];
+  QStage st[kQThreads / 64];
   float u8[256];
   uint32_t item;
 };
@@ -431,7 +472,7 @@ __global__ void __launch_bounds__(kQThreads) k_qraster(QArgs qa, FrameArgs fa) {
     const uint32_t beg = min(nq, pbeg);
     const uint32_t end = min(nq, pend);
     __syncthreads();  // visibility words initialised
-    TileStage &s = sm.st[t >> 6];
+    QStage &s = sm.st[t >> 6];
     {
       auto entry = [&](uint32_t j) -> uint32_t {
         const uint32_t x = *(j < end ? list + j : qa.pairs);
@@ -444,139 +485,227 @@ __global__ void __launch_bounds__(kQThreads) k_qraster(QArgs qa, FrameArgs fa) {
 #pragma unroll
         for (int q = 0; q < 3; ++q) out[q] = g[q];
       };
+      // The wave's column ring: records [head, tail) are live, fragments
+      // [0, ftot) appended in record order, [0, fcons) drawn.  Fragments are
+      // drawn 64 at a time, so every window but a batch's last is full.
+      uint32_t head = 0, tail = 0, ftot = 0, fcons = 0;
+      // Draws fragments [fcons, fcons + 64) (those below ftot).  The record of
+      // each is the last one starting at or before it: every record starting
+      // inside the window marks its first fragment, the record holding
+      // fragment fcons - 1 (or the first record) marks position 0 beforehand,
+      // and a max-scan carries each mark up to the next.  Live records number
+      // at most 64 before an append of at most 64, so they fit the ring and
+      // two looks per lane cover them.
+      auto window = [&]() {
+        s.mark[lane] = lane == 0 ? head : ~0u;
+        wave_fence();
+#pragma unroll
+        for (uint32_t q = 0; q < kQRing; q += 64) {
+          const uint32_t idx = head + q + (uint32_t)lane;
+          if (idx < tail) {
+            const uint32_t d = s.rec[idx & (kQRing - 1)].x - fcons;
+            if (d < 64u) s.mark[d] = idx;
+          }
+        }
+        wave_fence();
+        const int o = wave_max_scan_dpp((int)s.mark[lane]);
+        const uint32_t f = fcons + (uint32_t)lane;
+        if (f < ftot) {
+          const uint4 rc = s.rec[(uint32_t)o & (kQRing - 1)];
+          const uint32_t r = f - rc.x;
+          const uint32_t slot = rc.y & 63u;
+          const uint4 a = s.fa[slot];
+          const uint4 b = s.fb[slot];
+          // edge functions 1 and 2 down the column (exact: |row step| < 2^23)
+          const int e1 = (int)rc.z + __mul24((int)r, (int)a.x);
+          const int e2 = (int)rc.w + __mul24((int)r, (int)a.y);
+          const float rcp = __uint_as_float(a.z);
+          const float alpha = (float)e1 * rcp, beta = (float)e2 * rcp;  // raster.pikostage:269-270
+          const float z = interp(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(a.w), alpha,
+                                 beta);  // raster.pikostage:272
+          const uint32_t lx = (rc.y >> 6) & 63u, ly = ((rc.y >> 12) & 63u) + r;
+          atomicMin((long long *)(vis + (ly << ja.lgT) + lx),
+                    (long long)(((unsigned long long)__float_as_uint(z) << 32) | (unsigned long long)b.z));
+        }
+        head = (uint32_t)__builtin_amdgcn_readlane(o, 63);
+        wave_fence();  // the marks are read before the next window resets them
+      };
       // software pipeline: the setup records of the next batch and the
       // entries of the one after are in flight while a batch is rasterised
       uint32_t v_cur = beg + (uint32_t)t < end ? r_cur : ~0u;
       int4 gc[3];
       setup(v_cur, gc);
       uint32_t v_nxt = beg + (uint32_t)(kQThreads + t) < end ? r_nxt : ~0u;
+      const uint32_t half = 1u << (S - 1);
       for (uint32_t b0 = beg; b0 < end; b0 += kQThreads) {
         int4 gn[3];
         setup(v_nxt, gn);
         const uint32_t v_nn = entry(b0 + 2u * kQThreads + (uint32_t)t);
-        // each staged triangle goes by bbox columns or by 8-row spans
-        uint32_t ucol = 0, uspan = 0;
+        // Each staged triangle is drawn by a column walk (int32 edge
+        // functions and |row steps| of edges 1 and 2 below 2^23, so 24-bit
+        // products are exact) or by 8-row spans (the others).
+        uint32_t uspan = 0;
+        int walk = 0;  // bbox columns this lane's triangle walks
+        QEdgeWalk wa = {0, 0, 0, 0, 1}, wb = {0, 0, 0, 0, 1}, wc = {0, 0, 0, 0, 1};
+        bool bl = false, bh = false;  // the middle edge bounds the rows from below / above
+        int E1 = 0, E2 = 0, A1 = 0, A2 = 0, D1 = 0, D2 = 0, hm1 = -1, lxc = 0, ly0 = 0;
+        const int p[6] = {gc[0].x, gc[0].y, gc[0].z, gc[0].w, gc[1].x, gc[1].y};
+        int x0 = 0, y0 = 0, w = 0, h = 0;  // the bbox clipped to the tile (w = 0: nothing staged)
+        bool small = false, colw = false;
+        uint32_t spans = 0;
         if (v_cur != ~0u) {
-          const int p[6] = {gc[0].x, gc[0].y, gc[0].z, gc[0].w, gc[1].x, gc[1].y};
-          int x0, y0, x1, y1;
+          int x1, y1;
           clip_rect(p, S, ja.W, ja.H, x0, y0, x1, y1);
           x0 = max(x0, tl.xlo);
           y0 = max(y0, tl.ylo);
           x1 = min(x1, tl.xhi);
           y1 = min(y1, tl.yhi);
-          const int w = max(0, x1 - x0), h = max(0, y1 - y0);
+          w = max(0, x1 - x0);
+          h = max(0, y1 - y0);
           if (w > 0 && h > 0) {
-            const bool small = small_bbox(p, S);
-            const uint32_t spans = (uint32_t)w * (uint32_t)((h + kTileSpan - 1) / kTileSpan);
-            // int32-path triangles take the cheaper of the two: 8-row spans
-            // test every bbox pixel, columns find each column's covered rows
-            // and then touch only covered pixels.  Measured per unit, a column
-            // costs about as much as a span and a covered pixel of the column
-            // path about 0.45 of one (cost weights 2 : 1 : 0.45 mis-sent the
-            // small soup triangles of the 256-scene batch to columns: 4.96 ms
-            // against 4.31 ms per batch).  Tall slivers, whose bbox is mostly
-            // empty, go by columns; compact triangles by spans.  The area in
-            // pixels is |2 x signed area| / 2^(2S+1) of the fixed-point edges.
-            bool by_col = false;
-            if (small) {
-              const float area = fabsf(__int_as_float(gc[1].z)) > 0.0f
-                                     ? 0.5f / (fabsf(__int_as_float(gc[1].z)) * (float)(1 << (2 * S)))
-                                     : 0.0f;
-              const float area_clip = fminf(area, (float)(w * h));
-              by_col = (float)spans > (float)w + 0.45f * area_clip;
+            small = small_bbox(p, S);
+            const int lim = 1 << (23 - S);
+            spans = (uint32_t)w * (uint32_t)((h + kTileSpan - 1) / kTileSpan);
+            // Which int32-path triangles are walked (ja.qwalk, PIKO_COVER_*):
+            // all of them, or (spans) only tall slivers.  Spans test every
+            // bbox pixel; the walk touches only covered pixels but sets up
+            // each triangle's edge walks and hands its pixels out 64 at a
+            // time.  With spans, a triangle is walked when its spans outnumber
+            // its columns plus 0.45 per covered pixel, the measured exchange
+            // rate.  The area in pixels is |2 x signed area| / 2^(2S+1) of the
+            // fixed-point edges.
+            colw = small && abs(p[4] - p[2]) < lim && abs(p[0] - p[4]) < lim;
+            if (colw && !ja.qwalk) {
+              const float rd = fabsf(__int_as_float(gc[1].z));
+              const float area = rd > 0.0f ? 0.5f / (rd * (float)(1 << (2 * S))) : 0.0f;
+              colw = (float)spans > (float)w + 0.45f * fminf(area, (float)(w * h));
             }
-            if (by_col) ucol = (uint32_t)w;
-            else uspan = spans;
-            s.xy01[lane] = make_int4(p[0], p[1], p[2], p[3]);
-            s.xy2z[lane] = make_int4(p[4], p[5], gc[1].w, gc[2].x);
-            s.misc[lane] = make_uint4((uint32_t)gc[2].y, (uint32_t)gc[1].z, (uint32_t)x0 | ((uint32_t)y0 << 16),
-                                      (uint32_t)w | ((uint32_t)h << 16) | (small ? 0x80000000u : 0u));
-            s.tri[lane] = v_cur;
+            s.fa[lane] = make_uint4((uint32_t)(p[4] - p[2]) << S, (uint32_t)(p[0] - p[4]) << S, (uint32_t)gc[1].z,
+                                    (uint32_t)gc[1].w);
+            s.fb[lane] = make_uint4((uint32_t)gc[2].x, (uint32_t)gc[2].y, 0xFFFFFFFEu - v_cur, 0u);
+          } else {
+            w = 0;
+          }
+        }
+        if (w > 0) {
+          if (colw) {
+            // edge k at the centre of bbox pixel (c, j): E[k] + c*A[k] + j*D[k]
+            // (raster.pikostage:176-180, 223-241)
+            const uint32_t cx0 = ((uint32_t)x0 << S) + half, cy0 = ((uint32_t)y0 << S) + half;
+            int E[3], A[3], D[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              const uint32_t xa = (uint32_t)p[2 * k], ya = (uint32_t)p[2 * k + 1];
+              const uint32_t xb = (uint32_t)p[(2 * k + 2) % 6], yb = (uint32_t)p[(2 * k + 3) % 6];
+              E[k] = (int)((cy0 - ya) * (xb - xa) - (cx0 - xa) * (yb - ya));
+              A[k] = (int)((ya - yb) << S);
+              D[k] = (int)((xb - xa) << S);
+            }
+            // a vertical edge (D = 0) limits the columns: E + c*A >= 0
+            int c0 = 0, c1 = w - 1;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              if (D[k] != 0) continue;
+              if (A[k] == 0) {
+                if (E[k] < 0) c1 = -1;
+                continue;
+              }
+              const int aa = A[k] < 0 ? -A[k] : A[k];
+              int rr;
+              const int fq = q_floordiv(E[k], aa, __builtin_amdgcn_rcpf((float)aa), rr);
+              if (A[k] > 0) c0 = max(c0, -fq);  // c >= ceil(-E / A)
+              else c1 = min(c1, fq);            // c <= floor(E / -A)
+            }
+            if (c0 <= c1) {
+#pragma unroll
+              for (int k = 0; k < 3; ++k) E[k] = (int)((uint32_t)E[k] + (uint32_t)c0 * (uint32_t)A[k]);
+              // a front-facing triangle has an edge with D > 0 (a lower row
+              // bound) and one with D < 0 (an upper one); the third is either
+              // or vertical
+              const int iA = D[0] > 0 ? 0 : (D[1] > 0 ? 1 : 2);
+              const int iC = D[0] < 0 ? 0 : (D[1] < 0 ? 1 : 2);
+              const int iB = 3 - iA - iC;
+              auto pick = [](const int *v, int i) { return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]); };
+              wa.init(pick(E, iA), pick(A, iA), pick(D, iA));
+              wc.init(pick(E, iC), pick(A, iC), pick(D, iC));
+              const int DB = pick(D, iB);
+              bl = DB > 0;
+              bh = DB < 0;
+              if (DB != 0) wb.init(pick(E, iB), pick(A, iB), DB);
+              E1 = E[1];
+              E2 = E[2];
+              A1 = A[1];
+              A2 = A[2];
+              D1 = D[1];
+              D2 = D[2];
+              hm1 = h - 1;
+              lxc = x0 + c0 - tl.xlo;
+              ly0 = y0 - tl.ylo;
+              walk = c1 - c0 + 1;
+            }
+          } else {
+            uspan = spans;
+            s.sp.xy01[lane] = make_int4(p[0], p[1], p[2], p[3]);
+            s.sp.xr[lane] = make_int4(p[4], p[5], (int)((uint32_t)x0 | ((uint32_t)y0 << 16)),
+                                   (int)((uint32_t)w | ((uint32_t)h << 16) | (small ? 0x80000000u : 0u)));
           }
         }
         v_cur = v_nxt;
 #pragma unroll
         for (int q = 0; q < 3; ++q) gc[q] = gn[q];
         v_nxt = v_nn;
-        // 8-row spans: every bbox pixel tested (int64-path triangles always)
+        wave_fence();  // staged
+        // 8-row spans: every bbox pixel tested
         const uint32_t sinc = wave_scan_dpp(uspan);
         const uint32_t stot = wave_last(sinc);
-        QColumns &cl = sm.cols[t >> 6];
         if (stot) {
           s.pre[lane] = sinc - uspan;
           for (uint32_t c0 = 0; c0 < stot; c0 += 64) {
             const uint32_t cidx = c0 + (uint32_t)lane;
-            const int o = wave_owner(sinc - uspan, uspan, c0, cl.mark);  // fences s.pre too
+            const int o = wave_owner(sinc - uspan, uspan, c0, s.mark);  // fences s.pre too
             if (cidx < stot) q_span(s, o, cidx - s.pre[o], S, tl.xlo, tl.ylo, T, vis);
           }
           wave_fence();
         }
-        // columns (int32-path triangles): each lane takes a bbox column and
-        // finds its covered rows exactly, then the wave's covered pixels are
-        // spread evenly over the lanes
-        const uint32_t cinc = wave_scan_dpp(ucol);
-        const uint32_t ctot = wave_last(cinc);
-        const uint32_t cpre = cinc - ucol;
-        s.pre[lane] = cpre;
-        const uint32_t half = 1u << (S - 1);
-        for (uint32_t c0 = 0; c0 < ctot; c0 += 64) {
-          const uint32_t cidx = c0 + (uint32_t)lane;
-          const int o = wave_owner(cpre, ucol, c0, cl.mark);  // fences s.pre too
-          uint32_t cnt = 0;
-          if (cidx < ctot) {
-            const int4 a = s.xy01[o];
-            const int4 b = s.xy2z[o];
-            const uint4 m = s.misc[o];
-            const int px = (int)(m.z & 0xffffu) + (int)(cidx - s.pre[o]);
-            const int py0 = (int)(m.z >> 16), h = (int)((m.w >> 16) & 0x7fffu);
-            const uint32_t cx = ((uint32_t)px << S) + half, cy = ((uint32_t)py0 << S) + half;
-            const int xy[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
-            int E[3], D[3];
-#pragma unroll
-            for (int kk = 0; kk < 3; ++kk) {  // raster.pikostage:176-180, 223-241
-              const uint32_t xa = (uint32_t)xy[2 * kk], ya = (uint32_t)xy[2 * kk + 1];
-              const uint32_t xb = (uint32_t)xy[(2 * kk + 2) % 6], yb = (uint32_t)xy[(2 * kk + 3) % 6];
-              E[kk] = (int)((cy - ya) * (xb - xa) - (cx - xa) * (yb - ya));
-              D[kk] = (int)((xb - xa) << S);
-            }
-            int lo = 0, hi = h - 1;
-#pragma unroll
-            for (int kk = 0; kk < 3; ++kk) q_edge_rows(E[kk], D[kk], h, lo, hi);
+        // column walk: each lane steps its triangle's bbox columns, finding
+        // each column's covered rows exactly; the nonempty columns go to the
+        // ring, whose fragments are spread evenly over the lanes
+        const int wmax = (int)wave_last((uint32_t)wave_max_scan_dpp(walk));
+        for (int it = 0; it < wmax; ++it) {
+          uint32_t cnt = 0, info = 0;
+          int e1lo = 0, e2lo = 0;
+          if (it < walk) {
+            int lo = max(-wa.F, 0), hi = min(wc.F, hm1);
+            lo = bl ? max(lo, -wb.F) : lo;
+            hi = bh ? min(hi, wb.F) : hi;
             if (lo <= hi) {
               cnt = (uint32_t)(hi - lo + 1);
-              cl.info[lane] = (uint32_t)o | ((uint32_t)px << 6) | ((uint32_t)(py0 + lo) << 18);
-              cl.e1[lane] = (int)((uint32_t)E[1] + (uint32_t)lo * (uint32_t)D[1]);
-              cl.e2[lane] = (int)((uint32_t)E[2] + (uint32_t)lo * (uint32_t)D[2]);
+              info = (uint32_t)lane | ((uint32_t)(lxc + it) << 6) | ((uint32_t)(ly0 + lo) << 12);
+              e1lo = (int)((uint32_t)E1 + (uint32_t)__mul24(lo, D1));
+              e2lo = (int)((uint32_t)E2 + (uint32_t)__mul24(lo, D2));
             }
+            E1 = (int)((uint32_t)E1 + (uint32_t)A1);
+            E2 = (int)((uint32_t)E2 + (uint32_t)A2);
+            wa.step();
+            wb.step();
+            wc.step();
           }
-          const uint32_t finc = wave_scan_dpp(cnt);
-          const uint32_t ftot = wave_last(finc);
-          const uint32_t fpre = finc - cnt;
-          cl.pre[lane] = fpre;
-          for (uint32_t f0 = 0; f0 < ftot; f0 += 64) {
-            const uint32_t f = f0 + (uint32_t)lane;
-            const int c = wave_owner(fpre, cnt, f0, cl.mark);  // fences cl.pre, info, e1, e2 too
-            if (f < ftot) {
-              const uint32_t r = f - cl.pre[c];
-              const uint32_t info = cl.info[c];
-              const int oc = (int)(info & 63u);
-              const int px = (int)((info >> 6) & 0xfffu), py = (int)(info >> 18) + (int)r;
-              const int4 a = s.xy01[oc];
-              const int4 b = s.xy2z[oc];
-              const uint4 m = s.misc[oc];
-              const int e1 = (int)((uint32_t)cl.e1[c] + r * (((uint32_t)b.x - (uint32_t)a.z) << S));
-              const int e2 = (int)((uint32_t)cl.e2[c] + r * (((uint32_t)a.x - (uint32_t)b.x) << S));
-              const float rcp = __uint_as_float(m.y);
-              const float alpha = (float)e1 * rcp, beta = (float)e2 * rcp;  // raster.pikostage:269-270
-              const float z = interp(__int_as_float(b.w), __uint_as_float(m.x), __int_as_float(b.z), alpha,
-                                     beta);  // raster.pikostage:272
-              atomicMin((long long *)(vis + (py - tl.ylo) * T + (px - tl.xlo)),
-                        (long long)(((unsigned long long)__float_as_uint(z) << 32) |
-                                    (unsigned long long)(0xFFFFFFFEu - s.tri[oc])));
-            }
+          const unsigned long long bal = __builtin_amdgcn_ballot_w64(cnt != 0u);
+          const uint32_t rank =
+              __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+          const uint32_t inc = wave_scan_dpp(cnt);
+          if (cnt) s.rec[(tail + rank) & (kQRing - 1)] = make_uint4(ftot + inc - cnt, info, (uint32_t)e1lo, (uint32_t)e2lo);
+          tail += (uint32_t)__builtin_popcountll(bal);
+          ftot += wave_last(inc);
+          wave_fence();
+          while (ftot - fcons >= 64u) {
+            window();
+            fcons += 64u;
           }
-          wave_fence();  // the columns are read before the next window writes them
         }
+        if (ftot != fcons) window();  // the batch's last fragments: its slots are restaged next
+        head = tail;
+        fcons = ftot;
         wave_fence();  // the staged triangles are read before the next batch stages
       }
     }

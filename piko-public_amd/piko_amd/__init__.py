@@ -27,6 +27,7 @@ _lib = None
 PIKO_INPUT_PRETRANSFORMED = 0
 PIKO_INPUT_WORLD = 1
 RASTER_AUTO, RASTER_BINNED, RASTER_DIRECT, RASTER_TILED, RASTER_QUEUE = 0, 1, 2, 3, 4  # PIKO_RASTER_* (include/piko_hip.h)
+COVER_AUTO, COVER_WALK, COVER_SPANS = 0, 1, 2  # PIKO_COVER_* (include/piko_hip.h)
 SAMPLE_AUTO, SAMPLE_GLOBAL, SAMPLE_TILED = 0, 1, 2  # PIKO_SAMPLE_* (include/piko_hip.h)
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "no HIP device",
@@ -73,20 +74,22 @@ class ConstantState(ctypes.Structure):
 class Options(ctypes.Structure):
     _fields_ = [("bin_size", ctypes.c_int), ("subpixel_bits", ctypes.c_int),
                 ("input_kind", ctypes.c_int), ("pair_capacity", ctypes.c_longlong),
-                ("waves_per_cu", ctypes.c_int), ("raster_mode", ctypes.c_int), ("part_pairs", ctypes.c_int)]
+                ("waves_per_cu", ctypes.c_int), ("raster_mode", ctypes.c_int), ("part_pairs", ctypes.c_int),
+                ("queue_cover", ctypes.c_int)]
 
 
 class RasterJob(ctypes.Structure):
     """piko_raster_job: one screen of a batch."""
     _fields_ = [("count", ctypes.c_int), ("src_base", ctypes.c_longlong), ("bin_size", ctypes.c_int),
-                ("waves_per_cu", ctypes.c_int), ("queue_depth", ctypes.c_float), ("cs", ConstantState)]
+                ("waves_per_cu", ctypes.c_int), ("queue_depth", ctypes.c_float), ("cs", ConstantState),
+                ("queue_cover", ctypes.c_int)]
 
 
 class BatchOptions(ctypes.Structure):
     _fields_ = [("input_kind", ctypes.c_int), ("subpixel_bits", ctypes.c_int), ("max_jobs", ctypes.c_int),
                 ("max_tris", ctypes.c_longlong), ("max_pixels", ctypes.c_longlong),
                 ("pair_capacity", ctypes.c_longlong), ("raster_mode", ctypes.c_int), ("part_pairs", ctypes.c_int),
-                ("waves_per_cu", ctypes.c_int)]
+                ("waves_per_cu", ctypes.c_int), ("queue_cover", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -194,18 +197,19 @@ class RasterPipe:
         self.pikoScreen = PikoScreen(self)
 
     def allocate(self, scene, bin_size=16, subpixel_bits=8, pair_capacity=0, waves_per_cu=0, raster_mode=0,
-                 part_pairs=0):
+                 part_pairs=0, queue_cover=0):
         """RasterPipe::allocate (PTXBackend.cpp:649-789): uploads the input and
         sizes every stage.  `scene` is a scenes.TriScene or scenes.WorldScene.
         raster_mode: RASTER_AUTO, RASTER_BINNED, RASTER_DIRECT, RASTER_TILED or
-        RASTER_QUEUE (same image); part_pairs: queue entries per part item
-        (RASTER_QUEUE)."""
+        RASTER_QUEUE (same image); part_pairs: queue entries per part item and
+        queue_cover: COVER_AUTO, COVER_WALK or COVER_SPANS (RASTER_QUEUE)."""
         if self._h is not None:
             raise RuntimeError("pipe already allocated")
         self.W, self.H, self.T, self.S = scene.W, scene.H, bin_size, subpixel_bits
         self.kind = PIKO_INPUT_PRETRANSFORMED if scene.kind == "pretransformed" else PIKO_INPUT_WORLD
         self.cs = ConstantState.for_scene(scene)
-        opt = Options(bin_size, subpixel_bits, self.kind, pair_capacity, waves_per_cu, raster_mode, part_pairs)
+        opt = Options(bin_size, subpixel_bits, self.kind, pair_capacity, waves_per_cu, raster_mode, part_pairs,
+                      queue_cover)
         pos = np.ascontiguousarray(scene.pos, np.float32)
         attr = np.ascontiguousarray(scene.col if self.kind == 0 else scene.nrm)
         h = ctypes.c_void_p()
@@ -215,7 +219,7 @@ class RasterPipe:
         return self
 
     def allocate_device(self, d_pos, d_attr, count, cs, kind="world", bin_size=16, subpixel_bits=8,
-                        pair_capacity=0, waves_per_cu=0, raster_mode=0, part_pairs=0):
+                        pair_capacity=0, waves_per_cu=0, raster_mode=0, part_pairs=0, queue_cover=0):
         """RasterPipe::allocate over input that already lives on the device
         (piko_raster_allocate_device): d_pos / d_attr are device addresses of
         `count` triangles; cs is the ConstantState (screen size and camera)."""
@@ -224,7 +228,8 @@ class RasterPipe:
         self.W, self.H, self.T, self.S = cs.screenSizeX, cs.screenSizeY, bin_size, subpixel_bits
         self.kind = PIKO_INPUT_PRETRANSFORMED if kind == "pretransformed" else PIKO_INPUT_WORLD
         self.cs = cs
-        opt = Options(bin_size, subpixel_bits, self.kind, pair_capacity, waves_per_cu, raster_mode, part_pairs)
+        opt = Options(bin_size, subpixel_bits, self.kind, pair_capacity, waves_per_cu, raster_mode, part_pairs,
+                      queue_cover)
         h = ctypes.c_void_p()
         _check(lib().piko_raster_allocate_device(ctypes.byref(h), ctypes.byref(self.cs), ctypes.c_void_p(d_pos),
                                                  ctypes.c_void_p(d_attr), int(count), ctypes.byref(opt)),
@@ -295,7 +300,7 @@ class BatchPipe:
     whole batch in one kernel sequence (piko_batch_* in include/piko_hip.h)."""
 
     def __init__(self, pos, attr, kind, subpixel_bits=8, max_jobs=1024, max_tris=0, max_pixels=1 << 20,
-                 pair_capacity=0, on_device=False, raster_mode=0, part_pairs=0, waves_per_cu=0):
+                 pair_capacity=0, on_device=False, raster_mode=0, part_pairs=0, waves_per_cu=0, queue_cover=0):
         self.kind = PIKO_INPUT_PRETRANSFORMED if kind == "pretransformed" else PIKO_INPUT_WORLD
         self.S = subpixel_bits
         if not on_device:
@@ -307,7 +312,7 @@ class BatchPipe:
             pp, ap, n = pos, attr, max_tris
         self.input_tris = int(n)
         opt = BatchOptions(self.kind, subpixel_bits, max_jobs, max(max_tris, 1), max_pixels, pair_capacity,
-                           raster_mode, part_pairs, waves_per_cu)
+                           raster_mode, part_pairs, waves_per_cu, queue_cover)
         h = ctypes.c_void_p()
         _check(lib().piko_batch_create(ctypes.byref(h), pp, ap, int(n), 1 if on_device else 0, ctypes.byref(opt)),
                "batch_create")
@@ -315,7 +320,7 @@ class BatchPipe:
         self.jobs = []
 
     def submit(self, jobs):
-        """jobs: list of (count, src_base, bin_size, ConstantState[, waves_per_cu, queue_depth])."""
+        """jobs: list of (count, src_base, bin_size, ConstantState[, waves_per_cu, queue_depth[, queue_cover]])."""
         arr = (RasterJob * len(jobs))()
         for k, jb in enumerate(jobs):
             count, src, T, cs = jb[:4]
@@ -323,6 +328,8 @@ class BatchPipe:
             if len(jb) > 4:
                 arr[k].waves_per_cu = int(jb[4])
                 arr[k].queue_depth = float(jb[5])
+            if len(jb) > 6:
+                arr[k].queue_cover = int(jb[6])
         self.jobs = [tuple(jb[:4]) for jb in jobs]
         _check(lib().piko_batch_submit(self._h, arr, len(jobs)), "batch_submit")
 

@@ -129,24 +129,30 @@ def tune_batch(scene_list, pos, attr, parts, S=8, tile_sizes=TILE_SIZES, frames=
 
 
 def tune_queue_batch(scene_list, pos, attr, parts, S=8, tile_sizes=(32, 64), waves=(16, 24),
-                     part_pairs=(1024, 2048), class_tiles=(16, 32, 64), frames=3, log=None):
+                     part_pairs=(1024, 2048), class_tiles=(16, 32, 64), class_covers=None, frames=3, log=None):
     """The queue path's batch settings, timed in the batch itself.  First one
     tile size for every job with the persistent grid's waves per CU and the
     queue entries per part item (the raster is one persistent launch over
     all of the batch's tile parts, so those two are per batch); then, per
-    scene class (largest first), each other tile size in `class_tiles` for
-    that class's jobs, kept when the whole batch gets at least 2% faster.
-    Returns ({"bin_size": the common tile size, "waves_per_cu",
-    "part_pairs", "tiles": per-scene tile sizes, "class_bin_size":
-    {class: T} where it differs, "ms"}, trials)."""
-    from . import BatchPipe, ConstantState, RASTER_QUEUE
+    scene class (largest first), each other tile size in `class_tiles` and
+    each other coverage mode in `class_covers` (COVER_WALK / COVER_SPANS;
+    default both) for that class's jobs, kept when the whole batch gets at
+    least 2% faster.  Returns ({"bin_size": the common tile size,
+    "waves_per_cu", "part_pairs", "tiles": per-scene tile sizes, "covers":
+    per-scene coverage modes (0: the batch's), "class_bin_size" and
+    "class_cover": {class: value} where it differs, "ms"}, trials)."""
+    from . import BatchPipe, ConstantState, RASTER_QUEUE, COVER_WALK, COVER_SPANS
+    if class_covers is None:
+        class_covers = (COVER_WALK, COVER_SPANS)
     kind = scene_list[0].kind if scene_list else "pretransformed"
     ntris = sum(sc.n for sc in scene_list)
     npix = sum(sc.W * sc.H for sc in scene_list)
     best, trials = None, []
 
-    def timed(bp, tiles):
-        bp.submit([(n, base, T, ConstantState.for_scene(sc)) for (n, base, sc), T in zip(parts, tiles)])
+    def timed(bp, tiles, covers=None):
+        covers = covers or [0] * len(tiles)
+        bp.submit([(n, base, T, ConstantState.for_scene(sc), 0, 0.0, cv)
+                   for (n, base, sc), T, cv in zip(parts, tiles, covers)])
         bp.wait()
         bp.run_frames(1)
         t0 = time.perf_counter()
@@ -174,30 +180,43 @@ def tune_queue_batch(scene_list, pos, attr, parts, S=8, tile_sizes=(32, 64), wav
         groups.setdefault(scene_class(sc.n, sc.W, sc.H), []).append(k)
     order = sorted(groups, key=lambda c: -sum(scene_list[k].n * 48 + scene_list[k].W * scene_list[k].H * 8
                                               for k in groups[c]))
-    class_T = {}
-    if class_tiles and len(groups) > 1:
+    class_T, class_cv = {}, {}
+    covers = [0] * len(scene_list)
+    if (class_tiles and len(groups) > 1) or class_covers:
         bp = BatchPipe(pos, attr, kind, subpixel_bits=S, max_jobs=max(len(scene_list), 1), max_tris=ntris,
                        max_pixels=npix, raster_mode=RASTER_QUEUE, part_pairs=pp, waves_per_cu=w)
         try:
-            dt = timed(bp, tiles)
+            dt = timed(bp, tiles, covers)
             for cls in order:
-                for alt in class_tiles:
+                for alt in (class_tiles if len(groups) > 1 else ()):
                     if alt == tiles[groups[cls][0]]:
                         continue
                     trial = list(tiles)
                     for k in groups[cls]:
                         trial[k] = alt
-                    dt2 = timed(bp, trial)
+                    dt2 = timed(bp, trial, covers)
                     trials.append((("class",) + tuple(cls), alt, w, pp, dt2 * 1e3))
                     if dt2 < 0.98 * dt:
                         tiles, dt = trial, dt2
                         class_T[cls] = alt
                         if log:
                             log("queue batch class %s: T=%d (%.3f ms)" % (cls, alt, dt * 1e3))
+                for cv in class_covers:
+                    trial = list(covers)
+                    for k in groups[cls]:
+                        trial[k] = cv
+                    dt2 = timed(bp, tiles, trial)
+                    trials.append((("class",) + tuple(cls), "cover", cv, w, pp, dt2 * 1e3))
+                    if dt2 < 0.98 * dt:
+                        covers, dt = trial, dt2
+                        class_cv[cls] = cv
+                        if log:
+                            log("queue batch class %s: cover %d (%.3f ms)" % (cls, cv, dt * 1e3))
         finally:
             bp.destroy()
-    return {"bin_size": T, "waves_per_cu": w, "part_pairs": pp, "tiles": tiles,
-            "class_bin_size": {"%d,%d" % c: t for c, t in class_T.items()}, "ms": dt * 1e3}, trials
+    return {"bin_size": T, "waves_per_cu": w, "part_pairs": pp, "tiles": tiles, "covers": covers,
+            "class_bin_size": {"%d,%d" % c: t for c, t in class_T.items()},
+            "class_cover": {"%d,%d" % c: v for c, v in class_cv.items()}, "ms": dt * 1e3}, trials
 
 
 def save(table, path):

@@ -58,6 +58,10 @@ def test_invalid_arguments_rejected_without_device():
     assert L.piko_raster_allocate(ctypes.byref(h), ctypes.byref(cs), None, None, 0, ctypes.byref(bad_bin)) == -1
     bad_s = piko_amd.Options(16, 9, 0, 0)
     assert L.piko_raster_allocate(ctypes.byref(h), ctypes.byref(cs), None, None, 0, ctypes.byref(bad_s)) == -1
+    bad_cover = piko_amd.Options(16, 8, 0, 0, 0, piko_amd.RASTER_QUEUE, 0, 3)
+    assert L.piko_raster_allocate(ctypes.byref(h), ctypes.byref(cs), None, None, 0, ctypes.byref(bad_cover)) == -1
+    bo = piko_amd.BatchOptions(0, 8, 4, 1, 4096, 0, piko_amd.RASTER_QUEUE, 0, 0, -1)
+    assert L.piko_batch_create(ctypes.byref(h), None, None, 0, 0, ctypes.byref(bo)) == -1
     assert L.piko_raster_run_single(None) == -1
 
 

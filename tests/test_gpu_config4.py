@@ -2,7 +2,7 @@
 scenes from seeds 1000-1255 (triangle counts log-uniform in [1K, 500K],
 pixel counts log-uniform in [256^2, 2048^2], aspect 1, 4:3 or 16:9), all of
 them in one batch with autotuned tile sizes, on the tiled path and on the
-queue path (one persistent raster launch).  Every scene's colour words and
+queue path (one persistent raster launch) in both of its coverage modes.  Every scene's colour words and
 depth bits equal the oracle's (threaded, on the reference CPU schedule) and
 its plain launch's."""
 import sys
@@ -32,9 +32,10 @@ def test_config4_every_scene_matches_oracle_and_plain(oracle_mod):
     jobs = [(n, base, T, piko_amd.ConstantState.for_scene(s)) for (n, base, s), T in zip(parts, tiles)]
     pipes = {}
     try:
-        for name, mode in (("tiled", piko_amd.RASTER_TILED), ("queue", piko_amd.RASTER_QUEUE)):
+        for name, mode, qc in (("tiled", piko_amd.RASTER_TILED, 0), ("queue", piko_amd.RASTER_QUEUE, 0),
+                               ("queue-walk", piko_amd.RASTER_QUEUE, piko_amd.COVER_WALK)):
             bp = piko_amd.BatchPipe(pos, attr, "pretransformed", max_jobs=len(scs), max_tris=ntris,
-                                    max_pixels=npix, raster_mode=mode)
+                                    max_pixels=npix, raster_mode=mode, queue_cover=qc)
             pipes[name] = bp
             bp.submit(jobs)
             bp.wait()

@@ -3,7 +3,9 @@
 frame is checked in every raster mode: binned (queues sorted by bin, one wave
 per bin), direct (per-pixel visibility words, no queues), tiled (per-chunk
 tile lists, LDS visibility per tile) and queue (one persistent launch pulling
-bin and tile-part items from a device queue)."""
+bin and tile-part items from a device queue; "queue" walks the columns of
+every int32-path triangle, "queue-spans" covers compact ones by 8-row
+spans)."""
 import sys
 
 import numpy as np
@@ -19,14 +21,15 @@ def log(msg):
     sys.stdout.flush()
 
 
-MODES = ("binned", "direct", "tiled", "queue")
+MODES = ("binned", "direct", "tiled", "queue", "queue-spans")
 
 
 def gpu_frame(scene, T, S, pair_capacity=0, mode="tiled"):
     import piko_amd
     rm = {"binned": piko_amd.RASTER_BINNED, "direct": piko_amd.RASTER_DIRECT, "tiled": piko_amd.RASTER_TILED,
-          "queue": piko_amd.RASTER_QUEUE}[mode]
-    with piko_amd.RasterPipe().allocate(scene, T, S, pair_capacity, raster_mode=rm) as pipe:
+          "queue": piko_amd.RASTER_QUEUE, "queue-spans": piko_amd.RASTER_QUEUE}[mode]
+    cover = piko_amd.COVER_SPANS if mode == "queue-spans" else piko_amd.COVER_AUTO
+    with piko_amd.RasterPipe().allocate(scene, T, S, pair_capacity, raster_mode=rm, queue_cover=cover) as pipe:
         pipe.prepare()
         pipe.run_single()
         return pipe.pikoScreen.getData(), pipe.get_depth(), pipe.get_bins(), pipe.stats()
@@ -191,8 +194,9 @@ def test_direct_frames_across_camera_changes(oracle_mod):
             assert np.array_equal(c, oc) and np.array_equal(d.view(np.int32), od.view(np.int32)), step
 
 
+@pytest.mark.parametrize("cover", ["walk", "spans"])
 @pytest.mark.parametrize("T", [32, 64])
-def test_queue_frames_across_camera_changes(oracle_mod, T):
+def test_queue_frames_across_camera_changes(oracle_mod, T, cover):
     # queue-path runs of several frames with the camera changed between runs
     # and small parts (many split tiles): moving near the terrain overflows
     # tile ranges sized on the far view (the ranges grow and the frames
@@ -202,7 +206,9 @@ def test_queue_frames_across_camera_changes(oracle_mod, T):
     near = scenes.heightfield(seed=6, grid=80, W=400, H=240, eye=(0.0, 0.12, 0.25), target=(0.0, 0.0, -0.3))
     ofar, odfar, _ = oracle_mod.frame(far, 8, T)
     onear, odnear, _ = oracle_mod.frame(near, 8, T)
-    with piko_amd.RasterPipe().allocate(far, T, 8, raster_mode=piko_amd.RASTER_QUEUE, part_pairs=64) as pipe:
+    qc = piko_amd.COVER_WALK if cover == "walk" else piko_amd.COVER_SPANS
+    with piko_amd.RasterPipe().allocate(far, T, 8, raster_mode=piko_amd.RASTER_QUEUE, part_pairs=64,
+                                        queue_cover=qc) as pipe:
         pipe.prepare()
         for step, (sc, oc, od, n) in enumerate([(far, ofar, odfar, 4), (near, onear, odnear, 2), (near, onear, odnear, 5),
                                                 (far, ofar, odfar, 3), (near, onear, odnear, 1), (far, ofar, odfar, 6)]):
