@@ -44,10 +44,16 @@ constexpr int kQChunk = kQThreads * kQPer;          // triangles per bin item
 constexpr int kQMaxTiles = 4096;                    // tiles of one job (LDS histogram)
 constexpr uint32_t kQNoSlot = 0x80000000u;          // a run that did not fit its tile's range
 
-// Queue head of the raster launch.
+// Queue heads of the raster launch: the part items are dealt round-robin to
+// kQQueues queues, one per XCD (workgroup b runs on XCD b % 8), each head in
+// a line of its own.  With one head for the whole launch, the workgroups'
+// global adds serialised at one address: a launch drawing nothing (the items'
+// loads, barriers and hand-out alone) took 41 us on the 2M height field, 19
+// us with a head per XCD.
+constexpr int kQQueues = 8;
+constexpr int kQHeadStride = 64;  // words between heads (256 bytes)
 struct QState {
-  uint32_t next;        // next part item to hand out
-  uint32_t pad[3];
+  uint32_t next[kQQueues * kQHeadStride];  // next position of each queue to hand out
 };
 
 // One part item: tile (batch tile numbering), part index, parts of the tile,
@@ -330,26 +336,39 @@ __device__ __forceinline__ QTile q_tile(const FrameArgs &ja, int gt) {
 }
 
 // ---- k_qbin ----------------------------------------------------------------
+// A vertex position as stored in the input (three floats, 4-byte aligned).
+struct __attribute__((packed, aligned(4))) QVec3 {
+  float x, y, z;
+};
+
 template <int KIND, bool MULTI>
 __global__ void __launch_bounds__(kQThreads) k_qbin(QArgs qa, FrameArgs fa) {
   __shared__ uint32_t cnt[kQMaxTiles];
   const int t = threadIdx.x;
-  if (blockIdx.x == 0 && t == 0) qa.qs->next = 0;  // the raster launch's queue head
+  if (blockIdx.x == 0 && t < kQQueues) qa.qs->next[t * kQHeadStride] = 0;  // the raster launch's queue heads
   if ((int)blockIdx.x >= qa.nchunks) return;       // a frame without triangles runs one block
   const QChunk cd = qa.chunks[blockIdx.x];
   const FrameArgs &ja = MULTI ? qa.jobs[cd.job] : fa;
   const int ntiles = ja.nBx * ja.nBy;
   const int S = ja.S, lgT = ja.lgT, bsb = S + lgT;
   for (int q = t; q < ntiles; q += kQThreads) cnt[q] = 0;
-  // all of the thread's vertex positions first, so their loads overlap
+  // all of the thread's vertex positions first, so their loads overlap: one
+  // 12-byte load per vertex where nine 4-byte loads touched the same cache
+  // lines nine times over (a triangle's floats are only 4-byte aligned)
   float vp[kQPer][9];
 #pragma unroll
   for (int k = 0; k < kQPer; ++k) {
     const uint32_t il = (uint32_t)(k * kQThreads + t);
     const uint32_t v = cd.tri0 + min(il, cd.count - 1u);
     const size_t src = MULTI ? (size_t)ja.src_base + (size_t)(v - ja.vbase) : (size_t)v;
+    const QVec3 *p3 = reinterpret_cast<const QVec3 *>(qa.pos + 9 * src);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) vp[k][q] = il < cd.count ? qa.pos[9 * src + q] : 0.0f;
+    for (int q = 0; q < 3; ++q) {
+      const QVec3 a = p3[q];
+      vp[k][3 * q] = il < cd.count ? a.x : 0.0f;
+      vp[k][3 * q + 1] = il < cd.count ? a.y : 0.0f;
+      vp[k][3 * q + 2] = il < cd.count ? a.z : 0.0f;
+    }
   }
   __syncthreads();  // histogram cleared
   // per triangle: its tile rectangle (tx0 | ty0 << 16, tx1 | ty1 << 16), or
@@ -431,14 +450,22 @@ __global__ void __launch_bounds__(kQThreads) k_qraster(QArgs qa, FrameArgs fa) {
   unsigned long long *const vis = sm.vis;
   const int t = threadIdx.x, lane = t & 63;
   for (int q = t; q < 256; q += kQThreads) sm.u8[q] = kU8Table.v[q];
-  // The first item of a workgroup is its own index; later ones come from the
-  // queue head, one global add each, taken only when the workgroup is free
-  // (taking the next one early, while working on this one, was slower: a
-  // busy workgroup held an item other workgroups could have started).
+  // The first item of a workgroup is its own index; later ones come from its
+  // XCD's queue (item i is position i / nqs of queue i % nqs), one global add
+  // each, taken only when the workgroup is free.  Taking the next one earlier
+  // (at the start of an item, or at its last batch) was slower: a busy
+  // workgroup held an item other workgroups could have started, and the
+  // global add's return held up every load the wave issued after it.  Every
+  // queue has a workgroup (nqs is at most the grid), and its first positions
+  // are the static items of its workgroups.
+  const uint32_t nqs = min((uint32_t)kQQueues, gridDim.x);
+  const uint32_t myq = blockIdx.x % nqs;
+  const uint32_t statics = (gridDim.x - myq + nqs - 1) / nqs;
   bool first = true;
   for (;;) {
     __syncthreads();  // the previous item is done with the shared state
-    if (t == 0) sm.item = first ? blockIdx.x : gridDim.x + atomicAdd(&qa.qs->next, 1u);
+    if (t == 0)
+      sm.item = first ? blockIdx.x : (statics + atomicAdd(&qa.qs->next[myq * kQHeadStride], 1u)) * nqs + myq;
     first = false;
     __syncthreads();
     const uint32_t item = sm.item;
