@@ -2292,8 +2292,8 @@ void launch_queue(piko_raster_pipe *p, bool raster, Marks *mark) {
     else k_qraster<QT, KIND, false><<<rgrid, kQThreads, 0, p->stream>>>(qa, fa);                   \
     if (mark) (*mark)();                                                                            \
     if (p->q_nsplits > 0) {                                                                         \
-      if (p->multi) k_qmerge<QT, KIND, true><<<p->q_nsplits, kQThreads, 0, p->stream>>>(qa, fa);   \
-      else k_qmerge<QT, KIND, false><<<p->q_nsplits, kQThreads, 0, p->stream>>>(qa, fa);           \
+      if (p->multi) k_qmerge<QT, KIND, true><<<p->q_nsplits, kQMergeThreads, 0, p->stream>>>(qa, fa); \
+      else k_qmerge<QT, KIND, false><<<p->q_nsplits, kQMergeThreads, 0, p->stream>>>(qa, fa);         \
       if (mark) (*mark)();                                                                          \
     }                                                                                               \
   } while (0)

@@ -273,21 +273,22 @@ __device__ __forceinline__ bool q_tile_ok(const Edges &e, int tx, int ty, bool o
   return one || tr_accept(e, tx, ty, bsb);
 }
 
-// Resolve of one tile from its visibility words: depth = the winner's z,
-// colour = Gouraud of the winner (the clear colour where nothing is drawn).
-template <int KIND, bool MULTI>
+// Resolve of one tile from its visibility words by a workgroup of NT threads:
+// depth = the winner's z, colour = Gouraud of the winner (the clear colour
+// where nothing is drawn).
+template <int KIND, bool MULTI, int NT = kQThreads>
 __device__ __forceinline__ void q_resolve(const QArgs &qa, const FrameArgs &ja, const unsigned long long *vis,
                                           const float *s_u8, int xlo, int ylo, int xhi, int yhi) {
   const int T = 1 << ja.lgT, S = ja.S;
   const size_t fbb = MULTI ? (size_t)ja.fb_base : 0;
-  constexpr int kBatch = 4;  // pixels per thread whose gathers are in flight together
-  for (int q0 = 0; q0 < T * T; q0 += kBatch * kQThreads) {
+  constexpr int kBatch = NT >= 1024 ? 1 : 4;  // pixels per thread whose gathers are in flight together
+  for (int q0 = 0; q0 < T * T; q0 += kBatch * NT) {
     unsigned long long key[kBatch];
     int4 ga[kBatch], gb[kBatch];
     uint32_t c[kBatch][3];
 #pragma unroll
     for (int k = 0; k < kBatch; ++k) {
-      const int q = q0 + k * kQThreads + (int)threadIdx.x;
+      const int q = q0 + k * NT + (int)threadIdx.x;
       key[k] = q < T * T ? vis[q] : kVisSentinel;
       ga[k] = gb[k] = make_int4(0, 0, 0, 0);
       c[k][0] = c[k][1] = c[k][2] = 0;
@@ -301,7 +302,7 @@ __device__ __forceinline__ void q_resolve(const QArgs &qa, const FrameArgs &ja, 
     }
 #pragma unroll
     for (int k = 0; k < kBatch; ++k) {
-      const int q = q0 + k * kQThreads + (int)threadIdx.x;
+      const int q = q0 + k * NT + (int)threadIdx.x;
       if (q >= T * T) continue;
       const int px = xlo + (q & (T - 1)), py = ylo + (q >> ja.lgT);
       if (px >= xhi || py >= yhi) continue;
@@ -752,48 +753,39 @@ __global__ void __launch_bounds__(kQThreads) k_qraster(QArgs qa, FrameArgs fa) {
 }
 
 // ---- k_qmerge --------------------------------------------------------------
+// One workgroup of kQMergeThreads per split tile: a thread per pixel of a
+// 32-pixel tile, its parts' words loaded four at a time (256 threads with
+// four pixels each waited for their loads over more rounds: 10.7 us on the
+// 2M height field).
+constexpr int kQMergeThreads = 1024;
 template <int QT, int KIND, bool MULTI>
-__global__ void __launch_bounds__(kQThreads) k_qmerge(QArgs qa, FrameArgs fa) {
+__global__ void __launch_bounds__(kQMergeThreads) k_qmerge(QArgs qa, FrameArgs fa) {
   __shared__ unsigned long long vis[QT * QT];
   __shared__ float s_u8[256];
   const int t = threadIdx.x;
-  for (int q = t; q < 256; q += kQThreads) s_u8[q] = kU8Table.v[q];
+  if (t < 256) s_u8[t] = kU8Table.v[t];
   const QItem it = qa.items[qa.splits[blockIdx.x]];
   const int gt = (int)it.tile;
   const FrameArgs &ja = MULTI ? qa.jobs[job_search(qa.jobs, qa.njobs, (uint32_t)gt, true)] : fa;
   const QTile tl = q_tile(ja, gt);
   const int T = 1 << ja.lgT;
-  // the minimum over the parts' words: a thread's kPix pixels of two parts at
-  // a time, all their loads issued before any is used (a loop over one part
-  // word at a time waited for each load in turn; four parts at a time were
-  // no faster)
-  constexpr int kPix = 4;
   const unsigned long long *slots = qa.scratch + (size_t)it.slot * (size_t)(QT * QT);
-  for (int q0 = 0; q0 < T * T; q0 += kPix * kQThreads) {
-    long long m[kPix], a[kPix], b[kPix];
+  for (int q = t; q < T * T; q += kQMergeThreads) {
+    long long m = (long long)kVisSentinel;
+    uint32_t j = 0;
+    for (; j + 4 <= it.nparts; j += 4) {
+      long long w[4];
 #pragma unroll
-    for (int k = 0; k < kPix; ++k) m[k] = (long long)kVisSentinel;
-    for (uint32_t j = 0; j < it.nparts; j += 2) {
-      const bool two = j + 1 < it.nparts;
-#pragma unroll
-      for (int k = 0; k < kPix; ++k) {
-        const int q = min(q0 + k * kQThreads + t, T * T - 1);
-        a[k] = (long long)slots[(size_t)j * (size_t)(QT * QT) + q];
-        b[k] = (long long)slots[(size_t)(two ? j + 1 : j) * (size_t)(QT * QT) + q];
-      }
-#pragma unroll
-      for (int k = 0; k < kPix; ++k) m[k] = min(m[k], min(a[k], b[k]));
+      for (int k = 0; k < 4; ++k) w[k] = (long long)slots[(size_t)(j + k) * (size_t)(QT * QT) + q];
+      m = min(m, min(min(w[0], w[1]), min(w[2], w[3])));
     }
-#pragma unroll
-    for (int k = 0; k < kPix; ++k) {
-      const int q = q0 + k * kQThreads + t;
-      if (q < T * T) vis[q] = (unsigned long long)m[k];
-    }
+    for (; j < it.nparts; ++j) m = min(m, (long long)slots[(size_t)j * (size_t)(QT * QT) + q]);
+    vis[q] = (unsigned long long)m;
   }
   if (t == 0) {
     qa.tile_need[gt] = qa.tile_fill[gt];
     qa.tile_fill[gt] = 0;
   }
   __syncthreads();
-  q_resolve<KIND, MULTI>(qa, ja, vis, s_u8, tl.xlo, tl.ylo, tl.xhi, tl.yhi);
+  q_resolve<KIND, MULTI, kQMergeThreads>(qa, ja, vis, s_u8, tl.xlo, tl.ylo, tl.xhi, tl.yhi);
 }
