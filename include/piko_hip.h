@@ -124,7 +124,7 @@ typedef struct {
   int waves_per_cu;       /* persistent raster waves per CU; 0 = default (20;
                              QUEUE: 20, five 256-thread workgroups) */
   int raster_mode;        /* PIKO_RASTER_*; 0 = AUTO */
-  int part_pairs;         /* QUEUE: queue entries per part item; 0 = 1024 */
+  int part_pairs;         /* QUEUE: queue entries per part item; 0 = 2048 */
   int queue_cover;        /* QUEUE: PIKO_COVER_*; 0 = AUTO (WALK) */
 } piko_raster_options;
 

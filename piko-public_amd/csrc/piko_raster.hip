@@ -1664,9 +1664,11 @@ struct piko_raster_pipe {
   long long cap_qchunks = 0;
   int q_nchunks = 0;
   uint32_t *d_qbase = nullptr, *d_qcap = nullptr, *d_qfill = nullptr, *d_qneed = nullptr;
+  unsigned long long *d_qcost = nullptr;  // the counting pass's raster cost estimate per tile
   long long cap_qtiles = 0;
   int q_ntiles = 0;
   std::vector<uint32_t> q_cap;          // host copy of the tiles' queue capacities
+  std::vector<unsigned long long> q_cost;  // the tiles' cost estimates (the counting pass's; 0: none)
   bool q_caps_known = false;            // q_cap holds counts of a frame of the current jobs
   QItem *d_qitems = nullptr;
   long long cap_qitems = 0;
@@ -1914,6 +1916,8 @@ void free_queue(piko_raster_pipe *p) {
   q_free(p, p->d_qfill, tc);
   tc = p->cap_qtiles;
   q_free(p, p->d_qneed, tc);
+  tc = p->cap_qtiles;
+  q_free(p, p->d_qcost, tc);
   p->cap_qtiles = 0;
   q_free(p, p->d_qitems, p->cap_qitems);
   q_free(p, p->d_qsplits, p->cap_qsplits);
@@ -1924,32 +1928,51 @@ void free_queue(piko_raster_pipe *p) {
 }
 
 // Part items and queue ranges from the tiles' capacities: ranges in tile
-// order, items densest tile first, split tiles with their scratch slots.
+// order, split tiles with their scratch slots, a tile cut into parts of at
+// most part_pairs entries (equal slices of its range).  Items go costliest
+// first, by the counting pass's cost estimate per part (q_cost) where there
+// is one, else by entries: a near tile of a few hundred large triangles costs
+// as much as a horizon tile of thousands of small ones, and started last it
+// set the length of the launch (k_qraster 79.8 -> 76.8 us on the 2M height
+// field).  Cutting the costliest tiles into more parts balanced the
+// workgroups better but added as much per-item time as it saved.
 int q_build(piko_raster_pipe *p) {
   const int n = p->q_ntiles;
   std::vector<uint32_t> base((size_t)n);
-  unsigned long long total = 0;
+  unsigned long long total = 0, total_cost = 0;
   for (int t = 0; t < n; ++t) {
     base[t] = (uint32_t)total;
     total += p->q_cap[t];
+    total_cost += t < (int)p->q_cost.size() ? p->q_cost[t] : 0ull;
   }
   if (total >= (unsigned long long)kQNoSlot) return PIKO_ERR_CAPACITY;
   const long long P = p->q_part_pairs;
+  std::vector<uint32_t> np((size_t)n);
+  std::vector<unsigned long long> key((size_t)n);  // per-part cost (entries where no cost is known)
+  for (int t = 0; t < n; ++t) {
+    const long long cap = p->q_cap[t];
+    const unsigned long long c = t < (int)p->q_cost.size() ? p->q_cost[t] : 0ull;
+    np[t] = (uint32_t)std::max<long long>(1, cdiv(cap, P));
+    key[t] = total_cost > 0 ? c / np[t] : (unsigned long long)cap;
+  }
   std::vector<int> order((size_t)n);
   for (int t = 0; t < n; ++t) order[t] = t;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return p->q_cap[a] > p->q_cap[b]; });
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] > key[b]; });
   std::vector<QItem> items;
   std::vector<uint32_t> splits;
   items.reserve((size_t)n + (size_t)(total / (unsigned long long)P) + 1);
   long long slots = 0;
   for (int t : order) {
-    const uint32_t np = (uint32_t)std::max<long long>(1, cdiv((long long)p->q_cap[t], P));
-    const uint32_t slot = np > 1 ? (uint32_t)slots : 0u;
-    if (np > 1) {
-      slots += np;
+    const uint32_t nparts = np[t];
+    const uint32_t slot = nparts > 1 ? (uint32_t)slots : 0u;
+    if (nparts > 1) {
+      slots += nparts;
       splits.push_back((uint32_t)items.size());
     }
-    for (uint32_t k = 0; k < np; ++k) items.push_back(QItem{(uint32_t)t, k, np, slot, base[t], p->q_cap[t], 0u, 0u});
+    const unsigned long long cap = p->q_cap[t];
+    for (uint32_t k = 0; k < nparts; ++k)
+      items.push_back(QItem{(uint32_t)t, k, nparts, slot, base[t], p->q_cap[t], (uint32_t)(cap * k / nparts),
+                            (uint32_t)(cap * (k + 1) / nparts)});
   }
   if (items.size() > 0x7FFFFFFFull || slots > 0xFFFFFFFFll) return PIKO_ERR_CAPACITY;
   int rc;
@@ -2025,9 +2048,12 @@ int q_setup(piko_raster_pipe *p, const FrameArgs *jobs, int njobs, long long tot
     q_free(p, p->d_qfill, tc);
     tc = p->cap_qtiles;
     q_free(p, p->d_qneed, tc);
+    tc = p->cap_qtiles;
+    q_free(p, p->d_qcost, tc);
     const long long cap = std::max<long long>(ntiles, 1);
     for (uint32_t **a : {&p->d_qbase, &p->d_qcap, &p->d_qfill, &p->d_qneed})
       if ((rc = dalloc(p, a, cap, &p->q_bytes)) != PIKO_OK) return rc;
+    if ((rc = dalloc(p, &p->d_qcost, cap, &p->q_bytes)) != PIKO_OK) return rc;
     p->cap_qtiles = cap;
     HIPOK(hipMemsetAsync(p->d_qfill, 0, (size_t)cap * 4, p->stream));
   }
@@ -2035,6 +2061,7 @@ int q_setup(piko_raster_pipe *p, const FrameArgs *jobs, int njobs, long long tot
   if (!same_tiles) {
     p->q_ntiles = (int)ntiles;
     p->q_cap.assign((size_t)ntiles, 0u);
+    p->q_cost.assign((size_t)ntiles, 0ull);
     p->q_caps_known = false;
   }
   return q_build(p);
@@ -2271,13 +2298,16 @@ void launch_queue(piko_raster_pipe *p, bool raster, Marks *mark) {
   QArgs qa{p->d_pos,    p->d_attr,  p->d_geo,    p->d_qpairs,   p->d_qbase, p->d_qcap,
            p->d_qfill,  p->d_qneed, p->d_qitems, p->d_qsplits,  p->d_qchunks, p->d_qscr,
            p->d_jobs,   p->d_qs,    p->d_fs,     p->d_color,    p->d_depth,
-           (int)p->jobs.size(), p->q_nchunks, p->q_nitems, p->q_part_pairs, p->q_nsplits};
+           raster ? nullptr : p->d_qcost,
+           (int)p->jobs.size(), p->q_nchunks, p->q_nitems, p->q_nsplits};
   const FrameArgs &fa = p->fa_q;
   const unsigned bgrid = (unsigned)std::max(p->q_nchunks, 1);
-#define PIKO_QB_LAUNCH(KIND)                                                           \
-  do {                                                                                 \
-    if (p->multi) k_qbin<KIND, true><<<bgrid, kQThreads, 0, p->stream>>>(qa, fa);     \
-    else k_qbin<KIND, false><<<bgrid, kQThreads, 0, p->stream>>>(qa, fa);             \
+#define PIKO_QB_LAUNCH(KIND)                                                                          \
+  do {                                                                                                \
+    if (p->multi && raster) k_qbin<KIND, true, false><<<bgrid, kQThreads, 0, p->stream>>>(qa, fa);   \
+    else if (p->multi) k_qbin<KIND, true, true><<<bgrid, kQThreads, 0, p->stream>>>(qa, fa);         \
+    else if (raster) k_qbin<KIND, false, false><<<bgrid, kQThreads, 0, p->stream>>>(qa, fa);         \
+    else k_qbin<KIND, false, true><<<bgrid, kQThreads, 0, p->stream>>>(qa, fa);                      \
   } while (0)
   if (p->kind == PIKO_INPUT_PRETRANSFORMED) PIKO_QB_LAUNCH(PIKO_INPUT_PRETRANSFORMED);
   else PIKO_QB_LAUNCH(PIKO_INPUT_WORLD);
@@ -2311,11 +2341,17 @@ void launch_queue(piko_raster_pipe *p, bool raster, Marks *mark) {
 // tile's entries (every run is refused, the ranges being empty), and the
 // ranges are sized from those counts with some room for camera changes.
 int q_count(piko_raster_pipe *p) {
+  const int n = p->q_ntiles;
+  if (n > 0) HIPOK(hipMemsetAsync(p->d_qcost, 0, (size_t)n * sizeof(unsigned long long), p->stream));
   launch_queue(p, false, nullptr);
   HIPOK(hipGetLastError());
-  const int n = p->q_ntiles;
   std::vector<uint32_t> cnt((size_t)std::max(n, 1), 0u);
-  if (n > 0) HIPOK(hipMemcpyAsync(cnt.data(), p->d_qfill, (size_t)n * 4, hipMemcpyDeviceToHost, p->stream));
+  p->q_cost.assign((size_t)std::max(n, 1), 0ull);
+  if (n > 0) {
+    HIPOK(hipMemcpyAsync(cnt.data(), p->d_qfill, (size_t)n * 4, hipMemcpyDeviceToHost, p->stream));
+    HIPOK(hipMemcpyAsync(p->q_cost.data(), p->d_qcost, (size_t)n * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, p->stream));
+  }
   HIPOK(hipStreamSynchronize(p->stream));
   if (n > 0) HIPOK(hipMemsetAsync(p->d_qfill, 0, (size_t)n * 4, p->stream));
   HIPOK(hipMemsetAsync(&p->d_fs->overflow, 0, 4, p->stream));
@@ -2562,7 +2598,7 @@ int plain_pipe(piko_raster_pipe **out, const piko_constant_state *cs, const floa
   p->classes.assign(1, RasterClass{p->T, 0, p->nbins, opt->waves_per_cu > 0 ? opt->waves_per_cu * p->cus : 0});
   if (rc == PIKO_OK && p->mode == PIKO_RASTER_QUEUE) {
     p->fa_q = make_args(cs, opt->subpixel_bits, queue_T(p->T, p->W, p->H));
-    p->q_part_pairs = opt->part_pairs > 0 ? opt->part_pairs : 1024;
+    p->q_part_pairs = opt->part_pairs > 0 ? opt->part_pairs : 2048;
     p->q_wpc = opt->waves_per_cu > 0 ? opt->waves_per_cu : 20;
     p->q_walk = opt->queue_cover == PIKO_COVER_SPANS ? 0u : 1u;
     p->fa_q.qwalk = p->q_walk;
@@ -2906,7 +2942,7 @@ int piko_batch_create(piko_raster_pipe **out, const float *pos, const void *attr
   p->input_tris = input_tris;
   p->cap_jobs = opt->max_jobs;
   // a batch's queue path: longer parts and a wider persistent grid (the
-  // 256-scene batch's tuned setting; a single 1080p frame tunes to 1024 / 20)
+  // 256-scene batch's tuned setting; a single 1080p frame tunes to 2048 / 20)
   p->q_part_pairs = opt->part_pairs > 0 ? opt->part_pairs : 2048;
   p->q_wpc = opt->waves_per_cu > 0 ? opt->waves_per_cu : 24;
   p->q_walk = opt->queue_cover == PIKO_COVER_WALK ? 1u : 0u;  // AUTO: spans (the batched soups)

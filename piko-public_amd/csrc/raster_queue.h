@@ -57,11 +57,12 @@ struct QState {
 };
 
 // One part item: tile (batch tile numbering), part index, parts of the tile,
-// first scratch slot of the tile's parts (split tiles only).  The split tiles
-// are listed for k_qmerge as their first part's item.
+// first scratch slot of the tile's parts (split tiles only), the tile's queue
+// range (as tile_base / tile_cap) and the part's slice [lo, hi) of it.  The
+// split tiles are listed for k_qmerge as their first part's item.
 struct QItem {
   uint32_t tile, part, nparts, slot;
-  uint32_t base, cap, pad0, pad1;  // the tile's queue range (as tile_base / tile_cap)
+  uint32_t base, cap, lo, hi;
 };
 
 struct QChunk {
@@ -86,7 +87,8 @@ struct QArgs {
   FrameState *fs;
   uint32_t *color;
   float *depth;
-  int njobs, nchunks, nitems, part_pairs, nsplits;
+  unsigned long long *tile_cost;  // the counting pass's output: each tile's estimated raster cost
+  int njobs, nchunks, nitems, nsplits;
 };
 
 // Staged triangles of one wave of the queue raster (structure of arrays, so
@@ -342,9 +344,12 @@ struct __attribute__((packed, aligned(4))) QVec3 {
   float x, y, z;
 };
 
-template <int KIND, bool MULTI>
+// COUNT: the counting pass (the ranges are empty, so every run is refused),
+// which also sums each tile's estimated raster cost into qa.tile_cost.
+template <int KIND, bool MULTI, bool COUNT>
 __global__ void __launch_bounds__(kQThreads) k_qbin(QArgs qa, FrameArgs fa) {
   __shared__ uint32_t cnt[kQMaxTiles];
+  __shared__ uint32_t cst[COUNT ? kQMaxTiles : 1];  // the chunk's raster cost estimate per tile
   const int t = threadIdx.x;
   if (blockIdx.x == 0 && t < kQQueues) qa.qs->next[t * kQHeadStride] = 0;  // the raster launch's queue heads
   if ((int)blockIdx.x >= qa.nchunks) return;       // a frame without triangles runs one block
@@ -352,7 +357,11 @@ __global__ void __launch_bounds__(kQThreads) k_qbin(QArgs qa, FrameArgs fa) {
   const FrameArgs &ja = MULTI ? qa.jobs[cd.job] : fa;
   const int ntiles = ja.nBx * ja.nBy;
   const int S = ja.S, lgT = ja.lgT, bsb = S + lgT;
-  for (int q = t; q < ntiles; q += kQThreads) cnt[q] = 0;
+  constexpr bool costs = COUNT;
+  for (int q = t; q < ntiles; q += kQThreads) {
+    cnt[q] = 0;
+    if (costs) cst[q] = 0;
+  }
   // all of the thread's vertex positions first, so their loads overlap: one
   // 12-byte load per vertex where nine 4-byte loads touched the same cache
   // lines nine times over (a triangle's floats are only 4-byte aligned)
@@ -396,6 +405,14 @@ __global__ void __launch_bounds__(kQThreads) k_qbin(QArgs qa, FrameArgs fa) {
       for (int tx = tx0; tx <= tx1; ++tx)
         if (q_tile_ok(e, tx, ty, one, bsb)) {
           atomicAdd(&cnt[ty * ja.nBx + tx], 1u);
+          if (costs) {
+            // a part's time per entry: a fixed share plus its bbox columns
+            // and pixels in the tile (fragments ~ a fraction of the pixels),
+            // in units of about a quarter fragment
+            const int wx = min(x1, (tx + 1) << lgT) - max(x0, tx << lgT);
+            const int hy = min(y1, (ty + 1) << lgT) - max(y0, ty << lgT);
+            atomicAdd(&cst[ty * ja.nBx + tx], (uint32_t)(32 + 2 * wx + wx * hy));
+          }
           any = true;
         }
     if (!any) continue;
@@ -409,6 +426,7 @@ __global__ void __launch_bounds__(kQThreads) k_qbin(QArgs qa, FrameArgs fa) {
     const uint32_t n = cnt[q];
     if (!n) continue;
     const uint32_t gt = (MULTI ? ja.bin_base : 0u) + (uint32_t)q;
+    if (costs) atomicAdd(&qa.tile_cost[gt], (unsigned long long)cst[q]);
     const uint32_t o = atomicAdd(&qa.tile_fill[gt], n);
     if ((unsigned long long)o + n <= qa.tile_cap[gt]) {
       cnt[q] = qa.tile_base[gt] + o;
@@ -417,6 +435,7 @@ __global__ void __launch_bounds__(kQThreads) k_qbin(QArgs qa, FrameArgs fa) {
       atomicOr(&qa.fs->overflow, 8u);
     }
   }
+  if (costs) return;  // the counting pass: every run was refused
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < kQPer; ++k) {
@@ -479,8 +498,8 @@ __global__ void __launch_bounds__(kQThreads) k_qraster(QArgs qa, FrameArgs fa) {
     for (int q = t; q < T * T; q += kQThreads) vis[q] = kVisSentinel;
     // the part's slice of the tile's range: its entries load beside the fill
     // count (a slot past the count holds a stale index and is dropped below)
-    const uint32_t pbeg = min(it.cap, it.part * (uint32_t)qa.part_pairs);
-    const uint32_t pend = min(it.cap, pbeg + (uint32_t)qa.part_pairs);
+    const uint32_t pbeg = min(it.cap, it.lo);
+    const uint32_t pend = min(it.cap, it.hi);
     const uint32_t *list = qa.pairs + it.base;
     const uint32_t fill = qa.tile_fill[gt];
     // Every load of the pipeline below is issued unconditionally (an index
