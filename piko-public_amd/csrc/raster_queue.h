@@ -105,7 +105,8 @@ struct QArgs {
 //            all drawn, its first fragment's number in the wave's running
 //            count, slot | column << 6 | first covered row << 12 (tile-local),
 //            and edge functions 1 and 2 at that pixel
-//   mark     the owner marks of a window of 64 items
+//   mark     the owner marks of a window of 64 items (span path: lane
+//            numbers; column walk: the window's sequence number)
 constexpr uint32_t kQRing = 128;
 struct QStage {
   uint4 fa[64];
@@ -470,6 +471,10 @@ __global__ void __launch_bounds__(kQThreads) k_qraster(QArgs qa, FrameArgs fa) {
   unsigned long long *const vis = sm.vis;
   const int t = threadIdx.x, lane = t & 63;
   for (int q = t; q < 256; q += kQThreads) sm.u8[q] = kU8Table.v[q];
+  // The column walk's windows tag their marks with a sequence number from 64
+  // up, above the lane numbers the span path's owner search leaves there.
+  sm.st[t >> 6].mark[lane] = ~0u;
+  uint32_t wseq = 64;
   // The first item of a workgroup is its own index; later ones come from its
   // XCD's queue (item i is position i / nqs of queue i % nqs), one global add
   // each, taken only when the workgroup is free.  Taking the next one earlier
@@ -532,33 +537,33 @@ __global__ void __launch_bounds__(kQThreads) k_qraster(QArgs qa, FrameArgs fa) {
 #pragma unroll
         for (int q = 0; q < 3; ++q) out[q] = g[q];
       };
-      // The wave's column ring: records [head, tail) are live, fragments
-      // [0, ftot) appended in record order, [0, fcons) drawn.  Fragments are
-      // drawn 64 at a time, so every window but a batch's last is full.
-      uint32_t head = 0, tail = 0, ftot = 0, fcons = 0;
-      // Draws fragments [fcons, fcons + 64) (those below ftot).  The record of
-      // each is the last one starting at or before it: every record starting
-      // inside the window marks its first fragment, the record holding
-      // fragment fcons - 1 (or the first record) marks position 0 beforehand,
-      // and a max-scan carries each mark up to the next.  Live records number
-      // at most 64 before an append of at most 64, so they fit the ring and
-      // two looks per lane cover them.
+      // The wave's column ring: records (head, tail) hold fragments not yet
+      // drawn, `head` the record holding fragment fcons - 1 (at a batch's
+      // start, the record before its first); fragments [0, ftot) appended in
+      // record order, [0, fcons) drawn.  Fragments are drawn 64 at a time, so
+      // every window but a batch's last is full.
+      uint32_t head = ~0u, tail = 0, ftot = 0, fcons = 0;
+      // Draws fragments [fcons, fcons + 64) (those below ftot).  The records
+      // starting inside the window are head + 1, head + 2, ... (each holds at
+      // least one fragment), at most 64 of them: each tags the mark of its
+      // first fragment with the window's sequence number, a ballot of the
+      // tagged marks gives the window's record starts, and a fragment's record
+      // is head plus the starts at or below it.  Live records number at most
+      // 64 before an append of at most 64, so they fit the ring.
       auto window = [&]() {
-        s.mark[lane] = lane == 0 ? head : ~0u;
-        wave_fence();
-#pragma unroll
-        for (uint32_t q = 0; q < kQRing; q += 64) {
-          const uint32_t idx = head + q + (uint32_t)lane;
-          if (idx < tail) {
-            const uint32_t d = s.rec[idx & (kQRing - 1)].x - fcons;
-            if (d < 64u) s.mark[d] = idx;
-          }
+        const uint32_t idx = head + 1u + (uint32_t)lane;
+        if (idx < tail) {
+          const uint32_t d = s.rec[idx & (kQRing - 1)].x - fcons;
+          if (d < 64u) s.mark[d] = wseq;
         }
         wave_fence();
-        const int o = wave_max_scan_dpp((int)s.mark[lane]);
+        const unsigned long long starts = __builtin_amdgcn_ballot_w64(s.mark[lane] == wseq);
+        const uint32_t o = head + (uint32_t)((starts >> lane) & 1ull) +
+                           __builtin_amdgcn_mbcnt_hi((uint32_t)(starts >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)starts, 0u));
         const uint32_t f = fcons + (uint32_t)lane;
         if (f < ftot) {
-          const uint4 rc = s.rec[(uint32_t)o & (kQRing - 1)];
+          const uint4 rc = s.rec[o & (kQRing - 1)];
           const uint32_t r = f - rc.x;
           const uint32_t slot = rc.y & 63u;
           const uint4 a = s.fa[slot];
@@ -574,8 +579,8 @@ __global__ void __launch_bounds__(kQThreads) k_qraster(QArgs qa, FrameArgs fa) {
           atomicMin((long long *)(vis + (ly << ja.lgT) + lx),
                     (long long)(((unsigned long long)__float_as_uint(z) << 32) | (unsigned long long)b.z));
         }
-        head = (uint32_t)__builtin_amdgcn_readlane(o, 63);
-        wave_fence();  // the marks are read before the next window resets them
+        head += (uint32_t)__builtin_popcountll(starts);
+        ++wseq;
       };
       // software pipeline: the setup records of the next batch and the
       // entries of the one after are in flight while a batch is rasterised
@@ -751,8 +756,7 @@ __global__ void __launch_bounds__(kQThreads) k_qraster(QArgs qa, FrameArgs fa) {
           }
         }
         if (ftot != fcons) window();  // the batch's last fragments: its slots are restaged next
-        head = tail;
-        fcons = ftot;
+        fcons = ftot;  // head is now the batch's last record
         wave_fence();  // the staged triangles are read before the next batch stages
       }
     }
