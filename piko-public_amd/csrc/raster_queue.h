@@ -100,13 +100,14 @@ struct QArgs {
 //            (x2, y2, x | y << 16, w | h << 16 | int32 edges << 31), read by
 //            the span path only, which runs while the ring is empty (so the
 //            two share their memory)
-//   pre      exclusive prefix of the span counts (span path)
+//   pre      the span path's nonempty runs in lane order: first span << 6 |
+//            lane
 //   rec      the column ring: per nonempty bbox column whose fragments are not
 //            all drawn, its first fragment's number in the wave's running
 //            count, slot | column << 6 | first covered row << 12 (tile-local),
 //            and edge functions 1 and 2 at that pixel
-//   mark     the owner marks of a window of 64 items (span path: lane
-//            numbers; column walk: the window's sequence number)
+//   mark     the marks of a window of 64 spans or fragments, tagged with
+//            the window's sequence number
 constexpr uint32_t kQRing = 128;
 struct QStage {
   uint4 fa[64];
@@ -207,23 +208,6 @@ __device__ __forceinline__ int wave_max_scan_dpp(int x) {
   x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x142, 0xa, 0xf, false));  // row_bcast:15 -> rows 1, 3
   x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x143, 0xc, 0xf, false));  // row_bcast:31 -> rows 2, 3
   return x;
-}
-
-// The wave's items are cut into consecutive runs, one per lane: lane i owns
-// [start_i, start_i + len_i) (starts non-decreasing with i, empty runs
-// allowed).  Returns the lane owning item w0 + lane, for every item of the
-// window [w0, w0 + 64) below the total (beyond it, the last run's lane).
-// Each run that reaches into the window marks its first item there in
-// `mark` (64 words of the wave's own), and a max-scan carries each mark up
-// to the next: two LDS round trips, where a binary search over the starts
-// costs six dependent ones.  Every lane must be active.
-__device__ __forceinline__ int wave_owner(uint32_t start, uint32_t len, uint32_t w0, uint32_t *mark) {
-  const int lane = (int)(threadIdx.x & 63);
-  mark[lane] = ~0u;
-  wave_fence();
-  if (len && start < w0 + 64u && start + len > w0) mark[(start > w0 ? start : w0) - w0] = (uint32_t)lane;
-  wave_fence();
-  return wave_max_scan_dpp((int)mark[lane]);
 }
 
 // floor(n / d) and its remainder n - d * floor(n / d) in [0, d), exact, for
@@ -471,10 +455,10 @@ __global__ void __launch_bounds__(kQThreads) k_qraster(QArgs qa, FrameArgs fa) {
   unsigned long long *const vis = sm.vis;
   const int t = threadIdx.x, lane = t & 63;
   for (int q = t; q < 256; q += kQThreads) sm.u8[q] = kU8Table.v[q];
-  // The column walk's windows tag their marks with a sequence number from 64
-  // up, above the lane numbers the span path's owner search leaves there.
+  // Windows of spans and of column-walk fragments tag their marks with a
+  // sequence number, never ~0u.
   sm.st[t >> 6].mark[lane] = ~0u;
-  uint32_t wseq = 64;
+  uint32_t wseq = 0;
   // The first item of a workgroup is its own index; later ones come from its
   // XCD's queue (item i is position i / nqs of queue i % nqs), one global add
   // each, taken only when the workgroup is free.  Taking the next one earlier
@@ -711,11 +695,28 @@ __global__ void __launch_bounds__(kQThreads) k_qraster(QArgs qa, FrameArgs fa) {
         const uint32_t sinc = wave_scan_dpp(uspan);
         const uint32_t stot = wave_last(sinc);
         if (stot) {
-          s.pre[lane] = sinc - uspan;
+          // Lane i's spans are [start, start + uspan).  The nonempty runs are
+          // listed in lane order; those starting inside a window of 64 spans
+          // follow the run holding the window's first span minus one, and
+          // are found as the column walk finds its records (tagged marks and
+          // a ballot).
+          const uint32_t start = sinc - uspan;
+          const unsigned long long nz = __builtin_amdgcn_ballot_w64(uspan != 0u);
+          if (uspan) s.pre[__builtin_amdgcn_mbcnt_hi((uint32_t)(nz >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)nz, 0u))] =
+              start << 6 | (uint32_t)lane;
+          uint32_t base = ~0u;  // the run holding span c0 - 1
           for (uint32_t c0 = 0; c0 < stot; c0 += 64) {
+            if (uspan && start - c0 < 64u) s.mark[start - c0] = wseq;
+            wave_fence();
+            const unsigned long long st = __builtin_amdgcn_ballot_w64(s.mark[lane] == wseq);
+            const uint32_t rank = base + (uint32_t)((st >> lane) & 1ull) +
+                                  __builtin_amdgcn_mbcnt_hi((uint32_t)(st >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)st, 0u));
+            const uint32_t run = s.pre[rank & 63u];
+            base += (uint32_t)__builtin_popcountll(st);
+            ++wseq;
             const uint32_t cidx = c0 + (uint32_t)lane;
-            const int o = wave_owner(sinc - uspan, uspan, c0, s.mark);  // fences s.pre too
-            if (cidx < stot) q_span(s, o, cidx - s.pre[o], S, tl.xlo, tl.ylo, T, vis);
+            if (cidx < stot) q_span(s, (int)(run & 63u), cidx - (run >> 6), S, tl.xlo, tl.ylo, T, vis);
           }
           wave_fence();
         }
