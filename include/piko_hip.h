@@ -122,7 +122,8 @@ typedef struct {
   long long pair_capacity;/* (triangle, bin) queue capacity; 0 = automatic,
                              grows on demand between frames */
   int waves_per_cu;       /* persistent raster waves per CU; 0 = default (20;
-                             QUEUE: 20, five 256-thread workgroups) */
+                             QUEUE: 20, five 256-thread workgroups, at most
+                             the workgroups a CU holds at once) */
   int raster_mode;        /* PIKO_RASTER_*; 0 = AUTO */
   int part_pairs;         /* QUEUE: queue entries per part item; 0 = 2048 */
   int queue_cover;        /* QUEUE: PIKO_COVER_*; 0 = AUTO (WALK) */
@@ -236,7 +237,9 @@ typedef struct {
   long long pair_capacity; /* 0 = automatic */
   int raster_mode;         /* PIKO_RASTER_* for every batch; 0 = AUTO */
   int part_pairs;          /* QUEUE: queue entries per part item; 0 = 2048 */
-  int waves_per_cu;        /* QUEUE: persistent waves per CU; 0 = 24 */
+  int waves_per_cu;        /* QUEUE: persistent waves per CU, at most what a CU
+                              holds at once (three workgroups at 64-px tiles);
+                              0 = 24 */
   int queue_cover;         /* QUEUE: PIKO_COVER_* of jobs that name none; 0 = AUTO (SPANS) */
 } piko_batch_options;
 

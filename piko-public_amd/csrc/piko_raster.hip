@@ -2294,6 +2294,20 @@ void enqueue_tiled(piko_raster_pipe *p, Marks &mark) {
 
 // The queue frame's launches: the bin launch, then (raster) the persistent
 // raster launch over the part items and the merge of the split tiles.
+
+// Workgroups of one k_qraster instance that a CU holds at once (registers
+// and LDS), asked of the runtime once per instance.
+template <int QT, int KIND, bool MULTI>
+int q_resident() {
+  static const int n = [] {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_qraster<QT, KIND, MULTI>, kQThreads, 0) != hipSuccess)
+      (void)hipGetLastError();
+    return b > 0 ? b : 1 << 20;
+  }();
+  return n;
+}
+
 void launch_queue(piko_raster_pipe *p, bool raster, Marks *mark) {
   QArgs qa{p->d_pos,    p->d_attr,  p->d_geo,    p->d_qpairs,   p->d_qbase, p->d_qcap,
            p->d_qfill,  p->d_qneed, p->d_qitems, p->d_qsplits,  p->d_qchunks, p->d_qscr,
@@ -2315,11 +2329,15 @@ void launch_queue(piko_raster_pipe *p, bool raster, Marks *mark) {
   if (mark) (*mark)();
   if (!raster) return;
   const long long per_cu = std::max(1, p->q_wpc / (kQThreads / 64));
-  const int rgrid = (int)std::max<long long>(1, std::min<long long>(p->q_nitems, (long long)p->cus * per_cu));
+  // the persistent grid never exceeds what the CUs hold at once
+  auto rgrid = [&](int resident) {
+    return (int)std::max<long long>(
+        1, std::min<long long>(p->q_nitems, (long long)p->cus * std::min<long long>(per_cu, resident)));
+  };
 #define PIKO_QR_LAUNCH(QT, KIND)                                                                    \
   do {                                                                                              \
-    if (p->multi) k_qraster<QT, KIND, true><<<rgrid, kQThreads, 0, p->stream>>>(qa, fa);           \
-    else k_qraster<QT, KIND, false><<<rgrid, kQThreads, 0, p->stream>>>(qa, fa);                   \
+    if (p->multi) k_qraster<QT, KIND, true><<<rgrid(q_resident<QT, KIND, true>()), kQThreads, 0, p->stream>>>(qa, fa); \
+    else k_qraster<QT, KIND, false><<<rgrid(q_resident<QT, KIND, false>()), kQThreads, 0, p->stream>>>(qa, fa); \
     if (mark) (*mark)();                                                                            \
     if (p->q_nsplits > 0) {                                                                         \
       if (p->multi) k_qmerge<QT, KIND, true><<<p->q_nsplits, kQMergeThreads, 0, p->stream>>>(qa, fa); \
