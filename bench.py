@@ -533,6 +533,10 @@ def main():
         "config": {"workload": "heightfield-2M-1080p raster (VS->bin->raster->depth<=->Gouraud)",
                    "triangles": scene.n, "W": scene.W, "H": scene.H, "subpixel_bits": S, "raster_mode": mode,
                    "bin_size": None if mode == "direct" else T,
+                   # run_frames keeps two frames in flight on the queue path
+                   # (two streams, two sets of frame buffers); the event times
+                   # below are of one frame alone
+                   "frames_in_flight": 2 if mode == "queue" else 1,
                    "emitted": st["emitted"], "pairs": st["pairs"],
                    "base_freq": scenes.HF_FREQ, "eye": list(scenes.HF_EYE), "target": list(scenes.HF_TARGET),
                    # the scene's fragment statistics (oracle coverage counts, recorded
@@ -548,7 +552,10 @@ def main():
                      "traffic_source": traffic_src},
         "frame_roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(achieved / HBM_PEAK_GBS, 4), "event_frame_ms": round(frame_ms, 4),
-                           "launches": len(kernels), "algorithmic_bytes_per_frame": alg_bytes},
+                           "launches": len(kernels), "algorithmic_bytes_per_frame": alg_bytes,
+                           # the same bytes over the timed steps' wall clock per frame
+                           "achieved_per_step": round(alg_bytes / dt / 1e9, 2),
+                           "frac_per_step": round(alg_bytes / dt / 1e9 / HBM_PEAK_GBS, 4)},
     }
     if not args.no_cpu:
         log("timing the CPU oracle on a bounded sample")

@@ -156,8 +156,13 @@ int piko_raster_prepare(piko_raster_pipe *pipe, const piko_constant_state *cs);
 /* RasterPipe::run_single (PTXBackend.cpp:811-829): one frame, synchronous. */
 int piko_raster_run_single(piko_raster_pipe *pipe);
 
-/* nframes frames back to back on the pipe's stream, one synchronisation at
- * the end (throughput measurement). */
+/* nframes frames back to back, one synchronisation at the end (throughput
+ * measurement).  On the queue path a run of two or more frames keeps two
+ * frames in flight: the first such run makes a twin of the pipe (its own
+ * stream and frame buffers over the same input, about the pipe's device
+ * memory again) and frames alternate between the two; the last frame's
+ * image ends in the pipe's buffers.  Elsewhere frames run on the pipe's
+ * stream one after another. */
 int piko_raster_run_frames(piko_raster_pipe *pipe, int nframes);
 
 /* Like run_frames, timing each kernel of the frame with HIP events on the

@@ -1601,6 +1601,13 @@ struct RasterClass {
 
 struct piko_raster_pipe {
   bool multi = false;
+  // A plain queue pipe's second set of frame buffers and stream, made on the
+  // first run of two or more frames: run_frames keeps two frames in flight,
+  // alternating between the pipe and its twin (see piko_raster_run_frames).
+  piko_raster_pipe *twin = nullptr;
+  bool twin_tried = false;
+  piko_constant_state cs_cur{};  // the plain pipe's current uniforms and options (for the twin)
+  piko_raster_options opt0{};
   int kind = 0, S = 0;
   int W = 0, H = 0, T = 0, lgT = 0, nBx = 0, nBy = 0;  // plain pipe view
   FrameArgs fa{};                                         // plain pipe job
@@ -2640,8 +2647,30 @@ int plain_pipe(piko_raster_pipe **out, const piko_constant_state *cs, const floa
     piko_raster_destroy(p);
     return rc;
   }
+  p->cs_cur = *cs;
+  p->opt0 = *opt;
   *out = p;
   return PIKO_OK;
+}
+
+// The twin of a plain queue pipe: the same options and uniforms over the
+// same input (not copied), with its own frame buffers and stream.  Made once;
+// where it cannot be (device memory), frames run one after another.
+piko_raster_pipe *frame_twin(piko_raster_pipe *p) {
+  if (p->twin) return p->twin;
+  if (p->twin_tried || p->multi || p->mode != PIKO_RASTER_QUEUE || p->input_tris <= 0) return nullptr;
+  p->twin_tried = true;
+  piko_raster_pipe *t = nullptr;
+  if (plain_pipe(&t, &p->cs_cur, p->d_pos, p->d_attr, p->input_tris, &p->opt0, true) != PIKO_OK) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  if (t->mode != p->mode) {
+    piko_raster_destroy(t);
+    return nullptr;
+  }
+  p->twin = t;
+  return t;
 }
 
 // ---- allocation-free frame ---------------------------------------------------
@@ -2713,6 +2742,7 @@ const char *piko_status_string(int s) {
 
 int piko_raster_destroy(piko_raster_pipe *p) {
   if (!p) return PIKO_ERR_INVALID;
+  if (p->twin) piko_raster_destroy(p->twin);
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   free_pairs(p);
   free_tris(p);
@@ -2759,20 +2789,36 @@ int piko_raster_prepare(piko_raster_pipe *p, const piko_constant_state *cs) {
     ft.tab_base = p->fa_t.tab_base;
     p->fa_t = ft;
   }
+  p->cs_cur = *cs;
+  if (p->twin) return piko_raster_prepare(p->twin, cs);
   return PIKO_OK;
 }
 
+// Frames of the current state.  On the queue path a run of two or more
+// frames alternates between the pipe and its twin, each on its own stream and
+// frame buffers, so two frames are in flight: one frame's bin launch runs
+// beside the other's raster and fills the CU slots its last workgroups leave
+// (2M-triangle height field: 113 -> 93 us per frame in two independent pipes
+// at once).  The last frame's image ends in the pipe's buffers.
 int piko_raster_run_frames(piko_raster_pipe *p, int nframes) {
   if (!p || nframes < 0) return PIKO_ERR_INVALID;
+  piko_raster_pipe *q = nframes >= 2 ? frame_twin(p) : nullptr;
   for (int attempt = 0; attempt < 3; ++attempt) {
     for (int f = 0; f < nframes; ++f) {
-      int rc = enqueue_frame(p, nullptr, nullptr);
+      int rc = enqueue_frame(q && (f & 1) ? q : p, nullptr, nullptr);
       if (rc != PIKO_OK) return rc;
     }
-    int rerun = 0;
-    int rc = check_capacity(p, &rerun);
+    int rerun = 0, rerun_q = 0;
+    int rc = check_capacity(p, &rerun);  // waits for the pipe's frames
+    if (rc == PIKO_OK && q) rc = check_capacity(q, &rerun_q);  // and the twin's
     if (rc != PIKO_OK) return rc;
-    if (!rerun) return PIKO_OK;
+    if (rerun || rerun_q) continue;
+    if (q && !(nframes & 1)) {  // the last frame ran on the twin
+      HIPOK(hipMemcpyAsync(p->d_color, q->d_color, (size_t)p->npix * 4, hipMemcpyDeviceToDevice, p->stream));
+      HIPOK(hipMemcpyAsync(p->d_depth, q->d_depth, (size_t)p->npix * 4, hipMemcpyDeviceToDevice, p->stream));
+      HIPOK(hipStreamSynchronize(p->stream));
+    }
+    return PIKO_OK;
   }
   return PIKO_ERR_CAPACITY;
 }

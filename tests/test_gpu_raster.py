@@ -219,6 +219,23 @@ def test_queue_frames_across_camera_changes(oracle_mod, T, cover):
             assert np.array_equal(c, oc) and np.array_equal(d.view(np.int32), od.view(np.int32)), step
 
 
+@pytest.mark.parametrize("wpc", [1, 8, 64])
+def test_queue_waves_per_cu(oracle_mod, wpc):
+    # the persistent raster grid at one workgroup per CU, at two, and at a
+    # request above what a CU holds (capped to the resident workgroups), with
+    # small parts so that most workgroups take many items: the same image
+    import piko_amd
+    sc = scenes.heightfield(seed=4, grid=120, W=640, H=360)
+    oc, od, _ = oracle_mod.frame(sc, 8, 32)
+    log("queue waves_per_cu=%d" % wpc)
+    with piko_amd.RasterPipe().allocate(sc, 32, 8, raster_mode=piko_amd.RASTER_QUEUE, waves_per_cu=wpc,
+                                        part_pairs=128) as pipe:
+        pipe.prepare()
+        pipe.run_frames(2)
+        c, d = pipe.pikoScreen.getData(), pipe.get_depth()
+        assert np.array_equal(c, oc) and np.array_equal(d.view(np.int32), od.view(np.int32))
+
+
 def test_tiled_large_lists(oracle_mod):
     # every triangle spans more than 2x2 tiles of 16 pixels: all of them go to
     # the chunks' large lists and are clipped to each tile they touch
