@@ -160,9 +160,11 @@ int piko_raster_run_single(piko_raster_pipe *pipe);
  * measurement).  On the queue path a run of two or more frames keeps two
  * frames in flight: the first such run makes a twin of the pipe (its own
  * stream and frame buffers over the same input, about the pipe's device
- * memory again) and frames alternate between the two; the last frame's
- * image ends in the pipe's buffers.  Elsewhere frames run on the pipe's
- * stream one after another. */
+ * memory again) and frames alternate between the two; these frames take
+ * whole tiles as part items and a persistent grid of 12 waves per CU (the
+ * options' part_pairs and waves_per_cu apply to single frames); the last
+ * frame's image ends in the pipe's buffers.  Elsewhere frames run on the
+ * pipe's stream one after another. */
 int piko_raster_run_frames(piko_raster_pipe *pipe, int nframes);
 
 /* Like run_frames, timing each kernel of the frame with HIP events on the

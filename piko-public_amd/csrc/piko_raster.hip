@@ -1680,6 +1680,13 @@ struct piko_raster_pipe {
   QItem *d_qitems = nullptr;
   long long cap_qitems = 0;
   int q_nitems = 0;
+  // Frames in flight (a plain pipe and its twin in run_frames): one part per
+  // tile, costliest first, and a persistent grid of kQFramesWpc waves per CU,
+  // which leaves CU slots for the other frame's launches.
+  bool q_frames = false;                // launch with the frames-in-flight items and grid
+  QItem *d_qitems_f = nullptr;
+  long long cap_qitems_f = 0;
+  int q_nitems_f = 0;
   uint32_t *d_qsplits = nullptr;        // first part's item of each split tile
   long long cap_qsplits = 0;
   int q_nsplits = 0;
@@ -1927,6 +1934,7 @@ void free_queue(piko_raster_pipe *p) {
   q_free(p, p->d_qcost, tc);
   p->cap_qtiles = 0;
   q_free(p, p->d_qitems, p->cap_qitems);
+  q_free(p, p->d_qitems_f, p->cap_qitems_f);
   q_free(p, p->d_qsplits, p->cap_qsplits);
   q_free(p, p->d_qpairs, p->cap_qpairs);
   q_free(p, p->d_qscr, p->cap_qscr);
@@ -1982,10 +1990,23 @@ int q_build(piko_raster_pipe *p) {
                             (uint32_t)(cap * (k + 1) / nparts)});
   }
   if (items.size() > 0x7FFFFFFFull || slots > 0xFFFFFFFFll) return PIKO_ERR_CAPACITY;
+  // the frames-in-flight items (plain pipes): whole tiles, costliest first
+  std::vector<QItem> items_f;
+  if (!p->multi) {
+    std::vector<int> ord((size_t)n);
+    for (int t = 0; t < n; ++t) ord[t] = t;
+    auto cost = [&](int t) {
+      return total_cost > 0 ? (t < (int)p->q_cost.size() ? p->q_cost[t] : 0ull) : (unsigned long long)p->q_cap[t];
+    };
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return cost(a) > cost(b); });
+    items_f.reserve((size_t)n);
+    for (int t : ord) items_f.push_back(QItem{(uint32_t)t, 0u, 1u, 0u, base[t], p->q_cap[t], 0u, p->q_cap[t]});
+  }
   int rc;
   HIPOK(hipStreamSynchronize(p->stream));  // buffers may still be in use
   if ((rc = q_grow(p, p->d_qpairs, p->cap_qpairs, (long long)total)) != PIKO_OK) return rc;
   if ((rc = q_grow(p, p->d_qitems, p->cap_qitems, (long long)items.size())) != PIKO_OK) return rc;
+  if ((rc = q_grow(p, p->d_qitems_f, p->cap_qitems_f, (long long)items_f.size())) != PIKO_OK) return rc;
   if ((rc = q_grow(p, p->d_qscr, p->cap_qscr, slots * p->q_QT * p->q_QT)) != PIKO_OK) return rc;
   if ((rc = q_grow(p, p->d_qsplits, p->cap_qsplits, (long long)splits.size())) != PIKO_OK) return rc;
   if (n > 0) {
@@ -1997,8 +2018,12 @@ int q_build(piko_raster_pipe *p) {
                          p->stream));
   if (!splits.empty())
     HIPOK(hipMemcpyAsync(p->d_qsplits, splits.data(), splits.size() * 4, hipMemcpyHostToDevice, p->stream));
+  if (!items_f.empty())
+    HIPOK(hipMemcpyAsync(p->d_qitems_f, items_f.data(), items_f.size() * sizeof(QItem), hipMemcpyHostToDevice,
+                         p->stream));
   p->q_nitems = (int)items.size();
   p->q_nsplits = (int)splits.size();
+  p->q_nitems_f = (int)items_f.size();
   HIPOK(hipStreamSynchronize(p->stream));  // the host vectors go out of scope
   return PIKO_OK;
 }
@@ -2299,6 +2324,12 @@ void enqueue_tiled(piko_raster_pipe *p, Marks &mark) {
   }
 }
 
+// Persistent raster waves per CU with two frames in flight: three 256-thread
+// workgroups of one frame's raster leave two slots per CU to the other
+// frame's launches (2M height field with whole-tile items: 12 waves 81 us
+// per frame, 16 85 us, 20 87 us, 8 93 us).
+constexpr int kQFramesWpc = 12;
+
 // The queue frame's launches: the bin launch, then (raster) the persistent
 // raster launch over the part items and the merge of the split tiles.
 
@@ -2316,11 +2347,13 @@ int q_resident() {
 }
 
 void launch_queue(piko_raster_pipe *p, bool raster, Marks *mark) {
+  const bool fr = p->q_frames && !p->multi;
+  const int nitems = fr ? p->q_nitems_f : p->q_nitems, nsplits = fr ? 0 : p->q_nsplits;
   QArgs qa{p->d_pos,    p->d_attr,  p->d_geo,    p->d_qpairs,   p->d_qbase, p->d_qcap,
-           p->d_qfill,  p->d_qneed, p->d_qitems, p->d_qsplits,  p->d_qchunks, p->d_qscr,
+           p->d_qfill,  p->d_qneed, fr ? p->d_qitems_f : p->d_qitems, p->d_qsplits,  p->d_qchunks, p->d_qscr,
            p->d_jobs,   p->d_qs,    p->d_fs,     p->d_color,    p->d_depth,
            raster ? nullptr : p->d_qcost,
-           (int)p->jobs.size(), p->q_nchunks, p->q_nitems, p->q_nsplits};
+           (int)p->jobs.size(), p->q_nchunks, nitems, nsplits};
   const FrameArgs &fa = p->fa_q;
   const unsigned bgrid = (unsigned)std::max(p->q_nchunks, 1);
 #define PIKO_QB_LAUNCH(KIND)                                                                          \
@@ -2335,20 +2368,20 @@ void launch_queue(piko_raster_pipe *p, bool raster, Marks *mark) {
 #undef PIKO_QB_LAUNCH
   if (mark) (*mark)();
   if (!raster) return;
-  const long long per_cu = std::max(1, p->q_wpc / (kQThreads / 64));
+  const long long per_cu = std::max(1, (fr ? kQFramesWpc : p->q_wpc) / (kQThreads / 64));
   // the persistent grid never exceeds what the CUs hold at once
   auto rgrid = [&](int resident) {
     return (int)std::max<long long>(
-        1, std::min<long long>(p->q_nitems, (long long)p->cus * std::min<long long>(per_cu, resident)));
+        1, std::min<long long>(nitems, (long long)p->cus * std::min<long long>(per_cu, resident)));
   };
 #define PIKO_QR_LAUNCH(QT, KIND)                                                                    \
   do {                                                                                              \
     if (p->multi) k_qraster<QT, KIND, true><<<rgrid(q_resident<QT, KIND, true>()), kQThreads, 0, p->stream>>>(qa, fa); \
     else k_qraster<QT, KIND, false><<<rgrid(q_resident<QT, KIND, false>()), kQThreads, 0, p->stream>>>(qa, fa); \
     if (mark) (*mark)();                                                                            \
-    if (p->q_nsplits > 0) {                                                                         \
-      if (p->multi) k_qmerge<QT, KIND, true><<<p->q_nsplits, kQMergeThreads, 0, p->stream>>>(qa, fa); \
-      else k_qmerge<QT, KIND, false><<<p->q_nsplits, kQMergeThreads, 0, p->stream>>>(qa, fa);         \
+    if (nsplits > 0) {                                                                              \
+      if (p->multi) k_qmerge<QT, KIND, true><<<nsplits, kQMergeThreads, 0, p->stream>>>(qa, fa);     \
+      else k_qmerge<QT, KIND, false><<<nsplits, kQMergeThreads, 0, p->stream>>>(qa, fa);             \
       if (mark) (*mark)();                                                                          \
     }                                                                                               \
   } while (0)
@@ -2797,17 +2830,25 @@ int piko_raster_prepare(piko_raster_pipe *p, const piko_constant_state *cs) {
 // Frames of the current state.  On the queue path a run of two or more
 // frames alternates between the pipe and its twin, each on its own stream and
 // frame buffers, so two frames are in flight: one frame's bin launch runs
-// beside the other's raster and fills the CU slots its last workgroups leave
-// (2M-triangle height field: 113 -> 93 us per frame in two independent pipes
-// at once).  The last frame's image ends in the pipe's buffers.
+// beside the other's raster.  These frames take whole tiles as items and a
+// persistent grid of kQFramesWpc waves per CU, which leaves CU slots to the
+// other frame (2M-triangle height field: 117 us per frame one at a time, 93
+// in flight with the single-frame items and grid, 80 with these).  The last
+// frame's image ends in the pipe's buffers.
 int piko_raster_run_frames(piko_raster_pipe *p, int nframes) {
   if (!p || nframes < 0) return PIKO_ERR_INVALID;
   piko_raster_pipe *q = nframes >= 2 ? frame_twin(p) : nullptr;
   for (int attempt = 0; attempt < 3; ++attempt) {
+    p->q_frames = q != nullptr;
+    if (q) q->q_frames = true;
     for (int f = 0; f < nframes; ++f) {
       int rc = enqueue_frame(q && (f & 1) ? q : p, nullptr, nullptr);
-      if (rc != PIKO_OK) return rc;
+      if (rc != PIKO_OK) {
+        p->q_frames = false;
+        return rc;
+      }
     }
+    p->q_frames = false;  // single frames keep the split parts and the wider grid
     int rerun = 0, rerun_q = 0;
     int rc = check_capacity(p, &rerun);  // waits for the pipe's frames
     if (rc == PIKO_OK && q) rc = check_capacity(q, &rerun_q);  // and the twin's
