@@ -426,7 +426,7 @@ def main():
     ap.add_argument("--workload", default="all", choices=["raster", "batch", "serve", "reyes", "all", "batch-queue"],
                     help="batch-queue: only the 256-scene batch on the queue path at --batch-queue settings "
                          "(for kernel traces: one bin, one raster and one merge launch per frame)")
-    ap.add_argument("--batch-queue", default="64,24,2048",
+    ap.add_argument("--batch-queue", default="64,24,65536",
                     help="tile size, waves per CU and part size of the batch-queue workload")
     ap.add_argument("--reyes-sampling", default="auto", choices=["auto", "global", "tiled"],
                     help="how the Reyes frame composites samples (auto = global)")

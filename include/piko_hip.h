@@ -243,7 +243,9 @@ typedef struct {
   long long max_pixels;    /* initial per-batch pixel capacity (grows) */
   long long pair_capacity; /* 0 = automatic */
   int raster_mode;         /* PIKO_RASTER_* for every batch; 0 = AUTO */
-  int part_pairs;          /* QUEUE: queue entries per part item; 0 = 2048 */
+  int part_pairs;          /* QUEUE: queue entries per part item; 0 = 65536
+                              (whole tiles, no merge launch, at the batch's
+                              tile sizes) */
   int waves_per_cu;        /* QUEUE: persistent waves per CU, at most what a CU
                               holds at once (three workgroups at 64-px tiles);
                               0 = 24 */

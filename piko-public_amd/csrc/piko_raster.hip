@@ -3046,9 +3046,11 @@ int piko_batch_create(piko_raster_pipe **out, const float *pos, const void *attr
   p->multi = true;
   p->input_tris = input_tris;
   p->cap_jobs = opt->max_jobs;
-  // a batch's queue path: longer parts and a wider persistent grid (the
-  // 256-scene batch's tuned setting; a single 1080p frame tunes to 2048 / 20)
-  p->q_part_pairs = opt->part_pairs > 0 ? opt->part_pairs : 2048;
+  // a batch's queue path: whole tiles as parts and a wider persistent grid
+  // (the 256-scene batch: 4.05 ms per batch frame with 65536-entry parts,
+  // 4.07 with 8192, 4.13 with 4096, 4.22 with 2048; a single 1080p frame
+  // tunes to 2048 / 20)
+  p->q_part_pairs = opt->part_pairs > 0 ? opt->part_pairs : 65536;
   p->q_wpc = opt->waves_per_cu > 0 ? opt->waves_per_cu : 24;
   p->q_walk = opt->queue_cover == PIKO_COVER_WALK ? 1u : 0u;  // AUTO: spans (the batched soups)
   const int bins = (int)std::min<long long>(opt->max_pixels / 64 + 1024, 1ll << 30);

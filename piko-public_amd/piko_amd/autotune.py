@@ -129,7 +129,7 @@ def tune_batch(scene_list, pos, attr, parts, S=8, tile_sizes=TILE_SIZES, frames=
 
 
 def tune_queue_batch(scene_list, pos, attr, parts, S=8, tile_sizes=(32, 64), waves=(16, 24),
-                     part_pairs=(1024, 2048), class_tiles=(16, 32, 64), class_covers=None, frames=3, log=None):
+                     part_pairs=(2048, 65536), class_tiles=(16, 32, 64), class_covers=None, frames=3, log=None):
     """The queue path's batch settings, timed in the batch itself.  First one
     tile size for every job with the persistent grid's waves per CU and the
     queue entries per part item (the raster is one persistent launch over

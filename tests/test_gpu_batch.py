@@ -12,13 +12,20 @@ from piko_amd import scenes
 pytestmark = pytest.mark.gpu
 
 # auto: a batch's default (the tiled path); queue: its default coverage
-# (spans for compact triangles), queue-walk: every int32-path triangle walked
+# (spans for compact triangles) and parts (whole tiles), queue-walk: every
+# int32-path triangle walked, queue-parts: tiles cut into parts of 64 entries
+# that meet in the merge launch
 MODES = {"binned": piko_amd.RASTER_BINNED, "direct": piko_amd.RASTER_DIRECT, "tiled": piko_amd.RASTER_TILED,
-         "queue": piko_amd.RASTER_QUEUE, "queue-walk": piko_amd.RASTER_QUEUE, "auto": piko_amd.RASTER_AUTO}
+         "queue": piko_amd.RASTER_QUEUE, "queue-walk": piko_amd.RASTER_QUEUE, "queue-parts": piko_amd.RASTER_QUEUE,
+         "auto": piko_amd.RASTER_AUTO}
 
 
 def cover(mode):
     return piko_amd.COVER_WALK if mode == "queue-walk" else piko_amd.COVER_AUTO
+
+
+def part_size(mode):
+    return 64 if mode == "queue-parts" else 0
 
 
 def log(msg):
@@ -46,7 +53,7 @@ def test_batch_of_uneven_scenes_matches_plain_and_oracle(oracle_mod, mode):
     pos, attr, parts = piko_amd.concat_scenes(scs)
     log("batch of %d scenes, %d triangles, mode %s" % (len(scs), pos.shape[0], mode))
     with piko_amd.BatchPipe(pos, attr, "pretransformed", max_jobs=64, max_tris=1000, max_pixels=4096,
-                            raster_mode=MODES[mode], queue_cover=cover(mode)) as bp:
+                            raster_mode=MODES[mode], queue_cover=cover(mode), part_pairs=part_size(mode)) as bp:
         bp.submit([(n, base, T, piko_amd.ConstantState.for_scene(s)) for (n, base, s), T in zip(parts, Ts)])
         bp.wait()
         for j, (s, T) in enumerate(zip(scs, Ts)):
@@ -69,7 +76,7 @@ def test_batch_world_jobs_share_resident_geometry(oracle_mod, mode):
             ((-0.6, 0.8, 0.9), (0.1, -0.1, 0.0), 200, 200, 32)]
     log("world batch, mode %s" % mode)
     with piko_amd.BatchPipe(base.pos, base.nrm, "world", max_jobs=8, max_tris=base.n, max_pixels=1 << 16,
-                            raster_mode=MODES[mode], queue_cover=cover(mode)) as bp:
+                            raster_mode=MODES[mode], queue_cover=cover(mode), part_pairs=part_size(mode)) as bp:
         jobs, views = [], []
         for eye, tgt, W, H, T in cams:
             v = scenes.WorldScene(base.pos, base.nrm, W, H,
@@ -90,7 +97,7 @@ def test_batch_resubmit_different_sizes(mode):
     pos, attr, parts = piko_amd.concat_scenes(a)
     log("resubmit, mode %s" % mode)
     with piko_amd.BatchPipe(pos, attr, "pretransformed", max_jobs=8, max_tris=10, max_pixels=100,
-                            raster_mode=MODES[mode], queue_cover=cover(mode)) as bp:
+                            raster_mode=MODES[mode], queue_cover=cover(mode), part_pairs=part_size(mode)) as bp:
         for order in ([0, 1, 2], [2, 0], [1]):
             bp.submit([(parts[k][0], parts[k][1], 16, piko_amd.ConstantState.for_scene(a[k])) for k in order])
             for j, k in enumerate(order):

@@ -32,10 +32,12 @@ def test_config4_every_scene_matches_oracle_and_plain(oracle_mod):
     jobs = [(n, base, T, piko_amd.ConstantState.for_scene(s)) for (n, base, s), T in zip(parts, tiles)]
     pipes = {}
     try:
-        for name, mode, qc in (("tiled", piko_amd.RASTER_TILED, 0), ("queue", piko_amd.RASTER_QUEUE, 0),
-                               ("queue-walk", piko_amd.RASTER_QUEUE, piko_amd.COVER_WALK)):
+        # the queue path with its defaults (whole tiles), and walked with
+        # 2,048-entry parts (split tiles meet in the merge launch)
+        for name, mode, qc, pp in (("tiled", piko_amd.RASTER_TILED, 0, 0), ("queue", piko_amd.RASTER_QUEUE, 0, 0),
+                                   ("queue-walk", piko_amd.RASTER_QUEUE, piko_amd.COVER_WALK, 2048)):
             bp = piko_amd.BatchPipe(pos, attr, "pretransformed", max_jobs=len(scs), max_tris=ntris,
-                                    max_pixels=npix, raster_mode=mode, queue_cover=qc)
+                                    max_pixels=npix, raster_mode=mode, queue_cover=qc, part_pairs=pp)
             pipes[name] = bp
             bp.submit(jobs)
             bp.wait()
