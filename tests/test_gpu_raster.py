@@ -236,6 +236,25 @@ def test_queue_waves_per_cu(oracle_mod, wpc):
         assert np.array_equal(c, oc) and np.array_equal(d.view(np.int32), od.view(np.int32))
 
 
+@pytest.mark.parametrize("nframes", [2, 3])
+def test_queue_frames_in_flight_pretransformed(oracle_mod, nframes):
+    # run_frames keeps two frames in flight (the pipe and its twin); with an
+    # even count the last frame runs on the twin and its image is copied back
+    import piko_amd
+    sc = scenes.tris_uniform(21, 40000, 800, 600)
+    oc, od, _ = oracle_mod.frame(sc, 8, 32)
+    log("queue frames in flight: %d frames" % nframes)
+    with piko_amd.RasterPipe().allocate(sc, 32, 8, raster_mode=piko_amd.RASTER_QUEUE) as pipe:
+        pipe.prepare()
+        pipe.run_frames(nframes)
+        c, d = pipe.pikoScreen.getData(), pipe.get_depth()
+        assert np.array_equal(c, oc) and np.array_equal(d.view(np.int32), od.view(np.int32))
+        st = pipe.stats()
+        pipe.run_single()
+        assert pipe.stats()["emitted"] == st["emitted"]
+        assert np.array_equal(pipe.pikoScreen.getData(), oc)
+
+
 def test_tiled_large_lists(oracle_mod):
     # every triangle spans more than 2x2 tiles of 16 pixels: all of them go to
     # the chunks' large lists and are clipped to each tile they touch
